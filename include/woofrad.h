@@ -1,0 +1,140 @@
+/* woofrad.h -- C ABI of the MI355X RRTMG radiation library (libwoofrad.so).
+ *
+ * Plain pointers, sizes and integer status codes; no torch or HIP types in the signatures
+ * (a HIP stream is passed as `void *`).  All field arrays are COLUMN FASTEST: element
+ * (k, i) of a [nk][ncol] field is at x[k * ncol + i].  Vertical index 0 is the model top.
+ *
+ * Reference interfaces this ABI replaces (recastsystems/woof-nam):
+ *   - the radiation hook `rrtm_call(ctx, args: dict) -> None` that
+ *     namgpu/physics/driver/phys_rad_driver.py:255-262 calls on radiation steps (default
+ *     namgpu/physics/rrtm/rrtm_call.py:246 `run`).  The Python shim woof_nam_amd.rrtm_call.run keeps
+ *     that signature and calls wr_run_* below on the column form of the state;
+ *   - the chain under it: driver prep / grrad setup / gases / clouds
+ *     (phys_rrtm_driver_call.cu:83, phys_rrtm_grrad_setup.cu:102, phys_rrtm_grrad_gases.cu:56,
+ *     phys_rrtm_progcld2.cu:92), lwrad (phys_rrtm_lw_lwrad.cu:98,316), cldprop, setcoef,
+ *     taugb01..16, rtrnmc / rtrnmr, swrad (phys_rrtm_sw_swrad.cu:127,319), setcoef, taumol16..29,
+ *     spcvrtc, and the output copy phys_rrtm_driver_post.cu:71;
+ *   - phys_rad_cal_cldfra.cu:29 (wr_cal_cldfra) and the McICA generators
+ *     phys_rrtm_lw_mcica_subcol.cu:105 / phys_rrtm_sw_mcica_subcol.cu:129 (wr_mcica_masks).
+ */
+#ifndef WOOFRAD_H
+#define WOOFRAD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define WR_NBANDS_LW 16
+#define WR_NGPT_LW 140
+#define WR_NBANDS_SW 14
+#define WR_NGPT_SW 112
+
+/* status codes; the Python shim maps them onto the reference's named exceptions */
+enum {
+    WR_OK = 0,
+    WR_ERR_ARG = 1,        /* bad size / null pointer          -> ValueError          */
+    WR_ERR_TABLES = 2,     /* table file missing or malformed  -> FileNotFoundError   */
+    WR_ERR_HIP = 3,        /* HIP runtime error                -> RuntimeError        */
+    WR_ERR_NOMEM = 4,      /* device allocation failed         -> MemoryError         */
+    WR_ERR_UNSUPPORTED = 5 /* option not implemented           -> NotImplementedError */
+};
+
+/* Column inputs (model layers nlay, interfaces nlay+1; index 0 = model top). */
+typedef struct {
+    int32_t ncol;
+    int32_t nlay;            /* model layers; the radiation adds one layer above the model top */
+    const double *p_lev;     /* [nlay+1][ncol] interface pressure, Pa (increasing downward)    */
+    const double *t_lay;     /* [nlay][ncol]   temperature, K                                    */
+    const double *q_lay;     /* [nlay][ncol]   specific humidity, kg/kg                           */
+    const double *o3_lay;    /* [nlay][ncol]   ozone mass mixing ratio, kg/kg                     */
+    const double *cldf;      /* [nlay][ncol]   cloud fraction, 0..1                               */
+    const double *qc;        /* [nlay][ncol]   cloud liquid water mixing ratio, kg/kg             */
+    const double *qi;        /* [nlay][ncol]   cloud ice mixing ratio, kg/kg                      */
+    const double *t_sfc;     /* [ncol] skin temperature, K                                        */
+    const double *emis;      /* [ncol] surface LW emissivity (all 16 bands)                      */
+    const double *alb;       /* [ncol] surface SW albedo (beam/diffuse, visible/near-IR)          */
+    const double *cosz;      /* [ncol] cosine of the solar zenith angle (night when < 1e-4)      */
+    const uint64_t *col_id;  /* [ncol] global column index that keys the McICA stream; NULL = i  */
+} wr_columns;
+
+/* Call parameters (NAM values by default; see wr_default_params). */
+typedef struct {
+    int32_t mcica;           /* 1: McICA sub-columns (rtrnmc + masked SW two-stream)
+                                0: NAM route, max-random fractional cloud (rtrnmr + spcvrtc)     */
+    int32_t iovr;            /* cloud overlap of the McICA generator: 0 random, 1 max-random, 2 max */
+    uint64_t seed;           /* McICA stream key                                                  */
+    double solcon;           /* solar constant, W/m2                                              */
+    double re_liq;           /* cloud liquid effective radius, micron                             */
+    double re_ice;           /* cloud ice effective radius, micron                                */
+    double vmr[10];          /* co2, n2o, ch4, o2, co, cfc11, cfc12, cfc22, ccl4, cfc113 (mol/mol) */
+} wr_params;
+
+/* Outputs.  Levels are the radiation levels, nlay+2 of them, index 0 = top of the added layer
+ * (1e-6 hPa), index l >= 1 = model interface l-1.  Heating rates are on the model layers, K/s. */
+typedef struct {
+    double *hr_lw;           /* [nlay][ncol]   LW heating rate, K/s                             */
+    double *hr_sw;           /* [nlay][ncol]   SW heating rate, K/s (0 at night)                */
+    double *flw_up;          /* [nlay+2][ncol] LW upward flux, W/m2                              */
+    double *flw_dn;          /* [nlay+2][ncol] LW downward flux                                  */
+    double *fsw_up;          /* [nlay+2][ncol] SW upward flux (0 at night)                       */
+    double *fsw_dn;          /* [nlay+2][ncol] SW downward flux                                  */
+} wr_outputs;
+
+/* Optional diagnostics for bit-exact checks (any pointer may be NULL).  Radiation layers in the
+ * reference's internal SURFACE-FIRST order, nrl = nlay+1 of them. */
+typedef struct {
+    int32_t *lw_jp;          /* [nrl][ncol] */
+    int32_t *lw_jt;
+    int32_t *lw_jt1;
+    int32_t *lw_laytrop;     /* [ncol] */
+    int32_t *sw_jp;          /* [nrl][ncol], -1 on night columns */
+    int32_t *sw_jt;
+    int32_t *sw_jt1;
+    int32_t *sw_laytrop;     /* [ncol] */
+    uint32_t *lw_mask;       /* [nrl][5][ncol] McICA bits of the 140 LW g-points */
+    uint32_t *sw_mask;       /* [nrl][4][ncol] McICA bits of the 112 SW g-points (0 at night) */
+} wr_diag;
+
+typedef struct wr_ctx wr_ctx;
+
+/* Fill NAM default parameters: McICA on, max-random, solcon 1368.22, radii 10/50 micron,
+ * CO2 410, N2O 0.33, CH4 1.9 ppmv, O2 0.209 and the reference's default minor-gas VMRs
+ * (phys_rrtm_getgases.cu:56-65). */
+void wr_default_params(wr_params *p);
+
+/* Create a context on `device`: loads the table file once and keeps it device resident,
+ * allocates the work space for tiles of up to `max_cols` columns.  No later call allocates. */
+int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols);
+
+/* Plain FP64 radiation call on device pointers.  Columns beyond max_cols run in tiles. */
+int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
+                const wr_diag *diag, void *stream);
+
+/* Mixed precision: FP32 gas optics, Planck source and two-stream sweeps; FP64 for the SW
+ * reflectance/transmittance where w0 > 0.999999, for level-flux differencing into heating rates
+ * and for pressure/temperature index arithmetic.  Same inputs/outputs as wr_run_fp64. */
+int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
+                 const wr_diag *diag, void *stream);
+
+/* CAL_CLDFRA (phys_rad_cal_cldfra.cu:29-55): cldfra = 1 where qc(+qi) > 1e-6, else 0, on the
+ * interior points i in [i_lo, i_hi], j in [j_lo, j_hi] of [lm][jm][im] float32 fields; other
+ * points untouched.  f_qc / f_qi are the microphysics flags. */
+int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi, int32_t j_lo,
+                  int32_t j_hi, int32_t f_qc, int32_t f_qi, float *cldfra, const float *qc,
+                  const float *qi, void *stream);
+
+/* McICA sub-column masks alone (counter-based Philox4x32-10 stream keyed by seed and column):
+ * cldf [nrl][ncol] surface-first; lw_mask [nrl][5][ncol], sw_mask [nrl][4][ncol] (either may
+ * be NULL).  Bits equal those wr_run_* use. */
+int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t *col_id,
+                   uint64_t seed, int32_t iovr, uint32_t *lw_mask, uint32_t *sw_mask, void *stream);
+
+const char *wr_strerror(int code);
+void wr_destroy(wr_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* WOOFRAD_H */

@@ -1,0 +1,112 @@
+"""Python binding of the CPU FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and bench.py's CPU baseline
+as the checker.  The product (woof_nam_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+from woof_nam_amd import abi
+
+HERE = Path(__file__).resolve().parent
+LIB = HERE / "liboracle.so"
+TABLES = HERE.parent / "woof_nam_amd" / "data" / "rrtmg_tables.bin"
+
+_lib = None
+
+
+def build(force: bool = False) -> Path:
+    """Compile liboracle.so with its Makefile (gcc, -ffp-contract=off)."""
+    if force and LIB.exists():
+        LIB.unlink()
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return LIB
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not LIB.exists():
+            build()
+        L = C.CDLL(str(LIB))
+        L.oracle_init.argtypes = [C.c_char_p]
+        L.oracle_run.argtypes = [C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
+                                 C.POINTER(abi.WrDiag), C.c_int]
+        for f in ("oracle_exp", "oracle_log"):
+            getattr(L, f).argtypes = [C.c_double]
+            getattr(L, f).restype = C.c_double
+        L.oracle_pow.argtypes = [C.c_double, C.c_double]
+        L.oracle_pow.restype = C.c_double
+        rc = L.oracle_init(str(TABLES).encode())
+        if rc != 0:
+            raise RuntimeError(f"oracle: table load failed ({rc})")
+        _lib = L
+    return _lib
+
+
+def _ptr(a, ctype=C.c_double):
+    return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
+
+
+def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = None, diag: bool = False):
+    """Run the oracle on host float64 columns (gen.py layout).  Returns (outputs, diag|None)."""
+    L = lib()
+    params = params or abi.make_params()
+    t = np.ascontiguousarray(cols["t_lay"], dtype=np.float64)
+    nlay, ncol = t.shape
+    arrs = {k: np.ascontiguousarray(cols[k], dtype=np.float64) for k in abi.INPUT_FIELDS}
+    ids = cols.get("col_id")
+    ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint64)
+    ci = abi.WrColumns(ncol, nlay, *[_ptr(arrs[k]) for k in abi.INPUT_FIELDS],
+                       _ptr(ids, C.c_uint64) if ids is not None else None)
+    outs = {k: np.zeros(s) for k, s in abi.output_shapes(ncol, nlay).items()}
+    co = abi.WrOutputs(*[_ptr(outs[k]) for k in abi.OUTPUT_FIELDS])
+    dg = None
+    cd = None
+    if diag:
+        shp = abi.diag_shapes(ncol, nlay)
+        dg = {k: np.zeros(s, dtype=np.uint32 if k.endswith("mask") else np.int32) for k, s in shp.items()}
+        cd = abi.WrDiag(*[_ptr(dg[k], C.c_uint32 if k.endswith("mask") else C.c_int32) for k in abi.DIAG_FIELDS])
+    n = nthreads or int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    rc = L.oracle_run(C.byref(ci), C.byref(params), C.byref(co), C.byref(cd) if cd is not None else None, n)
+    if rc != 0:
+        raise RuntimeError(f"oracle_run failed ({rc})")
+    return outs, dg
+
+
+def lw_tables():
+    e, f, t = (np.zeros(10001) for _ in range(3))
+    lib().oracle_lw_tables(_ptr(e), _ptr(f), _ptr(t))
+    return e, f, t
+
+
+def sw_table():
+    e = np.zeros(10001)
+    lib().oracle_sw_table(_ptr(e))
+    return e
+
+
+def philox(ctr, key):
+    c = np.asarray(ctr, dtype=np.uint32)
+    k = np.asarray(key, dtype=np.uint32)
+    o = np.zeros(4, dtype=np.uint32)
+    lib().oracle_philox(_ptr(c, C.c_uint32), _ptr(k, C.c_uint32), _ptr(o, C.c_uint32))
+    return o
+
+
+def mcica(col: int, seed: int, kind: int, ngpt: int, iovr: int, cf):
+    cf = np.ascontiguousarray(cf, dtype=np.float64)
+    nl = cf.shape[0]
+    words = (ngpt + 31) // 32
+    bits = np.zeros((nl, words), dtype=np.uint32)
+    L = lib()
+    L.oracle_mcica.argtypes = [C.c_uint64, C.c_uint64, C.c_int, C.c_int, C.c_int, C.c_int,
+                               C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]
+    L.oracle_mcica(col, seed, kind, ngpt, nl, iovr, _ptr(cf), _ptr(bits, C.c_uint32), words)
+    return bits

@@ -1,0 +1,71 @@
+"""ctypes mirror of include/woofrad.h (structs, status codes, defaults).
+
+Only layouts and constants live here; the compute entry points are bound in ``_lib.py``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+NBANDS_LW, NGPT_LW, NBANDS_SW, NGPT_SW = 16, 140, 14, 112
+
+WR_OK, WR_ERR_ARG, WR_ERR_TABLES, WR_ERR_HIP, WR_ERR_NOMEM, WR_ERR_UNSUPPORTED = range(6)
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int32)
+_up = C.POINTER(C.c_uint32)
+
+
+class WrColumns(C.Structure):
+    _fields_ = [("ncol", C.c_int32), ("nlay", C.c_int32),
+                ("p_lev", _dp), ("t_lay", _dp), ("q_lay", _dp), ("o3_lay", _dp), ("cldf", _dp),
+                ("qc", _dp), ("qi", _dp), ("t_sfc", _dp), ("emis", _dp), ("alb", _dp), ("cosz", _dp),
+                ("col_id", C.POINTER(C.c_uint64))]
+
+
+class WrParams(C.Structure):
+    _fields_ = [("mcica", C.c_int32), ("iovr", C.c_int32), ("seed", C.c_uint64),
+                ("solcon", C.c_double), ("re_liq", C.c_double), ("re_ice", C.c_double),
+                ("vmr", C.c_double * 10)]
+
+
+class WrOutputs(C.Structure):
+    _fields_ = [("hr_lw", _dp), ("hr_sw", _dp), ("flw_up", _dp), ("flw_dn", _dp),
+                ("fsw_up", _dp), ("fsw_dn", _dp)]
+
+
+class WrDiag(C.Structure):
+    _fields_ = [("lw_jp", _ip), ("lw_jt", _ip), ("lw_jt1", _ip), ("lw_laytrop", _ip),
+                ("sw_jp", _ip), ("sw_jt", _ip), ("sw_jt1", _ip), ("sw_laytrop", _ip),
+                ("lw_mask", _up), ("sw_mask", _up)]
+
+
+INPUT_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi", "t_sfc", "emis", "alb", "cosz")
+OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn")
+DIAG_FIELDS = ("lw_jp", "lw_jt", "lw_jt1", "lw_laytrop", "sw_jp", "sw_jt", "sw_jt1", "sw_laytrop",
+               "lw_mask", "sw_mask")
+
+# NAM defaults (phys_rrtm_getgases.cu:56-65 minor-gas VMRs; CO2/N2O/CH4 of the benchmark workload)
+DEFAULT_VMR = (410.0e-6, 0.33e-6, 1.9e-6, 0.209, 1.50e-8, 3.520e-10, 6.358e-10, 1.500e-10,
+               1.397e-10, 8.2000e-11)
+
+
+def make_params(mcica: int = 1, iovr: int = 1, seed: int = 0, solcon: float = 1368.22,
+                re_liq: float = 10.0, re_ice: float = 50.0, vmr=DEFAULT_VMR) -> WrParams:
+    p = WrParams()
+    p.mcica, p.iovr, p.seed = int(mcica), int(iovr), int(seed)
+    p.solcon, p.re_liq, p.re_ice = float(solcon), float(re_liq), float(re_ice)
+    for i, v in enumerate(vmr):
+        p.vmr[i] = float(v)
+    return p
+
+
+def output_shapes(ncol: int, nlay: int) -> dict:
+    return {"hr_lw": (nlay, ncol), "hr_sw": (nlay, ncol), "flw_up": (nlay + 2, ncol),
+            "flw_dn": (nlay + 2, ncol), "fsw_up": (nlay + 2, ncol), "fsw_dn": (nlay + 2, ncol)}
+
+
+def diag_shapes(ncol: int, nlay: int) -> dict:
+    nrl = nlay + 1
+    return {"lw_jp": (nrl, ncol), "lw_jt": (nrl, ncol), "lw_jt1": (nrl, ncol), "lw_laytrop": (ncol,),
+            "sw_jp": (nrl, ncol), "sw_jt": (nrl, ncol), "sw_jt1": (nrl, ncol), "sw_laytrop": (ncol,),
+            "lw_mask": (nrl, 5, ncol), "sw_mask": (nrl, 4, ncol)}
