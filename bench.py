@@ -1,0 +1,164 @@
+"""Radiation benchmark: LW+SW RRTMG columns/s on the 954 x 835 x 60 NAM 12 km grid on one MI355X.
+
+One step = one radiation call (LW + SW, McICA) over all 796,590 columns with the inputs already
+resident in HBM.  Prints ONE JSON line (rank 0).  Also reported beside the throughput:
+  * latency_ms: per-call p50/p99 from HIP events on the call's stream;
+  * accuracy: heating-rate (K/day) and flux (W/m2) RMS against the CPU FP64 oracle on a sample;
+  * roofline: the dominant kernel's algorithmic HBM bytes / its mean launch duration (HIP events
+    on the launching stream, recorded by the library around every launch);
+  * cpu_baseline: the CPU FP64 oracle on a bounded sample of the same grid, on this host's cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+METRIC = "LW+SW radiation columns/s (60 lev) + p99 latency, at stated heating-rate error"
+HBM_PEAK_GBS = 8000.0
+KDAY = 86400.0
+
+# Algorithmic HBM bytes per column of each kernel (FP64 inputs it must read, outputs it must write).
+# Column API inputs: 6 layer fields x 60 + p_lev 61 + 4 surface scalars; outputs: 2 HR x 60 + 4 x 62 levels.
+NL = 60
+IN_LW = (6 * NL + (NL + 1) + 2) * 8          # T, q, O3, cldf, qc, qi, p_lev, T_s, emissivity
+IN_SW = (6 * NL + (NL + 1) + 3) * 8          # + albedo, cos z (day columns only)
+OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
+
+
+def kernel_bytes(name: str, ncol: int, nday: int) -> float:
+    if name.startswith("k_lw"):
+        return IN_LW * ncol
+    if name.startswith("k_sw"):
+        return IN_SW * nday
+    if name == "k_post":
+        return (NL + 1) * 8 * ncol + OUT_ALL * ncol
+    if name == "k_colprep":
+        return (6 * NL + NL + 1 + 1) * 8 * ncol
+    return 0.0
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--max-cols", type=int, default=262144)
+    ap.add_argument("--nx", type=int, default=954)
+    ap.add_argument("--ny", type=int, default=835)
+    ap.add_argument("--cpu-cols", type=int, default=0, help="CPU baseline sample (0 = sized from cores)")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    a = ap.parse_args(argv)
+    if a.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
+              "the radiation columns are independent and would shard by contiguous column ranges, "
+              "but no multi-GPU run is part of it", file=sys.stderr)
+        return 2
+
+    import torch
+
+    from woof_nam_amd import gen
+    from woof_nam_amd.radiation import Radiation
+
+    t0 = time.time()
+    cols = gen.grid(a.nx, a.ny, seed=0)
+    ncol = cols["t_lay"].shape[1]
+    nday = int(np.count_nonzero(cols["cosz"] >= 1.0e-4))
+    print(f"# grid {a.nx}x{a.ny}x{NL}: {ncol} columns, {nday} day, generated in {time.time() - t0:.1f} s",
+          file=sys.stderr, flush=True)
+    dev = torch.device("cuda", 0)
+    d = {k: torch.from_numpy(v).to(dev) for k, v in cols.items()}
+    rad = Radiation(0, max_cols=a.max_cols)
+    out, _ = rad.run(d, precision=a.precision)
+    torch.cuda.synchronize()
+    for _ in range(a.warmup):
+        rad.run(d, precision=a.precision, out=out)
+    torch.cuda.synchronize()
+    print("# warmup done", file=sys.stderr, flush=True)
+
+    stream = torch.cuda.current_stream(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(a.steps)]
+    rad.timing(True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    for s in range(a.steps):
+        evs[s][0].record(stream)
+        rad.run(d, precision=a.precision, out=out)
+        evs[s][1].record(stream)
+    torch.cuda.synchronize()
+    t_el = time.perf_counter() - t_start
+    launches = rad.timing_read()
+    rad.timing(False)
+    lat = np.array([b.elapsed_time(e) for b, e in ((x, y) for x, y in evs)])
+    ms_per_step = 1e3 * t_el / a.steps
+    value = ncol * a.steps / t_el
+    print(f"# {a.steps} steps: {ms_per_step:.1f} ms/step, {value:.4g} columns/s", file=sys.stderr, flush=True)
+
+    # dominant kernel and its roofline
+    tot, cnt = {}, {}
+    for name, ms in launches:
+        tot[name] = tot.get(name, 0.0) + ms
+        cnt[name] = cnt.get(name, 0) + 1
+    dom = max(tot, key=tot.get)
+    k_ms = tot[dom] / cnt[dom]
+    bytes_per_launch = kernel_bytes(dom, ncol, nday) * a.steps / cnt[dom]
+    achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    shares = sorted(((v / sum(tot.values()), k) for k, v in tot.items()), reverse=True)[:6]
+    print("# kernel time shares: " + ", ".join(f"{k} {100 * f:.1f}%" for f, k in shares), file=sys.stderr, flush=True)
+
+    res = {
+        "metric": METRIC, "value": value, "unit": "columns/s", "n_gpus": 1, "steps": a.steps,
+        "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "data": "synthetic (seeded generator: NAM sigma levels, ISA T + 300 km GRF, RH GRF, clouds, 12 UTC sun)",
+        "config": {"workload": f"NAM 12 km grid {a.nx}x{a.ny} = {ncol} columns x {NL} layers, LW+SW RRTMG, "
+                               f"McICA 140/112 g, {a.precision}",
+                   "columns": ncol, "day_columns": nday, "layers": NL, "precision": a.precision,
+                   "tile_columns": a.max_cols},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "per_call": "HIP events around each call on its stream"},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "launch_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
+    }
+
+    if not a.no_cpu:
+        from oracle import pyoracle
+        threads = min(16, os.cpu_count() or 1)
+        ns = a.cpu_cols or min(ncol, 12000 * threads)
+        sub = {k: np.ascontiguousarray(v[..., :ns]) for k, v in cols.items()}
+        pyoracle.run({k: v[..., :8] for k, v in sub.items()}, nthreads=1)
+        t1 = time.perf_counter()
+        ref, _ = pyoracle.run(sub, nthreads=threads)
+        tc = time.perf_counter() - t1
+        res["cpu_baseline"] = {"value": ns / tc, "unit": "columns/s", "cores": threads, "kind": "port",
+                               "sample": f"first {ns} columns of the same grid, CPU FP64 oracle (C, OpenMP), "
+                                         f"{tc:.1f} s"}
+        acc = {}
+        for k in ("hr_lw", "hr_sw"):
+            diff = out[k][:, :ns].cpu().numpy() - ref[k]
+            acc[k + "_rms_kday"] = float(np.sqrt(np.mean(diff ** 2)) * KDAY)
+        for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+            diff = out[k][:, :ns].cpu().numpy() - ref[k]
+            acc[k + "_rms_wm2"] = float(np.sqrt(np.mean(diff ** 2)))
+        acc["vs"] = f"CPU FP64 oracle on the first {ns} columns"
+        res["accuracy"] = acc
+    else:
+        res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+    rad.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

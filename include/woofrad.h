@@ -131,6 +131,15 @@ int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi
 int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t *col_id,
                    uint64_t seed, int32_t iovr, uint32_t *lw_mask, uint32_t *sw_mask, void *stream);
 
+/* Per-launch kernel timing for measurement (no counterpart in the reference: the reference
+ * publishes call-level times only).  wr_timing(ctx, 1) creates an event pool once and brackets
+ * every later kernel launch of the context with HIP events on the launching stream (up to 8192
+ * launches; reset on every wr_timing call).  wr_timing_read waits for the recorded launches and
+ * returns kernel id and duration (ms) of each, then clears the record; wr_kernel_name names an id. */
+int wr_timing(wr_ctx *ctx, int32_t enable);
+int wr_timing_read(wr_ctx *ctx, int32_t *kernel_id, float *ms, int32_t cap, int32_t *n);
+const char *wr_kernel_name(int32_t kernel_id);
+
 const char *wr_strerror(int code);
 void wr_destroy(wr_ctx *ctx);
 

@@ -110,3 +110,13 @@ def mcica(col: int, seed: int, kind: int, ngpt: int, iovr: int, cf):
                                C.POINTER(C.c_double), C.POINTER(C.c_uint32), C.c_int]
     L.oracle_mcica(col, seed, kind, ngpt, nl, iovr, _ptr(cf), _ptr(bits, C.c_uint32), words)
     return bits
+
+
+def cal_cldfra(cldfra, qc, qi, i_lo, i_hi, j_lo, j_hi, f_qc=True, f_qi=True):
+    """CAL_CLDFRA on host float32 [lm][jm][im] arrays, in place."""
+    lm, jm, im = cldfra.shape
+    L = lib()
+    fp = C.POINTER(C.c_float)
+    L.oracle_cal_cldfra.argtypes = [C.c_int] * 9 + [fp, fp, fp]
+    L.oracle_cal_cldfra(im, jm, lm, i_lo, i_hi, j_lo, j_hi, int(f_qc), int(f_qi), _ptr(cldfra, C.c_float),
+                        _ptr(qc, C.c_float), _ptr(qi, C.c_float))

@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/woofrad.h declares (no GPU calls)."""
+import ctypes as C
+import re
+from pathlib import Path
+
+from woof_nam_amd import _lib, abi
+
+HEADER = Path(__file__).resolve().parents[1] / "include" / "woofrad.h"
+
+
+def declared():
+    txt = re.sub(r"/\*.*?\*/", "", HEADER.read_text(), flags=re.S)
+    return sorted(set(re.findall(r"\b(wr_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_every_declared_symbol_is_exported():
+    L = _lib.load()
+    names = declared()
+    assert len(names) >= 11
+    assert set(names) == set(_lib.EXPORTS)
+    for n in names:
+        assert hasattr(L, n), n
+
+
+def test_struct_layouts_match_header():
+    # x86-64 SysV layout of the header structs
+    assert C.sizeof(abi.WrColumns) == 8 + 12 * 8
+    assert C.sizeof(abi.WrParams) == 4 + 4 + 8 + 3 * 8 + 10 * 8
+    assert C.sizeof(abi.WrOutputs) == 6 * 8
+    assert C.sizeof(abi.WrDiag) == 10 * 8
+
+
+def test_default_params_and_strerror():
+    L = _lib.load()
+    p = abi.WrParams()
+    L.wr_default_params(C.byref(p))
+    assert (p.mcica, p.iovr, p.seed) == (1, 1, 0)
+    assert p.solcon == 1368.22 and p.re_liq == 10.0 and p.re_ice == 50.0
+    assert tuple(p.vmr) == abi.DEFAULT_VMR
+    assert L.wr_strerror(abi.WR_ERR_UNSUPPORTED).decode().startswith("option")
+    assert L.wr_kernel_name(3).decode() == "k_lw_b01"
+    assert L.wr_kernel_name(32).decode() == "k_sw_b29"
+
+
+def test_null_arguments_rejected_without_gpu():
+    L = _lib.load()
+    assert L.wr_run_fp64(None, None, None, None, None, None) == abi.WR_ERR_ARG
+    assert L.wr_timing(None, 1) == abi.WR_ERR_ARG
+    assert L.wr_mcica_masks(-1, 61, None, None, 0, 1, None, None, None) == abi.WR_ERR_ARG

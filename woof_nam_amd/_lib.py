@@ -1,0 +1,72 @@
+"""ctypes binding of libwoofrad.so (the gfx950 HIP library declared in include/woofrad.h).
+
+The library is built in-tree by ``python -m woof_nam_amd.build``.  There is no CPU fallback:
+loading fails with an ImportError naming the missing file.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+from . import abi
+
+LIB_PATH = Path(__file__).resolve().parent / "libwoofrad.so"
+TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
+
+# symbols include/woofrad.h declares (the CPU test suite checks every one is exported)
+EXPORTS = ("wr_default_params", "wr_create", "wr_run_fp64", "wr_run_mixed", "wr_cal_cldfra",
+           "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy")
+
+_ERRORS = {
+    abi.WR_ERR_ARG: ValueError,
+    abi.WR_ERR_TABLES: FileNotFoundError,
+    abi.WR_ERR_HIP: RuntimeError,
+    abi.WR_ERR_NOMEM: MemoryError,
+    abi.WR_ERR_UNSUPPORTED: NotImplementedError,
+}
+
+_lib = None
+
+
+def load():
+    """Load libwoofrad.so once and declare the argument types of every entry point."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not LIB_PATH.exists():
+        raise ImportError(f"{LIB_PATH} is not built (python -m woof_nam_amd.build); "
+                          "the radiation path has no CPU fallback")
+    L = C.CDLL(str(LIB_PATH))
+    vp = C.c_void_p
+    L.wr_default_params.argtypes = [C.POINTER(abi.WrParams)]
+    L.wr_default_params.restype = None
+    L.wr_create.argtypes = [C.POINTER(vp), C.c_char_p, C.c_int, C.c_int32]
+    L.wr_create.restype = C.c_int
+    for f in ("wr_run_fp64", "wr_run_mixed"):
+        getattr(L, f).argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams),
+                                  C.POINTER(abi.WrOutputs), C.POINTER(abi.WrDiag), vp]
+        getattr(L, f).restype = C.c_int
+    L.wr_cal_cldfra.argtypes = [C.c_int32] * 9 + [vp, vp, vp, vp]
+    L.wr_cal_cldfra.restype = C.c_int
+    L.wr_mcica_masks.argtypes = [C.c_int32, C.c_int32, vp, vp, C.c_uint64, C.c_int32, vp, vp, vp]
+    L.wr_mcica_masks.restype = C.c_int
+    L.wr_timing.argtypes = [vp, C.c_int32]
+    L.wr_timing.restype = C.c_int
+    L.wr_timing_read.argtypes = [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int32, C.POINTER(C.c_int32)]
+    L.wr_timing_read.restype = C.c_int
+    L.wr_kernel_name.argtypes = [C.c_int32]
+    L.wr_kernel_name.restype = C.c_char_p
+    L.wr_strerror.argtypes = [C.c_int]
+    L.wr_strerror.restype = C.c_char_p
+    L.wr_destroy.argtypes = [vp]
+    L.wr_destroy.restype = None
+    _lib = L
+    return L
+
+
+def check(rc: int, what: str) -> None:
+    """Raise the named exception a status code stands for."""
+    if rc == abi.WR_OK:
+        return
+    msg = load().wr_strerror(rc).decode()
+    raise _ERRORS.get(rc, RuntimeError)(f"{what}: {msg} (code {rc})")

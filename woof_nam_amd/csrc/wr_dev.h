@@ -1,0 +1,183 @@
+// wr_dev.h -- device data structures and per-layer physics of the MI355X RRTMG kernels.
+//
+// Layout in HBM: every input/output field is column fastest ([level][column]); the kernels map
+// one thread to one column, so each wavefront load of a field is one contiguous 512-byte (FP64)
+// segment.  Everything per layer (driver prep, setcoef, Planck source, cloud optics) is rebuilt
+// on the fly inside the band kernels from the raw column inputs: it costs a few hundred VALU
+// operations per layer, far less than re-reading a materialised per-layer record for every band.
+//
+// Physics follows the reference CUDA kernels it cites (recastsystems/woof-nam namgpu/kernels/),
+// evaluated in the same order of operations; the layer index k is 0-based SURFACE-FIRST (the
+// reference's lwrad/swrad internal order), kin = nrl-1-k the top-first radiation layer.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "wr_math.h"
+
+namespace wr {
+
+constexpr int NL = 66;  // max radiation layers
+constexpr int NTBL = 10000;
+
+// physcons (phys_rrtm_lw_lwrad.cu:55-62, phys_rrtm_sw_swrad.cu:22-34)
+constexpr double CON_G = 9.80665;
+constexpr double CON_AVGD = 6.0221415e23;
+constexpr double CON_AMD = 28.9644;
+constexpr double CON_AMW = 18.0154;
+constexpr double CON_AMO3 = 47.9982;
+constexpr double CON_CP = 1.0046e3;
+constexpr double AMDW = CON_AMD / CON_AMW;
+constexpr double AMDO3 = CON_AMD / CON_AMO3;
+constexpr double EPS = 1.0e-6;
+constexpr double ONEMINUS = 1.0 - 1.0e-6;
+constexpr double FTINY = 1.0e-12;
+constexpr double STPFAC = 296.0 / 1013.0;
+constexpr double HEATFAC = CON_G * 1.0e-2 / CON_CP;  // K/s per (W/m2 / hPa)
+constexpr double FLUXFAC = 3.141592653589793 * 2.0e4;
+constexpr double S0 = 1368.22;
+
+__device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
+__device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
+
+// ---- tables --------------------------------------------------------------------------------
+template <typename R> struct LwBand {
+    const R *absa, *absb, *selfref, *forref, *fracrefa, *fracrefb;
+    const R *ka_m[3], *kb_m[3];
+    const R *x1, *x2;
+    R refrat_planck_a, refrat_planck_b, refrat_m_a, refrat_m_b, refrat_m_a3;
+};
+template <typename R> struct SwBand {
+    const R *absa, *absb, *selfref, *forref;
+    const R *rayl_v, *rayla, *raylb, *abso3a, *abso3b, *absch4, *absh2o, *absco2;
+    R rayl, strrat;
+};
+template <typename R> struct Tabs {
+    // index arithmetic stays FP64 in every precision
+    const double *preflog, *tref, *chi_mls, *sw_preflog, *sw_tref;
+    const R *totplnk, *delwave, *absliq1, *absice1;
+    const int32_t *ipat, *layreffr, *ix1, *ix2, *ibx;
+    const R *exp_tbl, *tfn_tbl, *tau_tbl, *sw_exp_tbl;
+    const R *sf01, *sf02, *sf03, *sw_strrat, *sw_specwt;
+    const R *extliq1, *ssaliq1, *asyliq1, *extice3, *ssaice3, *asyice3;
+    R scalekur;
+    LwBand<R> lw[16];
+    SwBand<R> sw[14];
+};
+
+// ---- column inputs ---------------------------------------------------------------------------
+struct ColIn {
+    const double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
+    const uint64_t *col_id;
+    int ncol;  // stride of the [k][column] fields
+    int nlay;  // model layers
+};
+
+struct RunPar {
+    int mcica, iovr;
+    uint64_t seed;
+    double solcon, re_liq, re_ice;
+    double vmr[10];
+};
+
+// Radiation-grid accessors of one column (grrad_setup phys_rrtm_grrad_setup.cu:102,
+// grrad_gases .cu:56): layer kin top-first, kin = 0 is the layer added above the model top.
+struct RadCol {
+    const ColIn *in;
+    int i;    // column index into the fields
+    int nrl;  // radiation layers = model layers + 1
+    __device__ __forceinline__ double fld(const double *a, int k) const { return a[(size_t)k * in->ncol + i]; }
+    __device__ __forceinline__ double plvl(int l) const
+    {
+        if (l == 0) return 1.0e-6;
+        const double v = 10.0 * (fld(in->p_lev, l - 1) * 1.0e-3);
+        return (l == 1 && v <= 1.0e-6) ? 2.0e-6 : v;
+    }
+    __device__ __forceinline__ double plyr(int l) const
+    {
+        if (l == 0) return 0.5 * plvl(1);
+        const double prsl = 0.5 * (fld(in->p_lev, l - 1) + fld(in->p_lev, l));
+        return 10.0 * (prsl * 1.0e-3);
+    }
+    __device__ __forceinline__ double tlyr(int l) const { return fld(in->t_lay, l == 0 ? 0 : l - 1); }
+    __device__ __forceinline__ double qlyr(int l) const { return fmax(1.0e-7, fld(in->q_lay, l == 0 ? 0 : l - 1)); }
+    __device__ __forceinline__ double olyr(int l) const { return fmax(1.0e-10, fld(in->o3_lay, l == 0 ? 0 : l - 1)); }
+    __device__ __forceinline__ double tskn() const { return in->t_sfc[i]; }
+    // level temperature by log-pressure interpolation; la(l) = log(plyr(l)), lb(l) = log(plvl(l))
+    __device__ __forceinline__ double tlvl(int l, double la_l, double la_lm1, double lb_l) const
+    {
+        if (l == 0) return tlyr(0);
+        if (l == nrl) return tskn();
+        const double t = tlyr(l), tm = tlyr(l - 1);
+        return t + (tm - t) * (lb_l - la_l) / (la_lm1 - la_l);
+    }
+    // cloud fraction alone (progcld2: ct < 0.001 -> 0); the added top layer is clear
+    __device__ __forceinline__ double cfrac(int l) const
+    {
+        if (l == 0) return 0.0;
+        const double ct = fld(in->cldf, l - 1);
+        return (ct < 0.001) ? 0.0 : ct;
+    }
+    // progcld2 with the cloud fraction given (phys_rrtm_progcld2.cu:92-255): (frac, cwp, cip)
+    __device__ __forceinline__ void cloud(int l, double &ct, double &cwp, double &cip) const
+    {
+        ct = 0.0, cwp = 0.0, cip = 0.0;
+        if (l == 0) return;
+        const int k0 = l - 1;
+        ct = fld(in->cldf, k0);
+        const double qc = fld(in->qc, k0), qi = fld(in->qi, k0);
+        double qcwat, qcice;
+        if (tlyr(l) - 273.15 > -40.0) {
+            qcwat = qc;
+            qcice = qi;
+        } else {
+            qcwat = 0.0;
+            qcice = qc + qi;
+        }
+        if (qcwat + qcice > 1.0e-12) {
+            const double cpath = fabs(plvl(l + 1) - plvl(l)) * (100000.0 / CON_G);
+            if (qcwat > 0.0) cwp = cpath * qcwat;
+            cip = cpath * qcice;
+        }
+        if (ct < 0.001) {
+            ct = 0.0;
+            cwp = 0.0;
+            cip = 0.0;
+        }
+    }
+};
+
+// ---- McICA sub-columns: phys_rrtm_lw_mcica_subcol.cu:105-168 draw order and max-random rule,
+// Philox4x32-10 stream keyed (seed, column), counter (column, g | kind << 16, k / 4) ------------
+template <typename CF> __device__ __forceinline__ uint64_t mcica_g(uint64_t col, uint64_t seed, int kind, int g,
+                                                                   int nrl, int iovr, CF cfk)
+{
+    uint64_t bits = 0;
+    double prev = 0.0, cfm1 = 0.0;
+    uint32_t out[4] = {0, 0, 0, 0};
+    for (int k = 0; k < nrl; ++k) {
+        if ((k & 3) == 0 && (iovr != 2 || k == 0))
+            philox4x32_10((uint32_t)col, (uint32_t)(col >> 32), (uint32_t)g | ((uint32_t)kind << 16), (uint32_t)(k >> 2),
+                          (uint32_t)seed, (uint32_t)(seed >> 32), out);
+        const uint32_t u = out[iovr == 2 ? 0 : (k & 3)];
+        const double r = (double)u * 0x1p-32;
+        const double cf = cfk(k);
+        double cd;
+        if (iovr == 1) {
+            if (k == 0)
+                cd = r;
+            else {
+                const double tem1 = 1.0 - cfm1;
+                cd = (prev > tem1) ? prev : r * tem1;
+            }
+            prev = cd;
+        } else {
+            cd = r;
+        }
+        if (cd >= 1.0 - cf) bits |= (uint64_t)1 << k;
+        cfm1 = cf;
+    }
+    return bits;
+}
+
+}  // namespace wr

@@ -1,0 +1,662 @@
+// wr_host.hip -- C ABI of libwoofrad: table upload, context, tile loop, diagnostics and the small
+// stand-alone kernels (CAL_CLDFRA, McICA masks).
+//
+// The table file (rrtmg_tables.bin, "WRTBL001") is read once per context and laid out in two
+// device blobs, one per optics precision (FP64 and FP32).  Index-arithmetic tables (reference
+// pressures/temperatures, chi_mls) stay FP64 in both.  The exp/transmittance look-up tables are
+// generated on the device by the same IEEE operation sequence on every platform.
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "wr_internal.h"
+
+using namespace wr;
+
+namespace {
+
+// ---- table file -------------------------------------------------------------------------------
+struct TEntry {
+    uint32_t dtype;  // 0 = float32, 1 = int32
+    uint64_t count;
+    const uint8_t *data;
+};
+struct TFile {
+    std::vector<uint8_t> buf;
+    std::unordered_map<std::string, TEntry> e;
+};
+
+int read_tables(const char *path, TFile &f)
+{
+    FILE *fp = fopen(path, "rb");
+    if (!fp) return WR_ERR_TABLES;
+    fseek(fp, 0, SEEK_END);
+    const long sz = ftell(fp);
+    fseek(fp, 0, SEEK_SET);
+    if (sz < 12) {
+        fclose(fp);
+        return WR_ERR_TABLES;
+    }
+    f.buf.resize((size_t)sz);
+    const size_t got = fread(f.buf.data(), 1, (size_t)sz, fp);
+    fclose(fp);
+    if (got != (size_t)sz || memcmp(f.buf.data(), "WRTBL001", 8) != 0) return WR_ERR_TABLES;
+    uint32_t n;
+    memcpy(&n, f.buf.data() + 8, 4);
+    const size_t rec = 48 + 4 + 4 + 16 + 8 + 8;
+    if (12 + (size_t)n * rec > (size_t)sz) return WR_ERR_TABLES;
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint8_t *p = f.buf.data() + 12 + (size_t)k * rec;
+        char name[49];
+        memcpy(name, p, 48);
+        name[48] = 0;
+        TEntry t;
+        uint64_t off;
+        memcpy(&t.dtype, p + 48, 4);
+        memcpy(&off, p + 72, 8);
+        memcpy(&t.count, p + 80, 8);
+        if (t.dtype > 1 || off + t.count * 4 > (uint64_t)sz) return WR_ERR_TABLES;
+        t.data = f.buf.data() + off;
+        f.e[std::string(name)] = t;
+    }
+    return WR_OK;
+}
+
+// Device blob builder.  bind() runs twice: once with base 0 to size the blob, once with the real
+// device base; both passes append the same bytes in the same order, so offsets agree.
+struct Blob {
+    std::vector<uint8_t> bytes;
+    uintptr_t base = 0;
+    bool bad = false;
+    size_t align()
+    {
+        size_t off = (bytes.size() + 255) & ~(size_t)255;
+        bytes.resize(off);
+        return off;
+    }
+    template <typename T> const T *conv(const TEntry &e)
+    {
+        const size_t off = align();
+        bytes.resize(off + e.count * sizeof(T));
+        T *dst = (T *)(bytes.data() + off);
+        if (e.dtype == 0) {
+            const float *s = (const float *)e.data;
+            for (uint64_t j = 0; j < e.count; ++j) dst[j] = (T)s[j];
+        } else {
+            const int32_t *s = (const int32_t *)e.data;
+            for (uint64_t j = 0; j < e.count; ++j) dst[j] = (T)s[j];
+        }
+        return (const T *)(base + off);
+    }
+    template <typename T> T *reserve(size_t n)
+    {
+        const size_t off = align();
+        bytes.resize(off + n * sizeof(T), 0);
+        return (T *)(base + off);
+    }
+};
+
+struct Binder {
+    const TFile &f;
+    Blob &b;
+    const TEntry *find(const std::string &nm) const
+    {
+        auto it = f.e.find(nm);
+        return it == f.e.end() ? nullptr : &it->second;
+    }
+    template <typename T> const T *need(const std::string &nm)
+    {
+        const TEntry *e = find(nm);
+        if (!e) {
+            b.bad = true;
+            return nullptr;
+        }
+        return b.conv<T>(*e);
+    }
+    template <typename T> const T *opt(const std::string &nm)
+    {
+        const TEntry *e = find(nm);
+        return e ? b.conv<T>(*e) : nullptr;
+    }
+    double scalar(const std::string &nm, bool *ok)
+    {
+        const TEntry *e = find(nm);
+        if (!e || e->count < 1 || e->dtype != 0) {
+            *ok = false;
+            return 0.0;
+        }
+        return (double)((const float *)e->data)[0];
+    }
+};
+
+std::string bname(const char *pre, int b, const char *key, bool two)
+{
+    char nm[64];
+    if (two)
+        snprintf(nm, sizeof nm, "%s/b%02d/%s", pre, b, key);
+    else
+        snprintf(nm, sizeof nm, "%s/b%d/%s", pre, b, key);
+    return nm;
+}
+
+// Host image of the table set in precision R; device pointers are base + offset.
+template <typename R> int bind_tables(const TFile &f, Blob &b, Tabs<R> &T)
+{
+    memset(&T, 0, sizeof(T));
+    Binder d{f, b};
+    T.preflog = d.need<double>("lw/preflog");
+    T.tref = d.need<double>("lw/tref");
+    T.chi_mls = d.need<double>("lw/chi_mls");
+    T.sw_preflog = d.need<double>("sw/preflog");
+    T.sw_tref = d.need<double>("sw/tref");
+    T.totplnk = d.need<R>("lw/totplnk");
+    T.delwave = d.need<R>("lw/delwave");
+    T.absliq1 = d.need<R>("lw/absliq1");
+    T.absice1 = d.need<R>("lw/absice1");
+    T.ipat = d.need<int32_t>("lw/ipat");
+    T.layreffr = d.need<int32_t>("sw/layreffr");
+    T.ix1 = d.need<int32_t>("sw/ix1");
+    T.ix2 = d.need<int32_t>("sw/ix2");
+    T.ibx = d.need<int32_t>("sw/ibx");
+    T.sf01 = d.need<R>("sw/sfluxref01");
+    T.sf02 = d.need<R>("sw/sfluxref02");
+    T.sf03 = d.need<R>("sw/sfluxref03");
+    T.sw_strrat = d.need<R>("sw/strrat");
+    T.sw_specwt = d.need<R>("sw/specwt");
+    T.extliq1 = d.need<R>("sw/extliq1");
+    T.ssaliq1 = d.need<R>("sw/ssaliq1");
+    T.asyliq1 = d.need<R>("sw/asyliq1");
+    T.extice3 = d.need<R>("sw/extice3");
+    T.ssaice3 = d.need<R>("sw/ssaice3");
+    T.asyice3 = d.need<R>("sw/asyice3");
+    bool ok = true;
+    T.scalekur = (R)d.scalar("sw/scalekur", &ok);
+    if (!ok) return WR_ERR_TABLES;
+    T.exp_tbl = b.reserve<R>(NTBL + 1);
+    T.tfn_tbl = b.reserve<R>(NTBL + 1);
+    T.tau_tbl = b.reserve<R>(NTBL + 1);
+    T.sw_exp_tbl = b.reserve<R>(NTBL + 1);
+
+    for (int k = 0; k < 16; ++k) {
+        LwBand<R> &B = T.lw[k];
+        const int bn = k + 1;
+        B.absa = d.need<R>(bname("lw", bn, "absa", true));
+        B.absb = (k != 5 && k != 11 && k != 12 && k != 14) ? d.need<R>(bname("lw", bn, "absb", true))
+                                                             : d.opt<R>(bname("lw", bn, "absb", true));
+        B.selfref = d.need<R>(bname("lw", bn, "selfref", true));
+        B.forref = d.need<R>(bname("lw", bn, "forref", true));
+        B.fracrefa = d.need<R>(bname("lw", bn, "fracrefa", true));
+        B.fracrefb = (k != 5 && k != 11 && k != 14) ? d.need<R>(bname("lw", bn, "fracrefb", true))
+                                                    : d.opt<R>(bname("lw", bn, "fracrefb", true));
+    }
+    // minor-gas and cross-section slots in the order the band code reads them
+    struct Slot {
+        int band;
+        int kind;  // 0 ka_m, 1 kb_m, 2 x1, 3 x2
+        int idx;
+        const char *key;
+    };
+    static const Slot slots[] = {
+        {1, 0, 0, "ka_mn2"},  {1, 1, 0, "kb_mn2"},  {3, 0, 0, "ka_mn2o"}, {3, 1, 0, "kb_mn2o"}, {5, 0, 0, "ka_mo3"},
+        {5, 2, 0, "ccl4"},    {6, 0, 0, "ka_mco2"}, {6, 2, 0, "cfc11adj"}, {6, 3, 0, "cfc12"},  {7, 0, 0, "ka_mco2"},
+        {7, 1, 0, "kb_mco2"}, {8, 0, 0, "ka_mco2"}, {8, 1, 0, "kb_mco2"}, {8, 0, 1, "ka_mo3"},  {8, 0, 2, "ka_mn2o"},
+        {8, 1, 2, "kb_mn2o"}, {8, 2, 0, "cfc12"},   {8, 3, 0, "cfc22adj"}, {9, 0, 0, "ka_mn2o"}, {9, 1, 0, "kb_mn2o"},
+        {11, 0, 0, "ka_mo2"}, {11, 1, 0, "kb_mo2"}, {13, 0, 0, "ka_mco2"}, {13, 0, 1, "ka_mco"}, {13, 1, 0, "kb_mo3"},
+        {15, 0, 0, "ka_mn2"},
+    };
+    for (const Slot &s : slots) {
+        LwBand<R> &B = T.lw[s.band - 1];
+        const R *p = d.need<R>(bname("lw", s.band, s.key, true));
+        if (s.kind == 0) B.ka_m[s.idx] = p;
+        if (s.kind == 1) B.kb_m[s.idx] = p;
+        if (s.kind == 2) B.x1 = p;
+        if (s.kind == 3) B.x2 = p;
+    }
+    // reference ratios of the binary-species bands from chi_mls (taugbNN.cu constants)
+    const TEntry *chi = d.find("lw/chi_mls");
+    if (!chi || chi->count < 7 * 59) return WR_ERR_TABLES;
+    const float *cm = (const float *)chi->data;
+    auto rr = [&](int g1, int l1, int g2, int l2) {
+        return (R)((double)cm[(l1 - 1) * 7 + (g1 - 1)] / (double)cm[(l2 - 1) * 7 + (g2 - 1)]);
+    };
+    T.lw[2].refrat_planck_a = rr(1, 9, 2, 9);
+    T.lw[2].refrat_planck_b = rr(1, 13, 2, 13);
+    T.lw[2].refrat_m_a = rr(1, 3, 2, 3);
+    T.lw[2].refrat_m_b = rr(1, 13, 2, 13);
+    T.lw[3].refrat_planck_a = rr(1, 11, 2, 11);
+    T.lw[3].refrat_planck_b = rr(3, 13, 2, 13);
+    T.lw[4].refrat_planck_a = rr(1, 5, 2, 5);
+    T.lw[4].refrat_planck_b = rr(3, 43, 2, 43);
+    T.lw[4].refrat_m_a = rr(1, 7, 2, 7);
+    T.lw[6].refrat_planck_a = rr(1, 3, 3, 3);
+    T.lw[6].refrat_m_a = rr(1, 3, 3, 3);
+    T.lw[8].refrat_planck_a = rr(1, 9, 6, 9);
+    T.lw[8].refrat_m_a = rr(1, 3, 6, 3);
+    T.lw[11].refrat_planck_a = rr(1, 10, 2, 10);
+    T.lw[12].refrat_planck_a = rr(1, 5, 4, 5);
+    T.lw[12].refrat_m_a = rr(1, 1, 4, 1);
+    T.lw[12].refrat_m_a3 = rr(1, 3, 4, 3);
+    T.lw[14].refrat_planck_a = rr(4, 1, 2, 1);
+    T.lw[14].refrat_m_a = rr(4, 1, 2, 1);
+    T.lw[15].refrat_planck_a = rr(1, 6, 6, 6);
+
+    const TEntry *srat = d.find("sw/strrat");
+    if (!srat || srat->count < 14) return WR_ERR_TABLES;
+    for (int k = 0; k < 14; ++k) {
+        SwBand<R> &B = T.sw[k];
+        const int jb = k + 16;
+        B.absa = d.opt<R>(bname("sw", jb, "absa", false));
+        B.absb = d.opt<R>(bname("sw", jb, "absb", false));
+        B.selfref = d.opt<R>(bname("sw", jb, "selfref", false));
+        B.forref = d.opt<R>(bname("sw", jb, "forref", false));
+        B.rayla = d.opt<R>(bname("sw", jb, "rayla", false));
+        B.raylb = d.opt<R>(bname("sw", jb, "raylb", false));
+        B.abso3a = d.opt<R>(bname("sw", jb, "abso3a", false));
+        B.abso3b = d.opt<R>(bname("sw", jb, "abso3b", false));
+        B.absch4 = d.opt<R>(bname("sw", jb, "absch4", false));
+        B.absh2o = d.opt<R>(bname("sw", jb, "absh2o", false));
+        B.absco2 = d.opt<R>(bname("sw", jb, "absco2", false));
+        const TEntry *ry = d.find(bname("sw", jb, "rayl", false));
+        if (ry && ry->count == 1)
+            B.rayl = (R)((const float *)ry->data)[0];
+        else if (ry)
+            B.rayl_v = b.conv<R>(*ry);
+        B.strrat = (R)((const float *)srat->data)[k];
+    }
+    // strrat(16)/(28) come with their band files (kgb16/kgb28); strrat(18) is a literal
+    // (phys_rrtm_sw_taumol18.py:43)
+    bool ok16 = true, ok28 = true;
+    const double s16 = d.scalar("sw/b16/strrat16", &ok16), s28 = d.scalar("sw/b28/strrat28", &ok28);
+    if (ok16) T.sw[0].strrat = (R)s16;
+    if (ok28) T.sw[12].strrat = (R)s28;
+    T.sw[2].strrat = (R)3.89589e+1;
+    return b.bad ? WR_ERR_TABLES : WR_OK;
+}
+
+template <typename R> int upload(const TFile &f, void **buf, Tabs<R> &host, Tabs<R> **dev)
+{
+    Blob sizing;
+    int rc = bind_tables<R>(f, sizing, host);
+    if (rc != WR_OK) return rc;
+    if (hipMalloc(buf, sizing.bytes.size()) != hipSuccess) return WR_ERR_NOMEM;
+    Blob place;
+    place.base = (uintptr_t)*buf;
+    rc = bind_tables<R>(f, place, host);
+    if (rc != WR_OK) return rc;
+    if (place.bytes.size() != sizing.bytes.size()) return WR_ERR_TABLES;
+    if (hipMemcpy(*buf, place.bytes.data(), place.bytes.size(), hipMemcpyHostToDevice) != hipSuccess) return WR_ERR_HIP;
+    if (hipMalloc((void **)dev, sizeof(Tabs<R>)) != hipSuccess) return WR_ERR_NOMEM;
+    if (hipMemcpy(*dev, &host, sizeof(Tabs<R>), hipMemcpyHostToDevice) != hipSuccess) return WR_ERR_HIP;
+    return WR_OK;
+}
+
+__global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int n)
+{
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < n) b[j] = (float)a[j];
+}
+
+// ---- diagnostics: setcoef indices and McICA words of one tile ---------------------------------
+__global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                             ColScal cs, wr_diag dg)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const size_t nc = (size_t)in.ncol;
+    const int day = cs.day[t];
+    for (int k = 0; k < nrl; ++k) {
+        const int kin = nrl - 1 - k;
+        const double la = log_d(rc.plyr(kin));
+        LwLay<double> L;
+        lw_layer<double>(T, rc, par, kin, la, L);
+        if (dg.lw_jp) dg.lw_jp[k * nc + i] = L.jp;
+        if (dg.lw_jt) dg.lw_jt[k * nc + i] = L.jt;
+        if (dg.lw_jt1) dg.lw_jt1[k * nc + i] = L.jt1;
+        SwLay<double> S;
+        if (day) sw_layer<double>(T, rc, par, kin, la, S);
+        if (dg.sw_jp) dg.sw_jp[k * nc + i] = day ? S.jp : -1;
+        if (dg.sw_jt) dg.sw_jt[k * nc + i] = day ? S.jt : -1;
+        if (dg.sw_jt1) dg.sw_jt1[k * nc + i] = day ? S.jt1 : -1;
+    }
+    if (dg.lw_laytrop) dg.lw_laytrop[i] = cs.lw_laytrop[t];
+    if (dg.sw_laytrop) dg.sw_laytrop[i] = day ? cs.sw_laytrop[t] : -1;
+    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    if (dg.lw_mask) {
+        const int lcf1 = cs.lcf1[t];
+        for (int k = 0; k < nrl; ++k)
+            for (int w = 0; w < 5; ++w) dg.lw_mask[((size_t)k * 5 + w) * nc + i] = 0u;
+        for (int g = 0; g < 140; ++g) {
+            const uint64_t mb = mcica_g(col, par.seed, 0, g, nrl, par.iovr,
+                                        [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+            for (int k = 0; k < nrl; ++k)
+                if ((mb >> k) & 1) dg.lw_mask[((size_t)k * 5 + (g >> 5)) * nc + i] |= 1u << (g & 31);
+        }
+    }
+    if (dg.sw_mask) {
+        for (int k = 0; k < nrl; ++k)
+            for (int w = 0; w < 4; ++w) dg.sw_mask[((size_t)k * 4 + w) * nc + i] = 0u;
+        if (day) {
+            const double cf1 = cs.zcf1[t];
+            for (int g = 0; g < 112; ++g) {
+                const uint64_t mb = mcica_g(col, par.seed, 1, g, nrl, par.iovr, [&](int k) {
+                    const double ct = rc.cfrac(nrl - 1 - k);
+                    return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
+                });
+                for (int k = 0; k < nrl; ++k)
+                    if ((mb >> k) & 1) dg.sw_mask[((size_t)k * 4 + (g >> 5)) * nc + i] |= 1u << (g & 31);
+            }
+        }
+    }
+}
+
+// ---- CAL_CLDFRA (phys_rad_cal_cldfra.cu:29-55) ----------------------------------------------------
+__global__ void k_cal_cldfra(int im, int jm, int lm, int i_lo, int i_hi, int j_lo, int j_hi, int f_qc, int f_qi,
+                             float *__restrict__ cldfra, const float *__restrict__ qc, const float *__restrict__ qi)
+{
+    const size_t p = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    const size_t tot = (size_t)im * jm * lm;
+    if (p >= tot) return;
+    const int i = (int)(p % im);
+    const int j = (int)((p / im) % jm);
+    if (i < i_lo || i > i_hi || j < j_lo || j > j_hi) return;
+    float v;
+    if (f_qi && f_qc)
+        v = (qc[p] + qi[p] > 1.0e-6f) ? 1.0f : 0.0f;
+    else if (f_qc)
+        v = (qc[p] > 1.0e-6f) ? 1.0f : 0.0f;
+    else
+        v = 0.0f;
+    cldfra[p] = v;
+}
+
+// ---- McICA masks from a given surface-first cloud fraction ---------------------------------------
+__global__ void __launch_bounds__(64) k_mcica_masks(int ncol, int nrl, const double *__restrict__ cldf,
+                                                    const uint64_t *__restrict__ col_id, uint64_t seed, int iovr,
+                                                    uint32_t *__restrict__ lw_mask, uint32_t *__restrict__ sw_mask)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncol) return;
+    const size_t nc = (size_t)ncol;
+    const uint64_t col = col_id ? col_id[i] : (uint64_t)i;
+    auto cf = [&](int k) { return cldf[k * nc + i]; };
+    for (int kind = 0; kind < 2; ++kind) {
+        uint32_t *m = kind == 0 ? lw_mask : sw_mask;
+        if (!m) continue;
+        const int ng = kind == 0 ? 140 : 112, words = kind == 0 ? 5 : 4;
+        for (int k = 0; k < nrl; ++k)
+            for (int w = 0; w < words; ++w) m[((size_t)k * words + w) * nc + i] = 0u;
+        for (int g = 0; g < ng; ++g) {
+            const uint64_t mb = mcica_g(col, seed, kind, g, nrl, iovr, cf);
+            for (int k = 0; k < nrl; ++k)
+                if ((mb >> k) & 1) m[((size_t)k * words + (g >> 5)) * nc + i] |= 1u << (g & 31);
+        }
+    }
+}
+
+struct DevGuard {  // make the context's device current for the call, restore afterwards
+    int prev = -1;
+    explicit DevGuard(int dev)
+    {
+        if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+        if (prev != dev) (void)hipSetDevice(dev);
+    }
+    ~DevGuard()
+    {
+        int cur;
+        if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+    }
+};
+
+int check_args(const wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o)
+{
+    if (!c || !in || !p || !o) return WR_ERR_ARG;
+    if (in->ncol < 0 || in->nlay < 2 || in->nlay + 1 > c->nl_max) return WR_ERR_ARG;
+    if (in->ncol == 0) return WR_OK;
+    if (!in->p_lev || !in->t_lay || !in->q_lay || !in->o3_lay || !in->cldf || !in->qc || !in->qi || !in->t_sfc ||
+        !in->emis || !in->alb || !in->cosz)
+        return WR_ERR_ARG;
+    if (!o->hr_lw || !o->hr_sw || !o->flw_up || !o->flw_dn || !o->fsw_up || !o->fsw_dn) return WR_ERR_ARG;
+    if (p->iovr < 0 || p->iovr > 2) return WR_ERR_ARG;
+    if (!(p->re_liq > 0.0) || !(p->re_ice > 0.0) || !(p->solcon > 0.0)) return WR_ERR_ARG;
+    return WR_OK;
+}
+
+ColIn to_colin(const wr_columns *in)
+{
+    ColIn ci;
+    ci.p_lev = in->p_lev, ci.t_lay = in->t_lay, ci.q_lay = in->q_lay, ci.o3_lay = in->o3_lay;
+    ci.cldf = in->cldf, ci.qc = in->qc, ci.qi = in->qi, ci.t_sfc = in->t_sfc, ci.emis = in->emis;
+    ci.alb = in->alb, ci.cosz = in->cosz, ci.col_id = in->col_id;
+    ci.ncol = in->ncol, ci.nlay = in->nlay;
+    return ci;
+}
+RunPar to_par(const wr_params *p)
+{
+    RunPar r;
+    r.mcica = p->mcica, r.iovr = p->iovr, r.seed = p->seed;
+    r.solcon = p->solcon, r.re_liq = p->re_liq, r.re_ice = p->re_ice;
+    for (int g = 0; g < 10; ++g) r.vmr[g] = p->vmr[g];
+    return r;
+}
+
+int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o, const wr_diag *dg, void *stream,
+        bool mixed)
+{
+    int rc = check_args(c, in, p, o);
+    if (rc != WR_OK || in->ncol == 0) return rc;
+    if (p->mcica != 1) return WR_ERR_UNSUPPORTED;
+    if (mixed) return WR_ERR_UNSUPPORTED;
+    DevGuard g(c->device);
+    hipStream_t s = (hipStream_t)stream;
+    const ColIn ci = to_colin(in);
+    const RunPar par = to_par(p);
+    OutPtrs op;
+    op.hr_lw = o->hr_lw, op.hr_sw = o->hr_sw, op.flw_up = o->flw_up, op.flw_dn = o->flw_dn;
+    op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn;
+    for (int col0 = 0; col0 < in->ncol; col0 += c->max_cols) {
+        const int n = (in->ncol - col0 < c->max_cols) ? in->ncol - col0 : c->max_cols;
+        rc = wr_launch_fp64(c, ci, par, op, col0, n, s);
+        if (rc != WR_OK) return rc;
+        if (dg) {
+            tbeg(c, KID_DIAG, s);
+            hipLaunchKernelGGL(k_diag, dim3((n + 63) / 64), dim3(64), 0, s, c->tabs_d, ci, par, col0, n, colscal(c), *dg);
+            tend(c, s);
+            if (hipGetLastError() != hipSuccess) return WR_ERR_HIP;
+        }
+    }
+    return WR_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols)
+{
+    if (!out || !table_path || max_cols <= 0) return WR_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return WR_ERR_HIP;
+    TFile f;
+    int rc = read_tables(table_path, f);
+    if (rc != WR_OK) return rc;
+    DevGuard g(device);
+    wr_ctx *c = (wr_ctx *)calloc(1, sizeof(wr_ctx));
+    if (!c) return WR_ERR_NOMEM;
+    c->device = device;
+    c->max_cols = max_cols;
+    c->nl_max = 64;  // McICA bits of a g-point over the layers are one 64-bit word
+    rc = upload<double>(f, &c->tab_buf_d, c->host_d, &c->tabs_d);
+    if (rc == WR_OK) rc = upload<float>(f, &c->tab_buf_f, c->host_f, &c->tabs_f);
+    if (rc == WR_OK) rc = wr_init_luts(c, nullptr);
+    if (rc == WR_OK) {
+        const double *src[4] = {c->host_d.exp_tbl, c->host_d.tfn_tbl, c->host_d.tau_tbl, c->host_d.sw_exp_tbl};
+        float *dst[4] = {(float *)c->host_f.exp_tbl, (float *)c->host_f.tfn_tbl, (float *)c->host_f.tau_tbl,
+                         (float *)c->host_f.sw_exp_tbl};
+        for (int q = 0; q < 4; ++q)
+            hipLaunchKernelGGL(k_d2f, dim3((NTBL + 256) / 256), dim3(256), 0, nullptr, src[q], dst[q], NTBL + 1);
+        if (hipGetLastError() != hipSuccess) rc = WR_ERR_HIP;
+    }
+    const size_t m = (size_t)max_cols;
+    if (rc == WR_OK) {
+        if (hipMalloc((void **)&c->acc, sizeof(double) * 4 * (NL + 1) * m) != hipSuccess ||
+            hipMalloc((void **)&c->iws, sizeof(int) * 6 * m) != hipSuccess ||
+            hipMalloc((void **)&c->dws, sizeof(double) * 2 * m) != hipSuccess ||
+            hipMalloc((void **)&c->sflux_d, sizeof(double) * 112 * m) != hipSuccess ||
+            hipMalloc((void **)&c->sflux_f, sizeof(float) * 112 * m) != hipSuccess)
+            rc = WR_ERR_NOMEM;
+    }
+    if (rc == WR_OK && hipDeviceSynchronize() != hipSuccess) rc = WR_ERR_HIP;
+    if (rc != WR_OK) {
+        wr_destroy(c);
+        return rc;
+    }
+    *out = c;
+    return WR_OK;
+}
+
+int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, const wr_diag *diag,
+                void *stream)
+{
+    return run(ctx, in, par, out, diag, stream, false);
+}
+
+int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, const wr_diag *diag,
+                 void *stream)
+{
+    return run(ctx, in, par, out, diag, stream, true);
+}
+
+int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi, int32_t j_lo, int32_t j_hi,
+                  int32_t f_qc, int32_t f_qi, float *cldfra, const float *qc, const float *qi, void *stream)
+{
+    if (im < 0 || jm < 0 || lm < 0 || !cldfra) return WR_ERR_ARG;
+    const size_t tot = (size_t)im * jm * lm;
+    if (tot == 0) return WR_OK;
+    if (f_qc && !qc) return WR_ERR_ARG;
+    if (f_qc && f_qi && !qi) return WR_ERR_ARG;
+    hipLaunchKernelGGL(k_cal_cldfra, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, (hipStream_t)stream, im, jm, lm,
+                       i_lo, i_hi, j_lo, j_hi, f_qc, f_qi, cldfra, qc, qi);
+    return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
+}
+
+int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t *col_id, uint64_t seed, int32_t iovr,
+                   uint32_t *lw_mask, uint32_t *sw_mask, void *stream)
+{
+    if (ncol < 0 || nrl < 1 || nrl > 64 || iovr < 0 || iovr > 2) return WR_ERR_ARG;
+    if (ncol == 0 || (!lw_mask && !sw_mask)) return WR_OK;
+    if (!cldf) return WR_ERR_ARG;
+    hipLaunchKernelGGL(k_mcica_masks, dim3((ncol + 63) / 64), dim3(64), 0, (hipStream_t)stream, ncol, nrl, cldf, col_id,
+                       seed, iovr, lw_mask, sw_mask);
+    return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
+}
+
+int wr_timing(wr_ctx *c, int32_t enable)
+{
+    if (!c) return WR_ERR_ARG;
+    DevGuard g(c->device);
+    if (enable && !c->ev) {
+        const int cap = 8192;
+        c->ev = (hipEvent_t *)calloc(2 * (size_t)cap, sizeof(hipEvent_t));
+        c->ev_kid = (int *)calloc(cap, sizeof(int));
+        if (!c->ev || !c->ev_kid) return WR_ERR_NOMEM;
+        for (int j = 0; j < 2 * cap; ++j)
+            if (hipEventCreate(&c->ev[j]) != hipSuccess) return WR_ERR_HIP;
+        c->ev_cap = cap;
+    }
+    c->timing = enable ? 1 : 0;
+    c->ev_n = 0;
+    return WR_OK;
+}
+
+int wr_timing_read(wr_ctx *c, int32_t *kid, float *ms, int32_t cap, int32_t *n)
+{
+    if (!c || !n || (cap > 0 && (!kid || !ms))) return WR_ERR_ARG;
+    DevGuard g(c->device);
+    const int m = c->ev_n < cap ? c->ev_n : cap;
+    for (int j = 0; j < m; ++j) {
+        if (hipEventSynchronize(c->ev[2 * j + 1]) != hipSuccess) return WR_ERR_HIP;
+        float t = 0.0f;
+        if (hipEventElapsedTime(&t, c->ev[2 * j], c->ev[2 * j + 1]) != hipSuccess) return WR_ERR_HIP;
+        kid[j] = c->ev_kid[j];
+        ms[j] = t;
+    }
+    *n = m;
+    c->ev_n = 0;
+    return WR_OK;
+}
+
+const char *wr_kernel_name(int32_t kid)
+{
+    static const char *lw[16] = {"k_lw_b01", "k_lw_b02", "k_lw_b03", "k_lw_b04", "k_lw_b05", "k_lw_b06",
+                                 "k_lw_b07", "k_lw_b08", "k_lw_b09", "k_lw_b10", "k_lw_b11", "k_lw_b12",
+                                 "k_lw_b13", "k_lw_b14", "k_lw_b15", "k_lw_b16"};
+    static const char *sw[14] = {"k_sw_b16", "k_sw_b17", "k_sw_b18", "k_sw_b19", "k_sw_b20", "k_sw_b21", "k_sw_b22",
+                                 "k_sw_b23", "k_sw_b24", "k_sw_b25", "k_sw_b26", "k_sw_b27", "k_sw_b28", "k_sw_b29"};
+    if (kid == KID_ZERO) return "k_zero";
+    if (kid == KID_COLPREP) return "k_colprep";
+    if (kid == KID_DAYLIST) return "k_daylist";
+    if (kid >= KID_LW0 && kid < KID_LW0 + 16) return lw[kid - KID_LW0];
+    if (kid >= KID_SW0 && kid < KID_SW0 + 14) return sw[kid - KID_SW0];
+    if (kid == KID_POST) return "k_post";
+    if (kid == KID_DIAG) return "k_diag";
+    return "?";
+}
+
+void wr_default_params(wr_params *p)
+{
+    memset(p, 0, sizeof(*p));
+    p->mcica = 1;
+    p->iovr = 1;
+    p->seed = 0;
+    p->solcon = 1368.22;
+    p->re_liq = 10.0;
+    p->re_ice = 50.0;
+    // co2, n2o, ch4, o2, co, cfc11, cfc12, cfc22, ccl4, cfc113 (phys_rrtm_getgases.cu:56-65)
+    const double v[10] = {410.0e-6, 0.33e-6, 1.9e-6, 0.209, 1.50e-8, 3.520e-10, 6.358e-10, 1.500e-10, 1.397e-10, 8.2000e-11};
+    for (int g = 0; g < 10; ++g) p->vmr[g] = v[g];
+}
+
+const char *wr_strerror(int code)
+{
+    switch (code) {
+    case WR_OK: return "ok";
+    case WR_ERR_ARG: return "invalid argument (size or null pointer)";
+    case WR_ERR_TABLES: return "table file missing or malformed";
+    case WR_ERR_HIP: return "HIP runtime error";
+    case WR_ERR_NOMEM: return "device allocation failed";
+    case WR_ERR_UNSUPPORTED: return "option not supported by this build";
+    default: return "unknown error";
+    }
+}
+
+void wr_destroy(wr_ctx *c)
+{
+    if (!c) return;
+    DevGuard g(c->device);
+    (void)hipDeviceSynchronize();
+    (void)hipFree(c->tab_buf_d);
+    (void)hipFree(c->tab_buf_f);
+    (void)hipFree(c->tabs_d);
+    (void)hipFree(c->tabs_f);
+    (void)hipFree(c->acc);
+    (void)hipFree(c->iws);
+    (void)hipFree(c->dws);
+    (void)hipFree(c->sflux_d);
+    (void)hipFree(c->sflux_f);
+    if (c->ev)
+        for (int j = 0; j < 2 * c->ev_cap; ++j)
+            if (c->ev[j]) (void)hipEventDestroy(c->ev[j]);
+    free(c->ev);
+    free(c->ev_kid);
+    free(c);
+}
+
+}  // extern "C"

@@ -1,0 +1,82 @@
+// wr_internal.h -- host-side context of libwoofrad (shared by the host ABI and the launch units).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/woofrad.h"
+#include "wr_kernels.h"
+
+struct wr_ctx {
+    int device;
+    int max_cols;
+    int nl_max;                      // radiation layers the work space is sized for
+    void *tab_buf_d, *tab_buf_f;     // device table storage (FP64, FP32)
+    wr::Tabs<double> *tabs_d;        // device copies of the table structs
+    wr::Tabs<float> *tabs_f;
+    wr::Tabs<double> host_d;         // host images (device pointers inside)
+    wr::Tabs<float> host_f;
+    // tile work space
+    double *acc;                     // 4 x [nl_max + 2][max_cols]
+    int *iws;                        // 6 x [max_cols]
+    double *dws;                     // 2 x [max_cols]
+    double *sflux_d;                 // [112][max_cols]
+    float *sflux_f;
+    // optional per-launch event timing (wr_timing)
+    int timing, ev_cap, ev_n;
+    hipEvent_t *ev;                  // 2 x ev_cap
+    int *ev_kid;
+};
+
+// kernel ids reported by wr_timing_read
+enum { KID_ZERO = 0, KID_COLPREP = 1, KID_DAYLIST = 2, KID_LW0 = 3, KID_SW0 = 19, KID_POST = 33, KID_DIAG = 34,
+       KID_COUNT = 35 };
+
+inline void tbeg(wr_ctx *c, int kid, hipStream_t s)
+{
+    if (c->timing && c->ev_n < c->ev_cap) {
+        (void)hipEventRecord(c->ev[2 * c->ev_n], s);
+        c->ev_kid[c->ev_n] = kid;
+    }
+}
+inline void tend(wr_ctx *c, hipStream_t s)
+{
+    if (c->timing && c->ev_n < c->ev_cap) {
+        (void)hipEventRecord(c->ev[2 * c->ev_n + 1], s);
+        c->ev_n++;
+    }
+}
+
+namespace wr {
+inline ColScal colscal(wr_ctx *c)
+{
+    const size_t m = (size_t)c->max_cols;
+    ColScal s;
+    s.lw_laytrop = c->iws + 0 * m;
+    s.lcf1 = c->iws + 1 * m;
+    s.day = c->iws + 2 * m;
+    s.sw_laytrop = c->iws + 3 * m;
+    s.dlist = c->iws + 4 * m;
+    s.nday = c->iws + 5 * m;
+    s.pwv = c->dws + 0 * m;
+    s.zcf1 = c->dws + 1 * m;
+    return s;
+}
+inline Acc tile_acc(wr_ctx *c, int n, int nrl)
+{
+    const size_t per = (size_t)(nrl + 1) * n;
+    Acc a;
+    a.lw_up = c->acc;
+    a.lw_dn = c->acc + per;
+    a.sw_up = c->acc + 2 * per;
+    a.sw_dn = c->acc + 3 * per;
+    return a;
+}
+}  // namespace wr
+
+// launch units
+int wr_launch_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
+                   hipStream_t s);
+int wr_launch_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
+                    hipStream_t s);
+int wr_launch_diag(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
+                   hipStream_t s);
+int wr_init_luts(wr_ctx *c, hipStream_t s);

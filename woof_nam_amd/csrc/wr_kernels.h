@@ -1,0 +1,489 @@
+// wr_kernels.h -- kernels of the MI355X RRTMG radiation path (templated on the optics precision R).
+//
+// Per column tile (n columns starting at col0):
+//   k_colprep   column scalars: LW laytrop, precipitable water, cloud flag; SW day flag, column cloud
+//               cover (max-random), SW laytrop and the solar source of the 112 g-points
+//   k_daylist   ordered compaction of day columns (one workgroup, deterministic order)
+//   k_lw<B>     one thread per column, all g-points of band B in registers: McICA masks, then the
+//               rtrnmc down sweep and up sweep (optics recomputed per sweep, so nothing per layer is
+//               stored); band fluxes added level by level into the FP64 accumulators
+//   k_sw<B>     one thread per day column: for each g-point of band B the two-stream adding method
+//               as one bottom-up sweep (reflectance of the atmosphere below each level) and one
+//               top-down sweep (direct/diffuse transmission and level fluxes)
+//   k_post      FP64 level fluxes -> heating rates, top-first outputs
+#pragma once
+#include "wr_lw.h"
+#include "wr_sw.h"
+
+namespace wr {
+
+struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each
+    double *lw_up, *lw_dn, *sw_up, *sw_dn;
+};
+struct ColScal {
+    int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
+    double *pwv, *zcf1;
+};
+
+template <typename R>
+__global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                                 ColScal cs, R *__restrict__ sflux)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const double tem1g = 100.0 * CON_G;
+    const double tem2c = 1.0e-20 * 1.0e3 * CON_AVGD;
+    double tem1 = 0.0, tem2 = 0.0, z0 = 1.0, z1 = 1.0;
+    int lt = 0, lcf = 0, swlt = nrl;
+    int jpk[NL];
+    for (int k = 0; k < nrl; ++k) {
+        const int kin = nrl - 1 - k;
+        const double dlp = rc.plvl(kin + 1) - rc.plvl(kin);
+        const double qv = rc.qlyr(kin);
+        const double h2ovmr = fmax(0.0, qv * AMDW / (1.0 - qv));
+        const double tem0 = (1.0 - h2ovmr) * CON_AMD + h2ovmr * CON_AMW;
+        const double cdry = tem2c * dlp / (tem1g * tem0 * (1.0 + h2ovmr));
+        const double ca0 = fmax(0.0, cdry * h2ovmr);
+        tem1 = tem1 + cdry + ca0;
+        tem2 = tem2 + ca0;
+        const double la = log_d(rc.plyr(kin));
+        if (la > 4.56) {
+            lt += 1;
+            swlt = k + 1;
+        }
+        jpk[k] = imax(1, imin(58, (int)(36.0 - 5.0 * (la + 0.04))));
+        double ct, cwp, cip;
+        rc.cloud(kin, ct, cwp, cip);
+        if (ct > EPS) lcf = 1;
+        if (ct > FTINY) {
+            z1 = fmin(z1, 1.0 - ct);
+        } else if (z1 < 1.0) {
+            z0 = z0 * z1;
+            z1 = 1.0;
+        }
+    }
+    cs.lw_laytrop[t] = lt;
+    cs.lcf1[t] = lcf;
+    const double tem0p = 10.0 * tem2 / (AMDW * tem1 * CON_G);
+    cs.pwv[t] = tem0p * rc.plvl(nrl);
+    const double cz = in.cosz[i];
+    const int day = cz >= 1.0e-4;
+    cs.day[t] = day;
+    if (!day) return;
+    z0 = z0 * z1;
+    if (z0 <= FTINY) z0 = 0.0;
+    if (z0 > ONEMINUS) z0 = 1.0;
+    cs.zcf1[t] = 1.0 - z0;
+    cs.sw_laytrop[t] = swlt;
+    // solar source per g-point (phys_rrtm_sw_taumol.cu:59-120)
+    for (int b = 0; b < 14; ++b) {
+        const int jb = b + 16, ibd = T->ibx[b], njb = SW_NG[b], ns = SW_NS[b], lr = T->layreffr[b];
+        if (jb == 16 || jb == 20 || jb == 23 || jb == 25 || jb == 26 || jb == 29) {
+            for (int j = 0; j < njb; ++j) sflux[(size_t)(ns + j) * n + t] = T->sf01[(ibd - 1) * 16 + j];
+        } else if (jb == 27) {
+            for (int j = 0; j < njb; ++j) sflux[(size_t)(ns + j) * n + t] = T->scalekur * T->sf01[(ibd - 1) * 16 + j];
+        } else {
+            int ks, mfs;
+            const R *tab;
+            if (jb == 17 || jb == 28) {
+                ks = nrl;
+                for (int k = swlt; k <= nrl - 1; ++k)
+                    if (jpk[k - 1] < lr && jpk[k] >= lr) {
+                        ks = k + 1;
+                        break;
+                    }
+                tab = T->sf02;
+                mfs = 5;
+            } else {
+                ks = swlt;
+                for (int k = 1; k <= swlt - 1; ++k)
+                    if (jpk[k - 1] < lr && jpk[k] >= lr) {
+                        ks = k + 1;
+                        break;
+                    }
+                tab = T->sf03;
+                mfs = 9;
+            }
+            SwLay<R> L;
+            const int kin = nrl - 1 - (ks - 1);
+            sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+            const R colm1 = L.colamt[T->ix1[b] - 1];
+            const R colm2 = L.colamt[T->ix2[b] - 1];
+            const R speccomb = colm1 + T->sw_strrat[b] * colm2;
+            const R specmult = T->sw_specwt[b] * fmin((R)0.999999, colm1 / speccomb);
+            const int jsi = (int)specmult;
+            const int js = 1 + jsi;
+            const R fs = specmult - (R)jsi;
+            const int base = ((ibd - 1) * mfs) * 16;
+            for (int j = 0; j < njb; ++j) {
+                const R a = tab[base + (js - 1) * 16 + j];
+                const R c = tab[base + js * 16 + j];
+                sflux[(size_t)(ns + j) * n + t] = a + fs * (c - a);
+            }
+        }
+    }
+}
+
+// ordered compaction of the day columns of a tile (single workgroup of 1024 threads)
+static __global__ void __launch_bounds__(1024) k_daylist(const int *__restrict__ day, int n, int *__restrict__ dlist,
+                                                  int *__restrict__ nday)
+{
+    __shared__ int part[1024];
+    const int tid = threadIdx.x;
+    const int chunk = (n + 1023) / 1024;
+    const int b = tid * chunk, e = min(n, b + chunk);
+    int cnt = 0;
+    for (int j = b; j < e; ++j) cnt += day[j];
+    part[tid] = cnt;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+        const int v = (tid >= off) ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    int pos = part[tid] - cnt;
+    for (int j = b; j < e; ++j)
+        if (day[j]) dlist[pos++] = j;
+    if (tid == 1023) *nday = part[1023];
+}
+
+static __global__ void k_zero(double *__restrict__ a, size_t n)
+{
+    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) a[j] = 0.0;
+}
+
+// ---- LW band kernel (rtrnmc, McICA) -------------------------------------------------------------
+constexpr double LW_A0[16] = {1.66, 1.55, 1.58, 1.66, 1.54, 1.454, 1.89, 1.33, 1.668, 1.66, 1.66, 1.66, 1.66, 1.66, 1.66, 1.66};
+constexpr double LW_A1[16] = {0.00, 0.25, 0.22, 0.00, 0.13, 0.446, -0.10, 0.40, -0.006, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00};
+constexpr double LW_A2[16] = {0.00, -12.0, -11.7, 0.00, -0.72, -0.243, 0.19, -0.062, 0.414, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00};
+
+template <typename R> struct GasT {  // rtrnmc gas transmittance and source factor of one g, one layer
+    R trng, atrgas, gasfac, odepth;
+};
+template <typename R> __device__ __forceinline__ GasT<R> lw_gas(const Tabs<R> *__restrict__ T, R sdif, R tau)
+{
+    GasT<R> q;
+    R od = fmax((R)0, sdif * tau);
+    if (od <= (R)0.06) {
+        q.atrgas = od - (R)0.5 * od * od;
+        q.trng = (R)1 - q.atrgas;
+        q.gasfac = (R)0.166667 * od;
+    } else {
+        const R tblind = od / ((R)(1.0 / 0.278) + od);
+        const int itgas = (int)((R)10000 * tblind + (R)0.5);
+        q.trng = T->exp_tbl[itgas];
+        q.atrgas = (R)1 - q.trng;
+        q.gasfac = T->tfn_tbl[itgas];
+        od = T->tau_tbl[itgas];
+    }
+    q.odepth = od;
+    return q;
+}
+template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__restrict__ T, R odtot, R &totfac, R &atrtot)
+{
+    if (odtot < (R)0.06) {
+        totfac = (R)0.166667 * odtot;
+        atrtot = odtot - (R)0.5 * odtot * odtot;
+    } else {
+        const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
+        const int ittot = (int)((R)10000 * tblind + (R)0.5);
+        totfac = T->tfn_tbl[ittot];
+        atrtot = (R)1 - T->exp_tbl[ittot];
+    }
+}
+
+template <typename R, int B>
+__global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                           ColScal cs, Acc acc)
+{
+    constexpr int NG = LW_NG[B];
+    constexpr int NS = LW_NS[B];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const int lcf1 = cs.lcf1[t];
+    const double pwv = cs.pwv[t];
+    const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
+    const double sfe = in.emis[i];
+    const R semis = (R)((sfe > EPS && sfe <= 1.0) ? sfe : 1.0);
+    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    uint64_t mb[NG];
+    for (int g = 0; g < NG; ++g)
+        mb[g] = mcica_g(col, par.seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+
+    // ---- down sweep, top to surface ----
+    R rad[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) rad[g] = (R)0;
+    double tz_top = rc.tlyr(0);
+    double la_c = log_d(rc.plyr(0));
+    for (int kin = 0; kin < nrl; ++kin) {
+        const int k = nrl - 1 - kin;
+        double tz_bot, la_n = 0.0;
+        if (kin + 1 < nrl) {
+            la_n = log_d(rc.plyr(kin + 1));
+            tz_bot = rc.tlvl(kin + 1, la_n, la_c, log_d(rc.plvl(kin + 1)));
+        } else {
+            tz_bot = rc.tskn();
+        }
+        LwLay<R> L;
+        lw_layer<R>(T, rc, par, kin, la_c, L);
+        R tau[NG], fr[NG];
+        lw_optics<R, B>(T, L, tau, fr);
+        const R blay = planck<R>(T, B, L.tavel);
+        const R dplnkd = planck<R>(T, B, tz_bot) - blay;
+        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const R plfrac = fr[g];
+            const R bbdgas = plfrac * (blay + dplnkd * q.gasfac);
+            const R gassrcd = bbdgas * q.atrgas;
+            const R clfm = ((mb[g] >> k) & 1) ? (R)1 : (R)0;
+            if (clfm >= (R)EPS) {
+                const R odcld = sdif * taucld;
+                const R efclrfr = (R)1 - ((R)1 - M<R>::exp(-odcld)) * clfm;
+                R totfac, atrtot;
+                lw_tot<R>(T, q.odepth + odcld, totfac, atrtot);
+                const R bbdtot = plfrac * (blay + dplnkd * totfac);
+                const R totsrcd = bbdtot * atrtot;
+                rad[g] = rad[g] * q.trng * efclrfr + gassrcd + clfm * (totsrcd - gassrcd);
+            } else {
+                rad[g] = rad[g] * q.trng + gassrcd;
+            }
+            s = s + rad[g];
+        }
+        acc.lw_dn[(size_t)k * n + t] += (double)s;
+        tz_top = tz_bot;
+        la_c = la_n;
+    }
+    // ---- up sweep, surface to top ----
+    const R reflct = (R)1 - semis;
+    double tz_bot = rc.tskn();
+    la_c = log_d(rc.plyr(nrl - 1));
+    for (int kin = nrl - 1; kin >= 0; --kin) {
+        const int k = nrl - 1 - kin;
+        double la_m = 0.0;
+        if (kin > 0) {
+            la_m = log_d(rc.plyr(kin - 1));
+            tz_top = rc.tlvl(kin, la_c, la_m, log_d(rc.plvl(kin)));
+        } else {
+            tz_top = rc.tlyr(0);
+        }
+        LwLay<R> L;
+        lw_layer<R>(T, rc, par, kin, la_c, L);
+        R tau[NG], fr[NG];
+        lw_optics<R, B>(T, L, tau, fr);
+        const R blay = planck<R>(T, B, L.tavel);
+        const R dplnku = planck<R>(T, B, tz_top) - blay;
+        if (k == 0) {  // surface emission and reflection
+            const R pk0 = planck<R>(T, B, rc.tskn());
+            R s0 = (R)0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const R rad0 = semis * fr[g] * pk0;
+                rad[g] = rad0 + reflct * rad[g];
+                s0 = s0 + rad[g];
+            }
+            acc.lw_up[t] += (double)s0;
+        }
+        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const R plfrac = fr[g];
+            const R bbugas = plfrac * (blay + dplnku * q.gasfac);
+            const R gassrcu = bbugas * q.atrgas;
+            const R clfm = ((mb[g] >> k) & 1) ? (R)1 : (R)0;
+            if (clfm > (R)EPS) {
+                const R odcld = sdif * taucld;
+                const R efclrfr = (R)1 - ((R)1 - M<R>::exp(-odcld)) * clfm;
+                R totfac, atrtot;
+                lw_tot<R>(T, q.odepth + odcld, totfac, atrtot);
+                const R bbutot = plfrac * (blay + dplnku * totfac);
+                const R totsrcu = bbutot * atrtot;
+                rad[g] = rad[g] * q.trng * efclrfr + gassrcu + clfm * (totsrcu - gassrcu);
+            } else {
+                rad[g] = rad[g] * q.trng + gassrcu;
+            }
+            s = s + rad[g];
+        }
+        acc.lw_up[(size_t)(k + 1) * n + t] += (double)s;
+        tz_bot = tz_top;
+        la_c = la_m;
+    }
+}
+
+// ---- SW band kernel (McICA two-stream) -------------------------------------------------------------
+template <typename R, int B> struct SwLayerRT {
+    R rb[SW_NG[B]], rd[SW_NG[B]], tb[SW_NG[B]], td[SW_NG[B]], ld[SW_NG[B]];
+};
+
+template <typename R, int B>
+__device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
+                                            int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o)
+{
+    constexpr int NG = SW_NG[B];
+    const int nrl = rc.nrl;
+    const int kin = nrl - 1 - k;
+    SwLay<R> L;
+    sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+    R tg[NG], tr[NG];
+    sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+    R ctau = (R)0, cssa = (R)1, casy = (R)0;
+    if (cf1 > 0.0 && L.ct > FTINY) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        R ztau0 = fmax((R)FTINY, tr[g] + tg[g] + (R)0);
+        R zssa0 = tr[g] + (R)0;
+        R zasy0 = (R)0;
+        if ((mb[g] >> k) & 1) {
+            ztau0 = ztau0 + ctau;
+            zssa0 = zssa0 + cssa;
+            zasy0 = zasy0 + casy;
+        }
+        layer_rt<R>(ztau0, zssa0, zasy0, cz, snt, T->sw_exp_tbl, o.rb[g], o.rd[g], o.tb[g], o.td[g], o.ld[g]);
+    }
+}
+
+template <typename R, int B>
+__global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                           ColScal cs, const R *__restrict__ sflux, Acc acc)
+{
+    constexpr int NG = SW_NG[B];
+    constexpr int NS = SW_NS[B];
+    const int tt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= *cs.nday) return;
+    const int t = cs.dlist[tt];
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl, nlp1 = nrl + 1;
+    const double czd = in.cosz[i];
+    const R cz = (R)czd, snt = (R)(1.0 / czd);
+    const double ssolar = (par.solcon / S0) * czd;
+    const double alb = in.alb[i];
+    const R asfc = (R)((SW_IDXSFC[B] != 0) ? alb : 0.5 * (alb + alb));
+    const double cf1 = cs.zcf1[t];
+    const int lt = cs.sw_laytrop[t];
+    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    uint64_t mb[NG];
+    R zsolar[NG];
+    for (int g = 0; g < NG; ++g) {
+        zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
+        mb[g] = mcica_g(col, par.seed, 1, NS + g, nrl, par.iovr, [&](int k) {
+            const double ct = rc.cfrac(nrl - 1 - k);
+            return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
+        });
+    }
+    // bottom-up: reflectance of the medium below each level (swflux sweep 1)
+    R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
+    for (int k = 0; k < nrl; ++k) {
+        SwLayerRT<R, B> o;
+        sw_layer_rt<R, B>(T, rc, par, k, lt, cf1, mb, cz, snt, o);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
+            zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
+            zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
+        }
+    }
+    // top-down: direct beam, diffuse transmission and reflection from above, level fluxes
+    R tdbt[NG], tdn[NG], rdnd[NG];
+    R su = (R)0, sd = (R)0;
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        tdbt[g] = (R)1, tdn[g] = (R)1, rdnd[g] = (R)0;
+        const R zden1 = (R)1 / ((R)1 - rdnd[g] * zrupd[g][nlp1]);
+        const R zfu = (tdbt[g] * zrupb[g][nlp1] + (tdn[g] - tdbt[g]) * zrupd[g][nlp1]) * zden1;
+        const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * zrupb[g][nlp1] * rdnd[g]) * zden1;
+        su = su + zsolar[g] * zfu;
+        sd = sd + zsolar[g] * zfd;
+    }
+    acc.sw_up[(size_t)nrl * n + t] += (double)su;
+    acc.sw_dn[(size_t)nrl * n + t] += (double)sd;
+    for (int s = nrl; s >= 1; --s) {
+        SwLayerRT<R, B> o;
+        sw_layer_rt<R, B>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o);
+        su = (R)0, sd = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const R tdbt_s = o.ld[g] * tdbt[g];
+            R tdn_s, rdnd_s;
+            if (s == nrl) {
+                tdn_s = o.tb[g];
+                rdnd_s = o.rd[g];
+            } else {
+                const R zden1 = (R)1 / ((R)1 - o.rd[g] * rdnd[g]);
+                tdn_s = tdbt[g] * o.tb[g] + (o.td[g] * ((tdn[g] - tdbt[g]) + tdbt[g] * o.rb[g] * rdnd[g])) * zden1;
+                rdnd_s = o.rd[g] + o.td[g] * o.td[g] * rdnd[g] * zden1;
+            }
+            tdbt[g] = tdbt_s, tdn[g] = tdn_s, rdnd[g] = rdnd_s;
+            const R zden1 = (R)1 / ((R)1 - rdnd_s * zrupd[g][s]);
+            const R zfu = (tdbt_s * zrupb[g][s] + (tdn_s - tdbt_s) * zrupd[g][s]) * zden1;
+            const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zrupb[g][s] * rdnd_s) * zden1;
+            su = su + zsolar[g] * zfu;
+            sd = sd + zsolar[g] * zfd;
+        }
+        acc.sw_up[(size_t)(s - 1) * n + t] += (double)su;
+        acc.sw_dn[(size_t)(s - 1) * n + t] += (double)sd;
+    }
+}
+
+// ---- post: FP64 fluxes -> heating rates, top-first outputs -------------------------------------------
+struct OutPtrs {
+    double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn;
+};
+static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, Acc acc, OutPtrs out)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl, nm = in.nlay;
+    const size_t nc = (size_t)in.ncol;
+    const double flxfac = 0.5 * FLUXFAC;
+    double fprev = acc.lw_up[t] * flxfac - acc.lw_dn[t] * flxfac;
+    out.flw_up[(size_t)nrl * nc + i] = acc.lw_up[t] * flxfac;
+    out.flw_dn[(size_t)nrl * nc + i] = acc.lw_dn[t] * flxfac;
+    for (int k = 1; k <= nrl; ++k) {
+        const double fu = acc.lw_up[(size_t)k * n + t] * flxfac, fd = acc.lw_dn[(size_t)k * n + t] * flxfac;
+        out.flw_up[(size_t)(nrl - k) * nc + i] = fu;
+        out.flw_dn[(size_t)(nrl - k) * nc + i] = fd;
+        const double fnet = fu - fd;
+        const int kin = nrl - k;  // surface-first layer k-1
+        const double rfdelp = HEATFAC / (rc.plvl(kin + 1) - rc.plvl(kin));
+        const double h = (fprev - fnet) * rfdelp;
+        if (k - 1 < nm) out.hr_lw[(size_t)(nm - 1 - (k - 1)) * nc + i] = h;
+        fprev = fnet;
+    }
+    if (cs.day[t]) {
+        double fu0 = acc.sw_up[t], fd0 = acc.sw_dn[t];
+        out.fsw_up[(size_t)nrl * nc + i] = fu0;
+        out.fsw_dn[(size_t)nrl * nc + i] = fd0;
+        for (int k = 0; k < nrl; ++k) {
+            const double fu1 = acc.sw_up[(size_t)(k + 1) * n + t], fd1 = acc.sw_dn[(size_t)(k + 1) * n + t];
+            out.fsw_up[(size_t)(nrl - 1 - k) * nc + i] = fu1;
+            out.fsw_dn[(size_t)(nrl - 1 - k) * nc + i] = fd1;
+            const int kin = nrl - 1 - k;
+            const double rf = HEATFAC / (rc.plvl(kin + 1) - rc.plvl(kin));
+            const double h = ((fd1 - fu1) - (fd0 - fu0)) * rf;
+            if (k < nm) out.hr_sw[(size_t)(nm - 1 - k) * nc + i] = h;
+            fu0 = fu1, fd0 = fd1;
+        }
+    } else {
+        for (int L = 0; L <= nrl; ++L) out.fsw_up[(size_t)L * nc + i] = 0.0, out.fsw_dn[(size_t)L * nc + i] = 0.0;
+        for (int k0 = 0; k0 < nm; ++k0) out.hr_sw[(size_t)k0 * nc + i] = 0.0;
+    }
+}
+
+}  // namespace wr

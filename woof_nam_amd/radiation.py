@@ -1,0 +1,156 @@
+"""Column API of the MI355X radiation path over torch device tensors.
+
+``Radiation(device, max_cols).run(cols, params, precision)`` calls the HIP library on the current
+torch stream.  Inputs are float64 device tensors in the column-fastest layout of include/woofrad.h
+([nk][ncol], index 0 = model top); outputs are allocated on the same device (or passed in).
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import torch
+
+from . import _lib, abi
+
+
+def _dptr(t):
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double)) if t is not None else None
+
+
+def _check_field(name, t, shape, dtype, device):
+    if not isinstance(t, torch.Tensor):
+        raise ValueError(f"{name}: expected a torch tensor, got {type(t).__name__}")
+    if t.dtype != dtype:
+        raise ValueError(f"{name}: dtype {t.dtype}, expected {dtype}")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{name}: shape {tuple(t.shape)}, expected {tuple(shape)}")
+    if t.device != device:
+        raise ValueError(f"{name}: on {t.device}, expected {device}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name}: must be contiguous")
+
+
+def input_shapes(ncol: int, nlay: int) -> dict:
+    s = {"p_lev": (nlay + 1, ncol)}
+    for k in ("t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi"):
+        s[k] = (nlay, ncol)
+    for k in ("t_sfc", "emis", "alb", "cosz"):
+        s[k] = (ncol,)
+    return s
+
+
+class Radiation:
+    """A device-resident table set and work space for tiles of up to ``max_cols`` columns."""
+
+    def __init__(self, device: int | torch.device = 0, max_cols: int = 131072, tables=None):
+        L = _lib.load()
+        self.device = torch.device("cuda", device if isinstance(device, int) else device.index or 0)
+        self.max_cols = int(max_cols)
+        h = C.c_void_p()
+        path = str(tables or _lib.TABLES).encode()
+        _lib.check(L.wr_create(C.byref(h), path, self.device.index, self.max_cols), "wr_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.load().wr_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def timing(self, enable: bool = True) -> None:
+        """Bracket every later kernel launch with HIP events (per-launch durations, see timing_read)."""
+        _lib.check(_lib.load().wr_timing(self._h, int(bool(enable))), "wr_timing")
+
+    def timing_read(self, cap: int = 8192) -> list:
+        """[(kernel name, ms), ...] of the launches recorded since timing() or the last read."""
+        L = _lib.load()
+        kid = (C.c_int32 * cap)()
+        ms = (C.c_float * cap)()
+        n = C.c_int32()
+        _lib.check(L.wr_timing_read(self._h, kid, ms, cap, C.byref(n)), "wr_timing_read")
+        return [(L.wr_kernel_name(kid[j]).decode(), float(ms[j])) for j in range(n.value)]
+
+    @staticmethod
+    def default_params(**kw) -> abi.WrParams:
+        p = abi.WrParams()
+        _lib.load().wr_default_params(C.byref(p))
+        for k, v in kw.items():
+            if k == "vmr":
+                for i, x in enumerate(v):
+                    p.vmr[i] = float(x)
+            else:
+                setattr(p, k, v)
+        return p
+
+    def run(self, cols: dict, params: abi.WrParams | None = None, precision: str = "fp64",
+            out: dict | None = None, diag: bool = False, stream: torch.cuda.Stream | None = None):
+        """Radiation call on device columns.  Returns (outputs, diag or None); asynchronous on the stream."""
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        t = cols["t_lay"]
+        nlay, ncol = int(t.shape[0]), int(t.shape[1])
+        for k, shp in input_shapes(ncol, nlay).items():
+            if k not in cols:
+                raise KeyError(f"missing input field {k!r}")
+            _check_field(k, cols[k], shp, torch.float64, self.device)
+        ids = cols.get("col_id")
+        if ids is not None:
+            _check_field("col_id", ids, (ncol,), torch.int64, self.device)
+        if out is None:
+            out = {k: torch.empty(s, dtype=torch.float64, device=self.device)
+                   for k, s in abi.output_shapes(ncol, nlay).items()}
+        else:
+            for k, s in abi.output_shapes(ncol, nlay).items():
+                _check_field(k, out[k], s, torch.float64, self.device)
+        dg = cd = None
+        if diag:
+            dg = {}
+            for k, s in abi.diag_shapes(ncol, nlay).items():
+                dg[k] = torch.zeros(s, dtype=torch.int32, device=self.device)
+            cd = abi.WrDiag(*[C.cast(C.c_void_p(dg[k].data_ptr()),
+                                     C.POINTER(C.c_uint32 if k.endswith("mask") else C.c_int32))
+                              for k in abi.DIAG_FIELDS])
+        ci = abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
+                           C.cast(C.c_void_p(ids.data_ptr()), C.POINTER(C.c_uint64)) if ids is not None else None)
+        co = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
+        params = params or self.default_params()
+        s = stream or torch.cuda.current_stream(self.device)
+        L = _lib.load()
+        fn = L.wr_run_fp64 if precision == "fp64" else L.wr_run_mixed
+        rc = fn(self._h, C.byref(ci), C.byref(params), C.byref(co), C.byref(cd) if cd is not None else None,
+                C.c_void_p(s.cuda_stream))
+        _lib.check(rc, f"wr_run_{precision}")
+        return out, dg  # mask words are returned as int32 bit patterns
+
+
+def cal_cldfra(cldfra: torch.Tensor, qc: torch.Tensor, qi: torch.Tensor | None, i_lo: int, i_hi: int,
+               j_lo: int, j_hi: int, f_qc: bool = True, f_qi: bool = True, stream=None) -> None:
+    """CAL_CLDFRA on [lm][jm][im] float32 device fields (in place on the interior points)."""
+    lm, jm, im = cldfra.shape
+    for nm, t in (("cldfra", cldfra), ("qc", qc)) + ((("qi", qi),) if qi is not None else ()):
+        _check_field(nm, t, (lm, jm, im), torch.float32, cldfra.device)
+    s = stream or torch.cuda.current_stream(cldfra.device)
+    rc = _lib.load().wr_cal_cldfra(im, jm, lm, i_lo, i_hi, j_lo, j_hi, int(bool(f_qc)), int(bool(f_qi)),
+                                   cldfra.data_ptr(), qc.data_ptr(), qi.data_ptr() if qi is not None else None,
+                                   C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "wr_cal_cldfra")
+
+
+def mcica_masks(cldf: torch.Tensor, col_id: torch.Tensor | None, seed: int, iovr: int, stream=None):
+    """McICA bits of the 140 LW and 112 SW g-points for a surface-first cloud fraction [nrl][ncol]."""
+    nrl, ncol = cldf.shape
+    _check_field("cldf", cldf, (nrl, ncol), torch.float64, cldf.device)
+    if col_id is not None:
+        _check_field("col_id", col_id, (ncol,), torch.int64, cldf.device)
+    lw = torch.zeros((nrl, 5, ncol), dtype=torch.int32, device=cldf.device)
+    sw = torch.zeros((nrl, 4, ncol), dtype=torch.int32, device=cldf.device)
+    s = stream or torch.cuda.current_stream(cldf.device)
+    rc = _lib.load().wr_mcica_masks(ncol, nrl, cldf.data_ptr(), col_id.data_ptr() if col_id is not None else None,
+                                    C.c_uint64(seed), iovr, lw.data_ptr(), sw.data_ptr(), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "wr_mcica_masks")
+    return lw, sw
