@@ -135,9 +135,13 @@ def main(argv=None):
     if not a.no_cpu:
         from oracle import pyoracle
         threads = min(16, os.cpu_count() or 1)
-        ns = a.cpu_cols or min(ncol, 12000 * threads)
+        # size the sample for about 15 s of CPU work from a short calibration run
+        cal = {k: np.ascontiguousarray(v[..., :4000]) for k, v in cols.items()}
+        t1 = time.perf_counter()
+        pyoracle.run(cal, nthreads=threads)
+        rate = 4000 / (time.perf_counter() - t1)
+        ns = a.cpu_cols or int(min(ncol, max(4000, 15.0 * rate)))
         sub = {k: np.ascontiguousarray(v[..., :ns]) for k, v in cols.items()}
-        pyoracle.run({k: v[..., :8] for k, v in sub.items()}, nthreads=1)
         t1 = time.perf_counter()
         ref, _ = pyoracle.run(sub, nthreads=threads)
         tc = time.perf_counter() - t1

@@ -460,8 +460,11 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     OutPtrs op;
     op.hr_lw = o->hr_lw, op.hr_sw = o->hr_sw, op.flw_up = o->flw_up, op.flw_dn = o->flw_dn;
     op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn;
-    for (int col0 = 0; col0 < in->ncol; col0 += c->max_cols) {
-        const int n = (in->ncol - col0 < c->max_cols) ? in->ncol - col0 : c->max_cols;
+    // equal tiles of at most max_cols columns (no small ragged tail tile)
+    const int ntiles = (in->ncol + c->max_cols - 1) / c->max_cols;
+    const int per = (in->ncol + ntiles - 1) / ntiles;
+    for (int col0 = 0; col0 < in->ncol; col0 += per) {
+        const int n = (in->ncol - col0 < per) ? in->ncol - col0 : per;
         rc = wr_launch_fp64(c, ci, par, op, col0, n, s);
         if (rc != WR_OK) return rc;
         if (dg) {
