@@ -57,6 +57,7 @@ def main(argv=None):
     ap.add_argument("--ny", type=int, default=835)
     ap.add_argument("--cpu-cols", type=int, default=0, help="CPU baseline sample (0 = sized from cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--acc-cols", type=int, default=65536, help="mixed-precision error-budget sample")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -131,6 +132,30 @@ def main(argv=None):
                      "launch_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
                      "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
     }
+
+    if a.precision == "mixed":
+        # error budget against the FP64 path (bit-identical to the CPU oracle) on a column sample,
+        # per layer and per band
+        from woof_nam_amd import accuracy
+        ns = min(ncol, a.acc_cols)
+        idx = torch.linspace(0, ncol - 1, ns, device=dev).long()
+        sub = {k: v[..., idx].contiguous() for k, v in d.items()}
+        rep = accuracy.report(rad, sub, per_band=True)
+        with open(os.path.join(ROOT, "gpurun_out", "accuracy_mixed.txt") if os.path.isdir(
+                os.path.join(ROOT, "gpurun_out")) else os.devnull, "w") as f:
+            f.write(accuracy.format_report(rep) + "\n")
+        worst_lw = max(rep["lw_by_band"].items(), key=lambda kv: kv[1]["flux_rms_wm2"])
+        worst_sw = max(rep["sw_by_band"].items(), key=lambda kv: kv[1]["flux_rms_wm2"])
+        lay_lw = int(np.argmax(rep["hr_lw_rms_by_layer_kday"]))
+        lay_sw = int(np.argmax(rep["hr_sw_rms_by_layer_kday"]))
+        res["accuracy_vs_fp64"] = {
+            "sample_columns": ns,
+            **{k: rep[k] for k in rep if k.endswith("_rms_kday") or k.endswith("_rms_wm2") or k.endswith("_max_kday")},
+            "worst_lw_band": {"band": worst_lw[0], **worst_lw[1]},
+            "worst_sw_band": {"band": worst_sw[0], **worst_sw[1]},
+            "worst_layer_hr_lw": {"layer": lay_lw, "rms_kday": rep["hr_lw_rms_by_layer_kday"][lay_lw]},
+            "worst_layer_hr_sw": {"layer": lay_sw, "rms_kday": rep["hr_sw_rms_by_layer_kday"][lay_sw]},
+        }
 
     if not a.no_cpu:
         from oracle import pyoracle

@@ -118,6 +118,12 @@ int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const w
 int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
                  const wr_diag *diag, void *stream);
 
+/* One band alone, for per-band error budgets (no counterpart in the reference): as wr_run_fp64
+ * (mixed = 0) or wr_run_mixed (mixed = 1), but only band `band` contributes to the fluxes and
+ * heating rates: 0..15 = LW bands 1..16 (SW outputs 0), 16..29 = SW bands 16..29 (LW outputs 0). */
+int wr_run_band(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
+                int32_t band, int32_t mixed, void *stream);
+
 /* CAL_CLDFRA (phys_rad_cal_cldfra.cu:29-55): cldfra = 1 where qc(+qi) > 1e-6, else 0, on the
  * interior points i in [i_lo, i_hi], j in [j_lo, j_hi] of [lm][jm][im] float32 fields; other
  * points untouched.  f_qc / f_qi are the microphysics flags. */

@@ -1,0 +1,72 @@
+"""Mixed-precision path against the FP64 oracle.
+
+Bar (stated in W/m2 and K/day): heating-rate RMS <= 0.05 K/day and flux RMS <= 0.5 W/m2 over all
+levels and columns; every column's worst heating-rate error is also bounded.  Table indices and
+McICA bits come from FP64 arithmetic in both paths and must be bit-identical.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import pyoracle
+from woof_nam_amd import abi, gen
+from woof_nam_amd.radiation import Radiation
+
+pytestmark = pytest.mark.gpu
+
+KDAY = 86400.0
+HR_RMS_KDAY = 0.05
+FLUX_RMS_WM2 = 0.5
+
+
+@pytest.fixture(scope="module")
+def rad(gpu_device):
+    r = Radiation(0, max_cols=4096)
+    yield r
+    r.close()
+
+
+def errors(out, ref):
+    e = {}
+    for k in ("hr_lw", "hr_sw"):
+        d = out[k].cpu().numpy() - ref[k]
+        e[k] = (float(np.sqrt(np.mean(d * d)) * KDAY), float(np.abs(d).max() * KDAY))
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        d = out[k].cpu().numpy() - ref[k]
+        e[k] = (float(np.sqrt(np.mean(d * d))), float(np.abs(d).max()))
+    return e
+
+
+@pytest.mark.parametrize("seed", [31, 32])
+def test_mixed_within_tolerance(rad, seed):
+    cols = gen.columns(3000, seed=seed)
+    print(f"case mixed ncol=3000 seed={seed}", flush=True)
+    ref, rdg = pyoracle.run(cols, diag=True)
+    dev = {k: torch.from_numpy(v).to(rad.device) for k, v in cols.items()}
+    out, dg = rad.run(dev, precision="mixed", diag=True)
+    torch.cuda.synchronize()
+    e = errors(out, ref)
+    print("RMS / max error (K/day, W/m2):", e)
+    for k in ("hr_lw", "hr_sw"):
+        assert e[k][0] <= HR_RMS_KDAY, (k, e[k])
+        assert e[k][1] <= 20 * HR_RMS_KDAY, (k, e[k])
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])
+    for k in abi.DIAG_FIELDS:
+        assert np.array_equal(dg[k].cpu().numpy().view(np.uint32), rdg[k].view(np.uint32)), k
+
+
+def test_mixed_grid_sample(rad):
+    """Grid columns (smooth fields, 12 UTC sun) including the low-sun edge of the domain."""
+    cols = gen.grid(120, 100, seed=3)
+    print("case mixed grid 120x100", flush=True)
+    ref, _ = pyoracle.run(cols)
+    dev = {k: torch.from_numpy(v).to(rad.device) for k, v in cols.items()}
+    out, _ = rad.run(dev, precision="mixed")
+    torch.cuda.synchronize()
+    e = errors(out, ref)
+    print("RMS / max error (K/day, W/m2):", e)
+    for k in ("hr_lw", "hr_sw"):
+        assert e[k][0] <= HR_RMS_KDAY, (k, e[k])
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])

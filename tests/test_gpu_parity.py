@@ -125,3 +125,19 @@ def test_bad_arguments(rad):
         rad.run(cols, params=Radiation.default_params(mcica=0))
     with pytest.raises(ValueError):
         rad.run(cols, params=Radiation.default_params(iovr=3))
+
+
+def test_band_runs_sum_to_full_call(rad):
+    """wr_run_band: the 16 LW and 14 SW single-band calls add up to the full call."""
+    cols = gen.columns(300, seed=44)
+    dev = to_dev(cols, rad.device)
+    full, _ = rad.run(dev)
+    acc = {k: torch.zeros_like(v) for k, v in full.items()}
+    for b in range(30):
+        o = rad.run_band(dev, b)
+        for k in acc:
+            acc[k] += o[k]
+    torch.cuda.synchronize()
+    for k in abi.OUTPUT_FIELDS:
+        a, b = acc[k].cpu().numpy(), full[k].cpu().numpy()
+        np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-9 * (np.abs(b).max() + 1e-300), err_msg=k)

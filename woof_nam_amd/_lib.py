@@ -14,7 +14,7 @@ LIB_PATH = Path(__file__).resolve().parent / "libwoofrad.so"
 TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 
 # symbols include/woofrad.h declares (the CPU test suite checks every one is exported)
-EXPORTS = ("wr_default_params", "wr_create", "wr_run_fp64", "wr_run_mixed", "wr_cal_cldfra",
+EXPORTS = ("wr_default_params", "wr_create", "wr_run_fp64", "wr_run_mixed", "wr_run_band", "wr_cal_cldfra",
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy")
 
 _ERRORS = {
@@ -46,6 +46,9 @@ def load():
         getattr(L, f).argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams),
                                   C.POINTER(abi.WrOutputs), C.POINTER(abi.WrDiag), vp]
         getattr(L, f).restype = C.c_int
+    L.wr_run_band.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
+                              C.c_int32, C.c_int32, vp]
+    L.wr_run_band.restype = C.c_int
     L.wr_cal_cldfra.argtypes = [C.c_int32] * 9 + [vp, vp, vp, vp]
     L.wr_cal_cldfra.restype = C.c_int
     L.wr_mcica_masks.argtypes = [C.c_int32, C.c_int32, vp, vp, C.c_uint64, C.c_int32, vp, vp, vp]

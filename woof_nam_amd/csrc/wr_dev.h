@@ -58,6 +58,7 @@ template <typename R> struct Tabs {
     const R *totplnk, *delwave, *absliq1, *absice1;
     const int32_t *ipat, *layreffr, *ix1, *ix2, *ibx;
     const R *exp_tbl, *tfn_tbl, *tau_tbl, *sw_exp_tbl;
+    const double *sw_exp_tbl64;  // FP64 SW exp table (layers that take the FP64 R/T branch)
     const R *sf01, *sf02, *sf03, *sw_strrat, *sw_specwt;
     const R *extliq1, *ssaliq1, *asyliq1, *extice3, *ssaice3, *asyice3;
     R scalekur;

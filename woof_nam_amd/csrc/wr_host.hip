@@ -447,12 +447,11 @@ RunPar to_par(const wr_params *p)
 }
 
 int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o, const wr_diag *dg, void *stream,
-        bool mixed)
+        bool mixed, int band = -1)
 {
     int rc = check_args(c, in, p, o);
     if (rc != WR_OK || in->ncol == 0) return rc;
     if (p->mcica != 1) return WR_ERR_UNSUPPORTED;
-    if (mixed) return WR_ERR_UNSUPPORTED;
     DevGuard g(c->device);
     hipStream_t s = (hipStream_t)stream;
     const ColIn ci = to_colin(in);
@@ -465,7 +464,7 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     const int per = (in->ncol + ntiles - 1) / ntiles;
     for (int col0 = 0; col0 < in->ncol; col0 += per) {
         const int n = (in->ncol - col0 < per) ? in->ncol - col0 : per;
-        rc = wr_launch_fp64(c, ci, par, op, col0, n, s);
+        rc = mixed ? wr_launch_mixed(c, ci, par, op, col0, n, band, s) : wr_launch_fp64(c, ci, par, op, col0, n, band, s);
         if (rc != WR_OK) return rc;
         if (dg) {
             tbeg(c, KID_DIAG, s);
@@ -498,6 +497,14 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     c->nl_max = 64;  // McICA bits of a g-point over the layers are one 64-bit word
     rc = upload<double>(f, &c->tab_buf_d, c->host_d, &c->tabs_d);
     if (rc == WR_OK) rc = upload<float>(f, &c->tab_buf_f, c->host_f, &c->tabs_f);
+    if (rc == WR_OK) {
+        // both precisions reach the FP64 SW exp table (the mixed path's FP64 R/T branch)
+        c->host_d.sw_exp_tbl64 = c->host_d.sw_exp_tbl;
+        c->host_f.sw_exp_tbl64 = c->host_d.sw_exp_tbl;
+        if (hipMemcpy(c->tabs_d, &c->host_d, sizeof(Tabs<double>), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->tabs_f, &c->host_f, sizeof(Tabs<float>), hipMemcpyHostToDevice) != hipSuccess)
+            rc = WR_ERR_HIP;
+    }
     if (rc == WR_OK) rc = wr_init_luts(c, nullptr);
     if (rc == WR_OK) {
         const double *src[4] = {c->host_d.exp_tbl, c->host_d.tfn_tbl, c->host_d.tau_tbl, c->host_d.sw_exp_tbl};
@@ -535,6 +542,13 @@ int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const 
                  void *stream)
 {
     return run(ctx, in, par, out, diag, stream, true);
+}
+
+int wr_run_band(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, int32_t band,
+                int32_t mixed, void *stream)
+{
+    if (band < 0 || band >= WR_NBANDS_LW + WR_NBANDS_SW) return WR_ERR_ARG;
+    return run(ctx, in, par, out, nullptr, stream, mixed != 0, band);
 }
 
 int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi, int32_t j_lo, int32_t j_hi,

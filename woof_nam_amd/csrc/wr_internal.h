@@ -74,9 +74,9 @@ inline Acc tile_acc(wr_ctx *c, int n, int nrl)
 
 // launch units
 int wr_launch_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
-                   hipStream_t s);
+                   int band, hipStream_t s);
 int wr_launch_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
-                    hipStream_t s);
+                    int band, hipStream_t s);
 int wr_launch_diag(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
                    hipStream_t s);
 int wr_init_luts(wr_ctx *c, hipStream_t s);

@@ -323,6 +323,9 @@ __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn 
 }
 
 // ---- SW band kernel (McICA two-stream) -------------------------------------------------------------
+// single-scattering albedo above which the mixed path evaluates the SW layer R/T in FP64
+constexpr double W0_FP64 = 0.999999;
+
 template <typename R, int B> struct SwLayerRT {
     R rb[SW_NG[B]], rd[SW_NG[B]], tb[SW_NG[B]], td[SW_NG[B]], ld[SW_NG[B]];
 };
@@ -342,10 +345,24 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
     if (cf1 > 0.0 && L.ct > FTINY) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
+        const bool cld = (mb[g] >> k) & 1;
+        if constexpr (sizeof(R) < sizeof(double)) {
+            // near-conservative scattering (w0 > 1 - 1e-6): 1 - w0 and gamma1 - gamma2 cancel in FP32,
+            // so these layers take the FP64 R/T from the FP32 optical properties, summed in FP64
+            const double t0 = fmax(FTINY, (double)tr[g] + (double)tg[g] + (cld ? (double)ctau : 0.0));
+            const double s0 = (double)tr[g] + (cld ? (double)cssa : 0.0);
+            if (s0 > W0_FP64 * t0) {
+                double rb, rd, tb, td, ld;
+                layer_rt<double>(t0, s0, cld ? (double)casy : 0.0, (double)cz, (double)snt, T->sw_exp_tbl64, rb, rd,
+                                 tb, td, ld);
+                o.rb[g] = (R)rb, o.rd[g] = (R)rd, o.tb[g] = (R)tb, o.td[g] = (R)td, o.ld[g] = (R)ld;
+                continue;
+            }
+        }
         R ztau0 = fmax((R)FTINY, tr[g] + tg[g] + (R)0);
         R zssa0 = tr[g] + (R)0;
         R zasy0 = (R)0;
-        if ((mb[g] >> k) & 1) {
+        if (cld) {
             ztau0 = ztau0 + ctau;
             zssa0 = zssa0 + cssa;
             zasy0 = zasy0 + casy;

@@ -404,10 +404,14 @@ __device__ __forceinline__ void layer_rt(R ztau0, R zssa0, R zasy0, R cz, R snt,
     const R zt1 = zrp1 * (za1 + zrkg4);
     const R zt2 = zrm1 * (za1 - zrkg4);
     const R zt3 = zrk2 * (zgam4 + za1 * cz);
-    const R zb1 = fmin(zrk * ztau1, (R)500);
+    // FP32: cap the exponents at 80 so exp(-zb) stays a normal number and 1/exp(-zb) finite (the
+    // FP32 table underflows to 0 past about 103); the R/T expressions depend on the capped terms
+    // only through factors below exp(-160)
+    const R zcap = (sizeof(R) < sizeof(double)) ? (R)80 : (R)500;
+    const R zb1 = fmin(zrk * ztau1, zcap);
     const R zexm1 = expm<R>(zb1, tbl);
     const R zexp1 = (R)1 / zexm1;
-    const R zb2 = fmin(ztau1 * snt, (R)500);
+    const R zb2 = fmin(ztau1 * snt, zcap);
     const R zexm2 = expm<R>(zb2, tbl);
     const R zexp2 = (R)1 / zexm2;
     const R ze1r45 = zr4 * zexp1 + zr5 * zexm1;

@@ -62,6 +62,27 @@ class Radiation:
         except Exception:
             pass
 
+    def run_band(self, cols: dict, band: int, params: abi.WrParams | None = None, precision: str = "fp64",
+                 out: dict | None = None, stream=None) -> dict:
+        """Fluxes and heating rates of one band alone: 0..15 = LW bands 1..16, 16..29 = SW bands 16..29."""
+        t = cols["t_lay"]
+        nlay, ncol = int(t.shape[0]), int(t.shape[1])
+        for k, shp in input_shapes(ncol, nlay).items():
+            _check_field(k, cols[k], shp, torch.float64, self.device)
+        if out is None:
+            out = {k: torch.empty(s, dtype=torch.float64, device=self.device)
+                   for k, s in abi.output_shapes(ncol, nlay).items()}
+        ids = cols.get("col_id")
+        ci = abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
+                           C.cast(C.c_void_p(ids.data_ptr()), C.POINTER(C.c_uint64)) if ids is not None else None)
+        co = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
+        params = params or self.default_params()
+        s = stream or torch.cuda.current_stream(self.device)
+        rc = _lib.load().wr_run_band(self._h, C.byref(ci), C.byref(params), C.byref(co), int(band),
+                                     int(precision == "mixed"), C.c_void_p(s.cuda_stream))
+        _lib.check(rc, "wr_run_band")
+        return out
+
     def timing(self, enable: bool = True) -> None:
         """Bracket every later kernel launch with HIP events (per-launch durations, see timing_read)."""
         _lib.check(_lib.load().wr_timing(self._h, int(bool(enable))), "wr_timing")
