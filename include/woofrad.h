@@ -57,6 +57,8 @@ typedef struct {
     const double *alb;       /* [ncol] surface SW albedo (beam/diffuse, visible/near-IR)          */
     const double *cosz;      /* [ncol] cosine of the solar zenith angle (night when < 1e-4)      */
     const uint64_t *col_id;  /* [ncol] global column index that keys the McICA stream; NULL = i  */
+    const uint64_t *seed_dev;/* optional device pointer to the McICA seed; when set it replaces
+                                wr_params.seed (lets a captured HIP graph be replayed per request) */
 } wr_columns;
 
 /* Call parameters (NAM values by default; see wr_default_params). */
@@ -136,6 +138,36 @@ int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi
  * be NULL).  Bits equal those wr_run_* use. */
 int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t *col_id,
                    uint64_t seed, int32_t iovr, uint32_t *lw_mask, uint32_t *sw_mask, void *stream);
+
+/* ---- gas-optics surrogate (no counterpart in the reference: a learned replacement of taugb/taumol)
+ *
+ * wr_layer_optics: per radiation layer (surface-first k, column i, sample s = k*ncol + i) of the
+ * plain FP64 path: feat[s][8] = ln p (hPa), T (K), ln q, ln o3, dp (hPa), 0, 0, 0 and, when targ is
+ * not NULL, targ[s][504] = ln tau_LW (140 g), Planck fraction (140), ln tau_SW gas+Rayleigh (112),
+ * SW single-scattering albedo (112).  Device pointers, float32.
+ *
+ * wr_mlp_gemm: C[M x N] = A[M x K] * B[K x N] on BF16 MFMA with FP32 accumulation; element (m,k)
+ * of A at A[m*lda_m + k*lda_k] (float32, or bfloat16 when a_bf16), B likewise, C (float32, may be
+ * NULL) and/or Cb (bfloat16, may be NULL) at m*ldc_m + n*ldc_n.  epilogue: 0 store, 1 + bias[n],
+ * 2 + bias then SiLU (pre-activation to Zout), 3 multiply by SiLU'(Zin).  splits > 1 (epilogue 0,
+ * C only): split-K partial sums at C + s*split_stride, summed by wr_mlp_splitk_reduce.
+ * wr_mlp_colsum: out[n] = sum_m X[m*ld + n].  wr_mlp_mse: dY = 2 (Y - T) scale, per-block sums of
+ * (Y - T)^2 in partial[nblocks].  wr_mlp_adam: Adam step `step` (1-based) on FP32 weights, with an
+ * optional BF16 copy wb.  wr_mlp_to_bf16: element-wise conversion. */
+int wr_layer_optics(wr_ctx *ctx, const wr_columns *in, const wr_params *par, float *feat, float *targ,
+                    void *stream);
+int wr_mlp_gemm(int32_t M, int32_t N, int32_t K, const void *A, int32_t a_bf16, int64_t lda_m,
+                int64_t lda_k, const void *B, int32_t b_bf16, int64_t ldb_k, int64_t ldb_n, float *C,
+                void *Cb, int64_t ldc_m, int64_t ldc_n, const float *bias, float *Zout,
+                const float *Zin, int32_t epilogue, int32_t splits, int64_t split_stride, void *stream);
+int wr_mlp_splitk_reduce(const float *part, int32_t splits, int64_t stride, float *out, int64_t n,
+                         void *stream);
+int wr_mlp_colsum(const float *X, int32_t M, int32_t N, int64_t ld, float *out, void *stream);
+int wr_mlp_mse(const float *Y, const float *T, int64_t n, float scale, float *dY, float *partial,
+               int32_t nblocks, void *stream);
+int wr_mlp_adam(float *w, void *wb, const float *grad, float *m1, float *m2, int64_t n, float lr,
+                float b1, float b2, float eps, int32_t step, void *stream);
+int wr_mlp_to_bf16(const float *a, void *b, int64_t n, void *stream);
 
 /* Per-launch kernel timing for measurement (no counterpart in the reference: the reference
  * publishes call-level times only).  wr_timing(ctx, 1) creates an event pool once and brackets

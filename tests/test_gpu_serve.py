@@ -1,0 +1,101 @@
+"""Served radiation: continuous batching, HBM-budget admission, graph replays.
+
+Each request's result must be the same alone and in a mix of other requests (bit for bit: every
+column depends only on its inputs, column id and seed), and the same as a direct call.
+"""
+import numpy as np
+import pytest
+import torch
+
+from woof_nam_amd import abi, gen
+from woof_nam_amd.radiation import Radiation
+from woof_nam_amd.serve import Request, Server, poisson_run
+
+pytestmark = pytest.mark.gpu
+
+
+def dev_cols(n, seed, device):
+    c = gen.columns(n, seed=seed)
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in c.items()}
+
+
+def same(a, b):
+    return all(torch.equal(a[k], b[k]) for k in abi.OUTPUT_FIELDS)
+
+
+@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+def test_alone_equals_mix_equals_direct(gpu_device, precision):
+    print(f"case serve alone-vs-mix {precision}", flush=True)
+    a_cols = dev_cols(1500, 1, gpu_device)
+    ids = torch.arange(10_000, 11_500, dtype=torch.int64, device=gpu_device)
+    srv = Server(0, tile=1024, slots=3, precision=precision)
+    a1 = Request(rid=0, kind="a", cols=a_cols, seed=5, col_id=ids)
+    srv.submit(a1)
+    srv.drain()
+    mix = [Request(rid=1, kind="b", cols=dev_cols(5000, 2, gpu_device), seed=9),
+           Request(rid=2, kind="a", cols=a_cols, seed=5, col_id=ids),
+           Request(rid=3, kind="c", cols=dev_cols(300, 3, gpu_device), seed=1),
+           Request(rid=4, kind="a", cols=a_cols, seed=6, col_id=ids)]
+    for r in mix:
+        srv.submit(r)
+    srv.drain()
+    torch.cuda.synchronize()
+    assert same(a1.out, mix[1].out)
+    assert not torch.equal(a1.out["hr_sw"], mix[3].out["hr_sw"]) or not torch.equal(a1.out["hr_lw"], mix[3].out["hr_lw"])
+    rad = Radiation(0, max_cols=4096)
+    direct, _ = rad.run({**a_cols, "col_id": ids}, params=Radiation.default_params(seed=5), precision=precision)
+    torch.cuda.synchronize()
+    assert same(a1.out, direct)
+    rad.close()
+    srv.close()
+
+
+def test_graph_replays_match_eager(gpu_device):
+    """Several replays with eager work and synchronisations in between all equal the eager call."""
+    print("case serve graph replays", flush=True)
+    cols = dev_cols(2048, 4, gpu_device)
+    srv_g = Server(0, tile=2048, slots=1, precision="mixed", graphs=True)
+    srv_e = Server(0, tile=2048, slots=1, precision="mixed", graphs=False)
+    ref = Request(rid=0, kind="x", cols=cols, seed=3)
+    srv_e.submit(ref)
+    srv_e.drain()
+    for rep in range(4):
+        r = Request(rid=rep, kind="x", cols=cols, seed=3)
+        srv_g.submit(r)
+        srv_g.drain()
+        torch.cuda.synchronize()
+        assert same(r.out, ref.out), f"replay {rep} differs"
+        _ = (cols["t_lay"] * 2.0).sum().item()      # eager work between replays
+    srv_g.close()
+    srv_e.close()
+
+
+def test_budget_admission(gpu_device):
+    print("case serve budget", flush=True)
+    cols = dev_cols(3000, 5, gpu_device)
+    one = 3000 * 8 * (6 * 60 + 61 + 4 + 2 * 60 + 4 * 62)
+    srv = Server(0, tile=1024, slots=2, budget_gb=1.5 * one / 1e9, precision="mixed")
+    reqs = [Request(rid=j, kind="x", cols=cols, seed=j) for j in range(3)]
+    for r in reqs:
+        srv.submit(r)
+    peak = 0
+    while not srv.idle():
+        srv.poll()
+        peak = max(peak, srv.inflight_bytes)
+    assert peak <= 1.5 * one
+    assert all(r.done is not None for r in reqs)
+    assert reqs[0].done <= reqs[1].admitted + 1e-9     # the second waited for the first to leave
+    with pytest.raises(MemoryError):
+        srv.submit(Request(rid=9, kind="big", cols=dev_cols(6000, 6, gpu_device)))
+    srv.close()
+
+
+def test_poisson_stream_reports_latency(gpu_device):
+    print("case serve poisson", flush=True)
+    kinds = {"small": (dev_cols(700, 7, gpu_device), None), "large": (dev_cols(3000, 8, gpu_device), None)}
+    srv = Server(0, tile=1024, slots=2, precision="mixed")
+    res = poisson_run(srv, kinds, rate=50.0, duration=1.0, seed=1)
+    srv.close()
+    assert res["requests"] > 10
+    assert res["p99_ms"] >= res["p50_ms"] > 0
+    assert res["columns_per_s"] > 0

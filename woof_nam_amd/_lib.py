@@ -15,7 +15,9 @@ TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 
 # symbols include/woofrad.h declares (the CPU test suite checks every one is exported)
 EXPORTS = ("wr_default_params", "wr_create", "wr_run_fp64", "wr_run_mixed", "wr_run_band", "wr_cal_cldfra",
-           "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy")
+           "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
+           "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
+           "wr_mlp_to_bf16")
 
 _ERRORS = {
     abi.WR_ERR_ARG: ValueError,
@@ -59,6 +61,18 @@ def load():
     L.wr_timing_read.restype = C.c_int
     L.wr_kernel_name.argtypes = [C.c_int32]
     L.wr_kernel_name.restype = C.c_char_p
+    i32, i64, f32 = C.c_int32, C.c_int64, C.c_float
+    L.wr_layer_optics.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), vp, vp, vp]
+    L.wr_mlp_gemm.argtypes = [i32, i32, i32, vp, i32, i64, i64, vp, i32, i64, i64, vp, vp, i64, i64, vp, vp, vp,
+                              i32, i32, i64, vp]
+    L.wr_mlp_splitk_reduce.argtypes = [vp, i32, i64, vp, i64, vp]
+    L.wr_mlp_colsum.argtypes = [vp, i32, i32, i64, vp, vp]
+    L.wr_mlp_mse.argtypes = [vp, vp, i64, f32, vp, vp, i32, vp]
+    L.wr_mlp_adam.argtypes = [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
+    L.wr_mlp_to_bf16.argtypes = [vp, vp, i64, vp]
+    for f in ("wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
+              "wr_mlp_to_bf16"):
+        getattr(L, f).restype = C.c_int
     L.wr_strerror.argtypes = [C.c_int]
     L.wr_strerror.restype = C.c_char_p
     L.wr_destroy.argtypes = [vp]

@@ -70,6 +70,7 @@ template <typename R> struct Tabs {
 struct ColIn {
     const double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
     const uint64_t *col_id;
+    const uint64_t *seed_dev;  // optional device copy of the McICA seed
     int ncol;  // stride of the [k][column] fields
     int nlay;  // model layers
 };

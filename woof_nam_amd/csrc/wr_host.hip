@@ -328,12 +328,13 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
     if (dg.lw_laytrop) dg.lw_laytrop[i] = cs.lw_laytrop[t];
     if (dg.sw_laytrop) dg.sw_laytrop[i] = day ? cs.sw_laytrop[t] : -1;
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     if (dg.lw_mask) {
         const int lcf1 = cs.lcf1[t];
         for (int k = 0; k < nrl; ++k)
             for (int w = 0; w < 5; ++w) dg.lw_mask[((size_t)k * 5 + w) * nc + i] = 0u;
         for (int g = 0; g < 140; ++g) {
-            const uint64_t mb = mcica_g(col, par.seed, 0, g, nrl, par.iovr,
+            const uint64_t mb = mcica_g(col, seed, 0, g, nrl, par.iovr,
                                         [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
             for (int k = 0; k < nrl; ++k)
                 if ((mb >> k) & 1) dg.lw_mask[((size_t)k * 5 + (g >> 5)) * nc + i] |= 1u << (g & 31);
@@ -345,7 +346,7 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
         if (day) {
             const double cf1 = cs.zcf1[t];
             for (int g = 0; g < 112; ++g) {
-                const uint64_t mb = mcica_g(col, par.seed, 1, g, nrl, par.iovr, [&](int k) {
+                const uint64_t mb = mcica_g(col, seed, 1, g, nrl, par.iovr, [&](int k) {
                     const double ct = rc.cfrac(nrl - 1 - k);
                     return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
                 });
@@ -433,7 +434,7 @@ ColIn to_colin(const wr_columns *in)
     ColIn ci;
     ci.p_lev = in->p_lev, ci.t_lay = in->t_lay, ci.q_lay = in->q_lay, ci.o3_lay = in->o3_lay;
     ci.cldf = in->cldf, ci.qc = in->qc, ci.qi = in->qi, ci.t_sfc = in->t_sfc, ci.emis = in->emis;
-    ci.alb = in->alb, ci.cosz = in->cosz, ci.col_id = in->col_id;
+    ci.alb = in->alb, ci.cosz = in->cosz, ci.col_id = in->col_id, ci.seed_dev = in->seed_dev;
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     return ci;
 }

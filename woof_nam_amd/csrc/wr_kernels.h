@@ -213,9 +213,10 @@ __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn 
     const double sfe = in.emis[i];
     const R semis = (R)((sfe > EPS && sfe <= 1.0) ? sfe : 1.0);
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     uint64_t mb[NG];
     for (int g = 0; g < NG; ++g)
-        mb[g] = mcica_g(col, par.seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+        mb[g] = mcica_g(col, seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
 
     // ---- down sweep, top to surface ----
     R rad[NG];
@@ -391,11 +392,12 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     const double cf1 = cs.zcf1[t];
     const int lt = cs.sw_laytrop[t];
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     uint64_t mb[NG];
     R zsolar[NG];
     for (int g = 0; g < NG; ++g) {
         zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
-        mb[g] = mcica_g(col, par.seed, 1, NS + g, nrl, par.iovr, [&](int k) {
+        mb[g] = mcica_g(col, seed, 1, NS + g, nrl, par.iovr, [&](int k) {
             const double ct = rc.cfrac(nrl - 1 - k);
             return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
         });

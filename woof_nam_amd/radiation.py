@@ -39,6 +39,19 @@ def input_shapes(ncol: int, nlay: int) -> dict:
     return s
 
 
+def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
+    """WrColumns over device tensors; optional col_id [ncol] and seed_dev [1] are int64 tensors."""
+    ids, sd = cols.get("col_id"), cols.get("seed_dev")
+    if ids is not None:
+        _check_field("col_id", ids, (ncol,), torch.int64, device)
+    if sd is not None:
+        _check_field("seed_dev", sd, (1,), torch.int64, device)
+    u64 = C.POINTER(C.c_uint64)
+    return abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
+                         C.cast(C.c_void_p(ids.data_ptr()), u64) if ids is not None else None,
+                         C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None)
+
+
 class Radiation:
     """A device-resident table set and work space for tiles of up to ``max_cols`` columns."""
 
@@ -72,9 +85,7 @@ class Radiation:
         if out is None:
             out = {k: torch.empty(s, dtype=torch.float64, device=self.device)
                    for k, s in abi.output_shapes(ncol, nlay).items()}
-        ids = cols.get("col_id")
-        ci = abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
-                           C.cast(C.c_void_p(ids.data_ptr()), C.POINTER(C.c_uint64)) if ids is not None else None)
+        ci = _colstruct(cols, ncol, nlay, self.device)
         co = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
         params = params or self.default_params()
         s = stream or torch.cuda.current_stream(self.device)
@@ -119,9 +130,6 @@ class Radiation:
             if k not in cols:
                 raise KeyError(f"missing input field {k!r}")
             _check_field(k, cols[k], shp, torch.float64, self.device)
-        ids = cols.get("col_id")
-        if ids is not None:
-            _check_field("col_id", ids, (ncol,), torch.int64, self.device)
         if out is None:
             out = {k: torch.empty(s, dtype=torch.float64, device=self.device)
                    for k, s in abi.output_shapes(ncol, nlay).items()}
@@ -136,8 +144,7 @@ class Radiation:
             cd = abi.WrDiag(*[C.cast(C.c_void_p(dg[k].data_ptr()),
                                      C.POINTER(C.c_uint32 if k.endswith("mask") else C.c_int32))
                               for k in abi.DIAG_FIELDS])
-        ci = abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
-                           C.cast(C.c_void_p(ids.data_ptr()), C.POINTER(C.c_uint64)) if ids is not None else None)
+        ci = _colstruct(cols, ncol, nlay, self.device)
         co = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
         params = params or self.default_params()
         s = stream or torch.cuda.current_stream(self.device)
