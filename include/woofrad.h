@@ -149,11 +149,30 @@ int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t
  * wr_mlp_gemm: C[M x N] = A[M x K] * B[K x N] on BF16 MFMA with FP32 accumulation; element (m,k)
  * of A at A[m*lda_m + k*lda_k] (float32, or bfloat16 when a_bf16), B likewise, C (float32, may be
  * NULL) and/or Cb (bfloat16, may be NULL) at m*ldc_m + n*ldc_n.  epilogue: 0 store, 1 + bias[n],
- * 2 + bias then SiLU (pre-activation to Zout), 3 multiply by SiLU'(Zin).  splits > 1 (epilogue 0,
+ * 2 + bias then SiLU (pre-activation to Zout when not NULL), 3 multiply by SiLU'(Zin).  splits > 1 (epilogue 0,
  * C only): split-K partial sums at C + s*split_stride, summed by wr_mlp_splitk_reduce.
  * wr_mlp_colsum: out[n] = sum_m X[m*ld + n].  wr_mlp_mse: dY = 2 (Y - T) scale, per-block sums of
  * (Y - T)^2 in partial[nblocks].  wr_mlp_adam: Adam step `step` (1-based) on FP32 weights, with an
  * optional BF16 copy wb.  wr_mlp_to_bf16: element-wise conversion. */
+/* Surrogate weights and normalisation (device pointers, owned by the caller, kept by the context):
+ * y = W3 SiLU(W2 SiLU(W1 x + b1) + b2) + b3, x = (feature - fmean) / fstd (5 features, padded to
+ * 8 with zeros), raw output = y * tstd + tmean. */
+typedef struct {
+    const void *w1, *w2, *w3;                   /* bfloat16 [128][8], [128][128], [504][128]   */
+    const float *b1, *b2, *b3;                  /* [128], [128], [504]                          */
+    const float *fmean, *fstd, *fmin, *fmax;    /* [5]: normalisation and training envelope     */
+    const float *tmean, *tstd;                  /* [504]                                        */
+} wr_surrogate;
+
+/* Attach a surrogate to a context and allocate its tile work space (about 2.6 KB per layer-column
+ * of max_cols x 64 layers).  wr_run_surrogate: the mixed-precision call with the LW and SW gas
+ * optics (taugb/taumol) replaced by the network for every column whose layers all lie inside the
+ * training envelope; other columns use the tables.  col_flag (optional, device, [ncol]) receives 1
+ * for columns that fell back to the tables. */
+int wr_surrogate_attach(wr_ctx *ctx, const wr_surrogate *s);
+int wr_run_surrogate(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
+                     int32_t *col_flag, void *stream);
+
 int wr_layer_optics(wr_ctx *ctx, const wr_columns *in, const wr_params *par, float *feat, float *targ,
                     void *stream);
 int wr_mlp_gemm(int32_t M, int32_t N, int32_t K, const void *A, int32_t a_bf16, int64_t lda_m,

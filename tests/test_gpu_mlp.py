@@ -5,7 +5,7 @@ import pytest
 import torch
 
 from oracle import mlp as omlp
-from woof_nam_amd import gen, surrogate as S
+from woof_nam_amd import abi, gen, surrogate as S
 from woof_nam_amd.radiation import Radiation
 
 pytestmark = pytest.mark.gpu
@@ -94,4 +94,32 @@ def test_training_reduces_loss_on_rrtmg_layers(gpu_device):
     assert hist[-1][1] < 0.2 * hist[0][1], hist
     Y, inside = S.predict(m, feat)
     assert bool(inside.all())
+    rad.close()
+
+
+def test_surrogate_path_runs_and_falls_back(gpu_device):
+    """The surrogate call: finite outputs, table fallback for a column outside the training envelope
+    (that column then equals the plain mixed path bit for bit), heating rates close to FP64."""
+    print("case surrogate path", flush=True)
+    rad = Radiation(0, max_cols=2048)
+    train = {k: torch.from_numpy(np.ascontiguousarray(v)).to(gpu_device) for k, v in gen.columns(3000, seed=41).items()}
+    feat, targ = S.layer_optics(rad, train)
+    m = S.MLP(gpu_device, seed=2)
+    S.fit(m, feat, targ, steps=600, batch=4096, lr=3e-3, log_every=200)
+    rad.attach_surrogate(m)
+    cols = gen.columns(1500, seed=42)
+    cols["t_lay"][:, 7] = 345.0                     # far outside the training temperatures
+    dev = {k: torch.from_numpy(np.ascontiguousarray(v)).to(gpu_device) for k, v in cols.items()}
+    out, info = rad.run(dev, precision="surrogate")
+    mix, _ = rad.run(dev, precision="mixed")
+    ref, _ = rad.run(dev, precision="fp64")
+    torch.cuda.synchronize()
+    flag = info["fallback"].cpu().numpy()
+    assert flag[7] == 1 and flag.sum() < 20, flag.sum()
+    for k in abi.OUTPUT_FIELDS:
+        assert torch.isfinite(out[k]).all(), k
+        assert torch.equal(out[k][:, 7], mix[k][:, 7]), k
+    hr = float(torch.sqrt(torch.mean((out["hr_lw"] - ref["hr_lw"]) ** 2))) * 86400
+    print("surrogate LW HR RMS K/day (short training):", hr)
+    assert hr < 5.0
     rad.close()

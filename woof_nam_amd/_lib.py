@@ -17,7 +17,7 @@ TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 EXPORTS = ("wr_default_params", "wr_create", "wr_run_fp64", "wr_run_mixed", "wr_run_band", "wr_cal_cldfra",
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
-           "wr_mlp_to_bf16")
+           "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate")
 
 _ERRORS = {
     abi.WR_ERR_ARG: ValueError,
@@ -71,8 +71,13 @@ def load():
     L.wr_mlp_adam.argtypes = [vp, vp, vp, vp, vp, i64, f32, f32, f32, f32, i32, vp]
     L.wr_mlp_to_bf16.argtypes = [vp, vp, i64, vp]
     for f in ("wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
-              "wr_mlp_to_bf16"):
+              "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate"):
         getattr(L, f).restype = C.c_int
+    L.wr_surrogate_attach.argtypes = [vp, C.POINTER(abi.WrSurrogate)]
+    L.wr_surrogate_attach.restype = C.c_int
+    L.wr_run_surrogate.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
+                                   vp, vp]
+    L.wr_run_surrogate.restype = C.c_int
     L.wr_strerror.argtypes = [C.c_int]
     L.wr_strerror.restype = C.c_char_p
     L.wr_destroy.argtypes = [vp]

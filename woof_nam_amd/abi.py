@@ -39,6 +39,11 @@ class WrDiag(C.Structure):
                 ("lw_mask", _up), ("sw_mask", _up)]
 
 
+class WrSurrogate(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("w1", "w2", "w3", "b1", "b2", "b3", "fmean", "fstd", "fmin", "fmax",
+                                           "tmean", "tstd")]
+
+
 INPUT_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi", "t_sfc", "emis", "alb", "cosz")
 OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn")
 DIAG_FIELDS = ("lw_jp", "lw_jt", "lw_jt1", "lw_laytrop", "sw_jp", "sw_jt", "sw_jt1", "sw_laytrop",

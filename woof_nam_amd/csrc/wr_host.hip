@@ -447,12 +447,15 @@ RunPar to_par(const wr_params *p)
     return r;
 }
 
+enum { MODE_FP64 = 0, MODE_MIXED = 1, MODE_SURR = 2 };
+
 int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o, const wr_diag *dg, void *stream,
-        bool mixed, int band = -1)
+        int mode, int band = -1, int32_t *flag_out = nullptr)
 {
     int rc = check_args(c, in, p, o);
     if (rc != WR_OK || in->ncol == 0) return rc;
     if (p->mcica != 1) return WR_ERR_UNSUPPORTED;
+    if (mode == MODE_SURR && !c->has_sur) return WR_ERR_ARG;
     DevGuard g(c->device);
     hipStream_t s = (hipStream_t)stream;
     const ColIn ci = to_colin(in);
@@ -465,7 +468,12 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     const int per = (in->ncol + ntiles - 1) / ntiles;
     for (int col0 = 0; col0 < in->ncol; col0 += per) {
         const int n = (in->ncol - col0 < per) ? in->ncol - col0 : per;
-        rc = mixed ? wr_launch_mixed(c, ci, par, op, col0, n, band, s) : wr_launch_fp64(c, ci, par, op, col0, n, band, s);
+        if (mode == MODE_SURR)
+            rc = wr_launch_surr(c, ci, par, op, col0, n, flag_out ? flag_out + col0 : nullptr, s);
+        else if (mode == MODE_MIXED)
+            rc = wr_launch_mixed(c, ci, par, op, col0, n, band, s);
+        else
+            rc = wr_launch_fp64(c, ci, par, op, col0, n, band, s);
         if (rc != WR_OK) return rc;
         if (dg) {
             tbeg(c, KID_DIAG, s);
@@ -536,20 +544,45 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
 int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, const wr_diag *diag,
                 void *stream)
 {
-    return run(ctx, in, par, out, diag, stream, false);
+    return run(ctx, in, par, out, diag, stream, MODE_FP64);
 }
 
 int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, const wr_diag *diag,
                  void *stream)
 {
-    return run(ctx, in, par, out, diag, stream, true);
+    return run(ctx, in, par, out, diag, stream, MODE_MIXED);
 }
 
 int wr_run_band(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, int32_t band,
                 int32_t mixed, void *stream)
 {
     if (band < 0 || band >= WR_NBANDS_LW + WR_NBANDS_SW) return WR_ERR_ARG;
-    return run(ctx, in, par, out, nullptr, stream, mixed != 0, band);
+    return run(ctx, in, par, out, nullptr, stream, mixed ? MODE_MIXED : MODE_FP64, band);
+}
+
+int wr_surrogate_attach(wr_ctx *c, const wr_surrogate *w)
+{
+    if (!c || !w || !w->w1 || !w->w2 || !w->w3 || !w->b1 || !w->b2 || !w->b3 || !w->fmean || !w->fstd || !w->fmin ||
+        !w->fmax || !w->tmean || !w->tstd)
+        return WR_ERR_ARG;
+    DevGuard g(c->device);
+    if (!c->sx) {
+        const size_t S = (size_t)c->nl_max * c->max_cols;
+        if (hipMalloc((void **)&c->sx, S * 8 * sizeof(float)) != hipSuccess ||
+            hipMalloc((void **)&c->sy, S * 504 * sizeof(float)) != hipSuccess ||
+            hipMalloc(&c->sh1, S * 128 * 2) != hipSuccess || hipMalloc(&c->sh2, S * 128 * 2) != hipSuccess ||
+            hipMalloc((void **)&c->sflag, (size_t)c->max_cols * sizeof(int)) != hipSuccess)
+            return WR_ERR_NOMEM;
+    }
+    c->sur = *w;
+    c->has_sur = 1;
+    return WR_OK;
+}
+
+int wr_run_surrogate(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, int32_t *col_flag,
+                     void *stream)
+{
+    return run(ctx, in, par, out, nullptr, stream, MODE_SURR, -1, col_flag);
 }
 
 int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi, int32_t j_lo, int32_t j_hi,
@@ -625,6 +658,8 @@ const char *wr_kernel_name(int32_t kid)
     if (kid >= KID_SW0 && kid < KID_SW0 + 14) return sw[kid - KID_SW0];
     if (kid == KID_POST) return "k_post";
     if (kid == KID_DIAG) return "k_diag";
+    if (kid == KID_FEAT) return "k_features";
+    if (kid == KID_MLP) return "mlp_gemm";
     return "?";
 }
 
@@ -669,6 +704,11 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->dws);
     (void)hipFree(c->sflux_d);
     (void)hipFree(c->sflux_f);
+    (void)hipFree(c->sx);
+    (void)hipFree(c->sy);
+    (void)hipFree(c->sh1);
+    (void)hipFree(c->sh2);
+    (void)hipFree(c->sflag);
     if (c->ev)
         for (int j = 0; j < 2 * c->ev_cap; ++j)
             if (c->ev[j]) (void)hipEventDestroy(c->ev[j]);

@@ -20,6 +20,12 @@ struct wr_ctx {
     double *dws;                     // 2 x [max_cols]
     double *sflux_d;                 // [112][max_cols]
     float *sflux_f;
+    // learned gas optics (wr_surrogate_attach): network weights and tile work space
+    int has_sur;
+    wr_surrogate sur;
+    float *sx, *sy;                  // features [S][8], outputs [504][S]
+    void *sh1, *sh2;                 // hidden activations, bfloat16 [S][128]
+    int *sflag;                      // [max_cols] per-column table-optics flag
     // optional per-launch event timing (wr_timing)
     int timing, ev_cap, ev_n;
     hipEvent_t *ev;                  // 2 x ev_cap
@@ -28,7 +34,7 @@ struct wr_ctx {
 
 // kernel ids reported by wr_timing_read
 enum { KID_ZERO = 0, KID_COLPREP = 1, KID_DAYLIST = 2, KID_LW0 = 3, KID_SW0 = 19, KID_POST = 33, KID_DIAG = 34,
-       KID_COUNT = 35 };
+       KID_FEAT = 35, KID_MLP = 36, KID_COUNT = 37 };
 
 inline void tbeg(wr_ctx *c, int kid, hipStream_t s)
 {
@@ -80,3 +86,5 @@ int wr_launch_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const
 int wr_launch_diag(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
                    hipStream_t s);
 int wr_init_luts(wr_ctx *c, hipStream_t s);
+int wr_launch_surr(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
+                   int32_t *flag_out, hipStream_t s);

@@ -196,9 +196,44 @@ template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__re
     }
 }
 
-template <typename R, int B>
+// Learned gas optics of one tile (surrogate path): normalised network outputs Y[j * S + k * n + t]
+// (j < 504 outputs, surface-first layer k, tile column t), de-normalised as y * tstd[j] + tmean[j];
+// flag[t] != 0 sends column t back to the table optics (inputs outside the training envelope).
+struct Surr {
+    const float *Y, *tmean, *tstd;
+    const int *flag;
+    long long S;
+};
+
+__device__ __forceinline__ float surr_out(const Surr &sg, int j, size_t s)
+{
+    return sg.Y[(size_t)j * sg.S + s] * sg.tstd[j] + sg.tmean[j];
+}
+
+template <typename R, int B> __device__ __forceinline__ void surr_lw(const Surr &sg, size_t s, R *tau, R *fr)
+{
+#pragma unroll
+    for (int g = 0; g < LW_NG[B]; ++g) {
+        tau[g] = (R)__expf(surr_out(sg, LW_NS[B] + g, s));
+        fr[g] = (R)fminf(1.0f, fmaxf(0.0f, surr_out(sg, 140 + LW_NS[B] + g, s)));
+    }
+}
+
+// the layer quantities the LW transfer needs besides the gas optics
+template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCol &rc, int kin, LwLay<R> &L)
+{
+    double ct, cwp, cip;
+    rc.cloud(kin, ct, cwp, cip);
+    L.ct = ct;
+    L.clwp = cwp * 1.5;
+    L.ciwp = cip * 1.5;
+    L.tavel = rc.tlyr(kin);
+    L.delp = rc.plvl(kin + 1) - rc.plvl(kin);
+}
+
+template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, Acc acc)
+                                           ColScal cs, Acc acc, Surr sg)
 {
     constexpr int NG = LW_NG[B];
     constexpr int NS = LW_NS[B];
@@ -208,6 +243,7 @@ __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn 
     const RadCol rc{&in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = cs.lcf1[t];
+    const int table = SURR ? sg.flag[t] : 1;  // 0: learned gas optics for this column
     const double pwv = cs.pwv[t];
     const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
     const double sfe = in.emis[i];
@@ -234,9 +270,14 @@ __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn 
             tz_bot = rc.tskn();
         }
         LwLay<R> L;
-        lw_layer<R>(T, rc, par, kin, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics<R, B>(T, L, tau, fr);
+        if (SURR && !table) {
+            lw_layer_cloud<R>(rc, kin, L);
+            surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
+        } else {
+            lw_layer<R>(T, rc, par, kin, la_c, L);
+            lw_optics<R, B>(T, L, tau, fr);
+        }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
@@ -279,9 +320,14 @@ __global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn 
             tz_top = rc.tlyr(0);
         }
         LwLay<R> L;
-        lw_layer<R>(T, rc, par, kin, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics<R, B>(T, L, tau, fr);
+        if (SURR && !table) {
+            lw_layer_cloud<R>(rc, kin, L);
+            surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
+        } else {
+            lw_layer<R>(T, rc, par, kin, la_c, L);
+            lw_optics<R, B>(T, L, tau, fr);
+        }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
         if (k == 0) {  // surface emission and reflection
@@ -331,17 +377,29 @@ template <typename R, int B> struct SwLayerRT {
     R rb[SW_NG[B]], rd[SW_NG[B]], tb[SW_NG[B]], td[SW_NG[B]], ld[SW_NG[B]];
 };
 
-template <typename R, int B>
+template <typename R, int B, bool SURR = false>
 __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
-                                            int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o)
+                                            int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o,
+                                            const Surr &sg, size_t s, int table)
 {
     constexpr int NG = SW_NG[B];
     const int nrl = rc.nrl;
     const int kin = nrl - 1 - k;
     SwLay<R> L;
-    sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
     R tg[NG], tr[NG];
-    sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+    if (SURR && !table) {
+        rc.cloud(kin, L.ct, L.cwp, L.cip);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const R tau = (R)__expf(surr_out(sg, 280 + SW_NS[B] + g, s));
+            const R ssa = (R)fminf(1.0f, fmaxf(0.0f, surr_out(sg, 392 + SW_NS[B] + g, s)));
+            tr[g] = tau * ssa;
+            tg[g] = tau - tr[g];
+        }
+    } else {
+        sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+        sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+    }
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
     if (cf1 > 0.0 && L.ct > FTINY) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
 #pragma unroll
@@ -372,9 +430,9 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
     }
 }
 
-template <typename R, int B>
+template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, const R *__restrict__ sflux, Acc acc)
+                                           ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg)
 {
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
@@ -391,6 +449,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     const R asfc = (R)((SW_IDXSFC[B] != 0) ? alb : 0.5 * (alb + alb));
     const double cf1 = cs.zcf1[t];
     const int lt = cs.sw_laytrop[t];
+    const int table = SURR ? sg.flag[t] : 1;
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
     const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     uint64_t mb[NG];
@@ -408,7 +467,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
     for (int k = 0; k < nrl; ++k) {
         SwLayerRT<R, B> o;
-        sw_layer_rt<R, B>(T, rc, par, k, lt, cf1, mb, cz, snt, o);
+        sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
@@ -432,7 +491,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     acc.sw_dn[(size_t)nrl * n + t] += (double)sd;
     for (int s = nrl; s >= 1; --s) {
         SwLayerRT<R, B> o;
-        sw_layer_rt<R, B>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o);
+        sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table);
         su = (R)0, sd = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {

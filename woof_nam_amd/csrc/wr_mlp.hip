@@ -51,7 +51,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
     __shared__ __bf16 Bs[64][16 + 4];  // transposed: Bs[n][k]
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
-    const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+    const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;  // M on x: up to 2^31 rows
     const int k_begin = blockIdx.z * g.k_per_split;
     const int k_end = min(g.K, k_begin + g.k_per_split);
     const TA *A = (const TA *)g.A;
@@ -110,7 +110,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
                 float v = acc[i][j][q];
                 if (EPI == EPI_BIAS || EPI == EPI_BIAS_SILU) v += g.bias[n];
                 if (EPI == EPI_BIAS_SILU) {
-                    g.Zout[o] = v;
+                    if (g.Zout) g.Zout[o] = v;
                     v = silu(v);
                 }
                 if (EPI == EPI_DSILU) v *= dsilu(g.Zin[o]);
@@ -251,7 +251,7 @@ __global__ void __launch_bounds__(64) k_optics(const Tabs<double> *__restrict__ 
 template <typename TA, typename TB>
 int gemm_dispatch(int epi, const GemmArgs &g, int splits, hipStream_t s)
 {
-    dim3 grid((g.N + 63) / 64, (g.M + 63) / 64, splits);
+    dim3 grid((g.M + 63) / 64, (g.N + 63) / 64, splits);
     switch (epi) {
     case EPI_STORE: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_STORE>), grid, dim3(256), 0, s, g); break;
     case EPI_BIAS: hipLaunchKernelGGL((k_gemm<TA, TB, EPI_BIAS>), grid, dim3(256), 0, s, g); break;
@@ -271,9 +271,9 @@ int wr_mlp_gemm(int32_t M, int32_t N, int32_t K, const void *A, int32_t a_bf16, 
                 int64_t ldc_n, const float *bias, float *Zout, const float *Zin, int32_t epilogue, int32_t splits,
                 int64_t split_stride, void *stream)
 {
-    if (M < 0 || N < 0 || K < 0 || splits < 1 || !A || !B || (!C && !Cb)) return WR_ERR_ARG;
+    if (M < 0 || N < 0 || K < 0 || splits < 1 || splits > 65535 || N > 65535 * 64 || !A || !B || (!C && !Cb))
+        return WR_ERR_ARG;
     if ((epilogue == EPI_BIAS || epilogue == EPI_BIAS_SILU) && !bias) return WR_ERR_ARG;
-    if (epilogue == EPI_BIAS_SILU && !Zout) return WR_ERR_ARG;
     if (epilogue == EPI_DSILU && !Zin) return WR_ERR_ARG;
     if (splits > 1 && (epilogue != EPI_STORE || !C || Cb)) return WR_ERR_ARG;
     if (M == 0 || N == 0) return WR_OK;

@@ -94,6 +94,19 @@ class Radiation:
         _lib.check(rc, "wr_run_band")
         return out
 
+    def attach_surrogate(self, mlp) -> None:
+        """Use a trained surrogate.MLP for precision="surrogate" calls (its tensors must stay alive)."""
+        if mlp.norm is None:
+            raise ValueError("surrogate has no normalisation: train it (surrogate.fit) or load it first")
+        n = mlp.norm
+        self._sur_keep = {k: v.float().contiguous() for k, v in n.items()}
+        w = abi.WrSurrogate(*[C.c_void_p(t.data_ptr()) for t in (
+            mlp.wb["W1"], mlp.wb["W2"], mlp.wb["W3"], mlp.w["b1"], mlp.w["b2"], mlp.w["b3"],
+            self._sur_keep["fmean"], self._sur_keep["fstd"], self._sur_keep["fmin"], self._sur_keep["fmax"],
+            self._sur_keep["tmean"], self._sur_keep["tstd"])])
+        self._sur_mlp = mlp
+        _lib.check(_lib.load().wr_surrogate_attach(self._h, C.byref(w)), "wr_surrogate_attach")
+
     def timing(self, enable: bool = True) -> None:
         """Bracket every later kernel launch with HIP events (per-launch durations, see timing_read)."""
         _lib.check(_lib.load().wr_timing(self._h, int(bool(enable))), "wr_timing")
@@ -122,8 +135,8 @@ class Radiation:
     def run(self, cols: dict, params: abi.WrParams | None = None, precision: str = "fp64",
             out: dict | None = None, diag: bool = False, stream: torch.cuda.Stream | None = None):
         """Radiation call on device columns.  Returns (outputs, diag or None); asynchronous on the stream."""
-        if precision not in ("fp64", "mixed"):
-            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        if precision not in ("fp64", "mixed", "surrogate"):
+            raise ValueError(f"precision must be 'fp64', 'mixed' or 'surrogate', got {precision!r}")
         t = cols["t_lay"]
         nlay, ncol = int(t.shape[0]), int(t.shape[1])
         for k, shp in input_shapes(ncol, nlay).items():
@@ -149,6 +162,14 @@ class Radiation:
         params = params or self.default_params()
         s = stream or torch.cuda.current_stream(self.device)
         L = _lib.load()
+        if precision == "surrogate":
+            if diag:
+                raise ValueError("diagnostics are not available on the surrogate path")
+            flag = torch.zeros(ncol, dtype=torch.int32, device=self.device)
+            rc = L.wr_run_surrogate(self._h, C.byref(ci), C.byref(params), C.byref(co), C.c_void_p(flag.data_ptr()),
+                                    C.c_void_p(s.cuda_stream))
+            _lib.check(rc, "wr_run_surrogate")
+            return out, {"fallback": flag}
         fn = L.wr_run_fp64 if precision == "fp64" else L.wr_run_mixed
         rc = fn(self._h, C.byref(ci), C.byref(params), C.byref(co), C.byref(cd) if cd is not None else None,
                 C.c_void_p(s.cuda_stream))

@@ -216,7 +216,7 @@ def fit(mlp: MLP, feat, targ, steps=2000, batch=8192, lr=2e-3, lr_end=1e-4, seed
         loss = mlp.loss_and_grad(xb, tb)
         frac = (it - 1) / max(1, steps - 1)
         mlp.adam(lr=lr_end + 0.5 * (lr - lr_end) * (1 + math.cos(math.pi * frac)))
-        if it % log_every == 0 or it == steps:
+        if it == 1 or it % log_every == 0 or it == steps:
             lv = float(loss)
             hist.append((it, lv))
             log(f"step {it}: loss {lv:.4e} ({time.perf_counter() - t0:.1f} s)")
