@@ -46,6 +46,41 @@ def kernel_bytes(name: str, ncol: int, nday: int) -> float:
     return 0.0
 
 
+def served_block(d, dev, ncol, seconds):
+    """Config-3 serving: Poisson arrivals at 40 req/s of the parent grid, 3 km / 6 km nests and
+    ensemble members (one member per request, seeds 1..8), mixed precision, 200 GB HBM budget."""
+    import torch
+
+    from woof_nam_amd import gen
+    from woof_nam_amd.serve import Server, autotune, poisson_run
+
+    def dev_grid(nx, ny, seed, id0):
+        c = gen.grid(nx, ny, seed=seed)
+        g = {k: torch.from_numpy(v).to(dev) for k, v in c.items()}
+        return g, torch.arange(id0, id0 + nx * ny, dtype=torch.int64, device=dev)
+
+    pid = torch.arange(ncol, dtype=torch.int64, device=dev)
+    kinds = {"parent": (d, pid), "nest3km": dev_grid(333, 333, 11, 10_000_000),
+             "nest6km": dev_grid(200, 200, 12, 20_000_000), "member": (d, pid)}
+    t0 = time.perf_counter()
+    tune = autotune(0, tiles=(65536, 131072, 262144), slots=(2, 4), ncol=262144, precision="mixed")
+    best = tune["best"]
+    print(f"# autotune {time.perf_counter() - t0:.1f} s: tile {best['tile']} slots {best['slots']}",
+          file=sys.stderr, flush=True)
+    srv = Server(0, tile=best["tile"], slots=best["slots"], budget_gb=200.0, precision="mixed",
+                 filler={k: v[..., :best["tile"]] for k, v in d.items()})
+    r = poisson_run(srv, kinds, rate=40.0, duration=seconds, seed=0)
+    srv.close()
+    r["autotune"] = tune["all"]
+    r["tile"], r["slots"] = best["tile"], best["slots"]
+    r["arrivals"] = f"Poisson 40 req/s for {seconds} s; kinds uniform over parent 796,590 / nest 110,889 / " \
+                    "nest 40,000 / one ensemble member of the parent (own seed); offered load exceeds capacity, " \
+                    "so p99 includes queueing"
+    print(f"# served: {r['requests']} requests, {r['columns_per_s']:.4g} columns/s, p50 {r['p50_ms']:.0f} ms, "
+          f"p99 {r['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
+    return r
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -58,6 +93,11 @@ def main(argv=None):
     ap.add_argument("--cpu-cols", type=int, default=0, help="CPU baseline sample (0 = sized from cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--acc-cols", type=int, default=65536, help="mixed-precision error-budget sample")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison block")
+    ap.add_argument("--served-seconds", type=float, default=3.0,
+                    help="seconds of Poisson arrivals (40 req/s) for the served block (0 = skip)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="first copy 1 GiB between two device buffers (reference for HBM byte counters)")
     a = ap.parse_args(argv)
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
@@ -70,6 +110,12 @@ def main(argv=None):
     from woof_nam_amd import gen
     from woof_nam_amd.radiation import Radiation
 
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     t0 = time.time()
     cols = gen.grid(a.nx, a.ny, seed=0)
     ncol = cols["t_lay"].shape[1]
@@ -133,7 +179,29 @@ def main(argv=None):
                      "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
     }
 
-    if a.precision == "mixed":
+    if a.precision == "fp64" and not a.no_mixed:
+        # the same workload on the mixed-precision path: throughput, speedup, errors against FP64
+        mout, _ = rad.run(d, precision="mixed")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(a.steps):
+            rad.run(d, precision="mixed", out=mout)
+        torch.cuda.synchronize()
+        tm = (time.perf_counter() - t1) / a.steps
+        mixed = {"value": ncol / tm, "ms_per_step": tm * 1e3, "speedup_vs_fp64": ms_per_step / (tm * 1e3)}
+        for k in ("hr_lw", "hr_sw"):
+            mixed[k + "_rms_kday"] = float(torch.sqrt(torch.mean((mout[k] - out[k]) ** 2))) * KDAY
+        for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+            mixed[k + "_rms_wm2"] = float(torch.sqrt(torch.mean((mout[k] - out[k]) ** 2)))
+        mixed["vs"] = "FP64 path on all columns (the FP64 path equals the CPU oracle bit for bit)"
+        res["mixed"] = mixed
+        del mout
+        print(f"# mixed: {tm * 1e3:.1f} ms/step, {ncol / tm:.4g} columns/s", file=sys.stderr, flush=True)
+
+    if a.served_seconds > 0:
+        res["served"] = served_block(d, dev, ncol, a.served_seconds)
+
+    if a.precision == "mixed" and a.acc_cols > 0:
         # error budget against the FP64 path (bit-identical to the CPU oracle) on a column sample,
         # per layer and per band
         from woof_nam_amd import accuracy

@@ -70,3 +70,19 @@ def test_mixed_grid_sample(rad):
         assert e[k][0] <= HR_RMS_KDAY, (k, e[k])
     for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
         assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])
+
+
+def test_mixed_nam_route_within_tolerance(rad):
+    cols = gen.columns(2500, seed=71)
+    print("case mixed nam route", flush=True)
+    p = Radiation.default_params(mcica=0)
+    ref, _ = pyoracle.run(cols, params=p)
+    dev = {k: torch.from_numpy(v).to(rad.device) for k, v in cols.items()}
+    out, _ = rad.run(dev, params=p, precision="mixed")
+    torch.cuda.synchronize()
+    e = errors(out, ref)
+    print("RMS / max error (K/day, W/m2):", e)
+    for k in ("hr_lw", "hr_sw"):
+        assert e[k][0] <= HR_RMS_KDAY, (k, e[k])
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])

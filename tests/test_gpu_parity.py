@@ -121,8 +121,8 @@ def test_bad_arguments(rad):
     bad["t_lay"] = bad["t_lay"].float()
     with pytest.raises(ValueError):
         rad.run(bad)
-    with pytest.raises(NotImplementedError):
-        rad.run(cols, params=Radiation.default_params(mcica=0))
+    with pytest.raises(ValueError):
+        rad.run(cols, params=Radiation.default_params(mcica=2))
     with pytest.raises(ValueError):
         rad.run(cols, params=Radiation.default_params(iovr=3))
 
@@ -141,3 +141,22 @@ def test_band_runs_sum_to_full_call(rad):
     for k in abi.OUTPUT_FIELDS:
         a, b = acc[k].cpu().numpy(), full[k].cpu().numpy()
         np.testing.assert_allclose(a, b, rtol=1e-11, atol=1e-9 * (np.abs(b).max() + 1e-300), err_msg=k)
+
+
+@pytest.mark.parametrize("seed", [61, 62])
+def test_nam_route_matches_oracle(rad, seed):
+    """mcica = 0: rtrnmr (maximum-random fractional clouds) and cldfrc-mixed spcvrtc with zcf1/zcf0
+    weighting, the route NAM runs; bit for bit against the oracle, indices bit-exact, masks zero."""
+    cols = gen.columns(1500, seed=seed)
+    cols["cldf"][:, :100] = 1.0                   # overcast columns: zcf0 = 0
+    cols["cldf"][:, 100:200] = 0.0                # clear columns: zcf1 = 0, SW clear pass only
+    cols["cldf"][20:30, 200:300] = 0.0            # cloud blocks separated by clear layers
+    p = Radiation.default_params(mcica=0)
+    out, dg, ref, rdg = run_both(rad, cols, p)
+    compare_diag(dg, rdg, "nam")
+    assert not dg["lw_mask"].any() and not dg["sw_mask"].any()
+    compare(out, ref, "nam")
+    for k in abi.OUTPUT_FIELDS:
+        a = out[k].cpu().numpy()
+        assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), \
+            f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"

@@ -37,8 +37,16 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found: the HIP toolchain is required to build libwoofrad")
 
 
+# The mixed-precision and surrogate units contract FP32 multiply-adds and use the hardware FP32
+# reciprocal/sqrt; the FP64 index arithmetic inside them keeps contraction off by pragma.
+UNIT_FLAGS = {
+    "wr_mixed.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+    "wr_surr.hip": ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"],
+}
+
+
 def _compile(src: Path, obj: Path) -> None:
-    cmd = [hipcc(), *CFLAGS, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")

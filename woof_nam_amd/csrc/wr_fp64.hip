@@ -38,5 +38,6 @@ int wr_init_luts(wr_ctx *c, hipStream_t s)
 int wr_launch_fp64(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs &out, int col0, int n, int band,
                    hipStream_t s)
 {
+    if (par.mcica == 0) return launch_tile_nam<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
     return launch_tile<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
 }

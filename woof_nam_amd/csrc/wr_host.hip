@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
         const int lcf1 = cs.lcf1[t];
         for (int k = 0; k < nrl; ++k)
             for (int w = 0; w < 5; ++w) dg.lw_mask[((size_t)k * 5 + w) * nc + i] = 0u;
-        for (int g = 0; g < 140; ++g) {
+        for (int g = 0; g < (par.mcica ? 140 : 0); ++g) {
             const uint64_t mb = mcica_g(col, seed, 0, g, nrl, par.iovr,
                                         [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
             for (int k = 0; k < nrl; ++k)
@@ -343,7 +343,7 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
     if (dg.sw_mask) {
         for (int k = 0; k < nrl; ++k)
             for (int w = 0; w < 4; ++w) dg.sw_mask[((size_t)k * 4 + w) * nc + i] = 0u;
-        if (day) {
+        if (day && par.mcica) {
             const double cf1 = cs.zcf1[t];
             for (int g = 0; g < 112; ++g) {
                 const uint64_t mb = mcica_g(col, seed, 1, g, nrl, par.iovr, [&](int k) {
@@ -454,8 +454,8 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
 {
     int rc = check_args(c, in, p, o);
     if (rc != WR_OK || in->ncol == 0) return rc;
-    if (p->mcica != 1) return WR_ERR_UNSUPPORTED;
-    if (mode == MODE_SURR && !c->has_sur) return WR_ERR_ARG;
+    if (p->mcica != 0 && p->mcica != 1) return WR_ERR_ARG;
+    if (mode == MODE_SURR && (!c->has_sur || p->mcica != 1)) return p->mcica != 1 ? WR_ERR_UNSUPPORTED : WR_ERR_ARG;
     DevGuard g(c->device);
     hipStream_t s = (hipStream_t)stream;
     const ColIn ci = to_colin(in);
@@ -527,7 +527,9 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     if (rc == WR_OK) {
         if (hipMalloc((void **)&c->acc, sizeof(double) * 4 * (NL + 1) * m) != hipSuccess ||
             hipMalloc((void **)&c->iws, sizeof(int) * 6 * m) != hipSuccess ||
-            hipMalloc((void **)&c->dws, sizeof(double) * 2 * m) != hipSuccess ||
+            hipMalloc((void **)&c->dws, sizeof(double) * 3 * m) != hipSuccess ||
+            hipMalloc(&c->rtr, (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m) != hipSuccess ||
+            hipMalloc((void **)&c->swc, sizeof(double) * 2 * (NL + 1) * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_d, sizeof(double) * 112 * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_f, sizeof(float) * 112 * m) != hipSuccess)
             rc = WR_ERR_NOMEM;
@@ -660,6 +662,7 @@ const char *wr_kernel_name(int32_t kid)
     if (kid == KID_DIAG) return "k_diag";
     if (kid == KID_FEAT) return "k_features";
     if (kid == KID_MLP) return "mlp_gemm";
+    if (kid == KID_RTRFAC) return "k_rtr_fac";
     return "?";
 }
 
@@ -702,6 +705,8 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->acc);
     (void)hipFree(c->iws);
     (void)hipFree(c->dws);
+    (void)hipFree(c->rtr);
+    (void)hipFree(c->swc);
     (void)hipFree(c->sflux_d);
     (void)hipFree(c->sflux_f);
     (void)hipFree(c->sx);

@@ -17,7 +17,9 @@ struct wr_ctx {
     // tile work space
     double *acc;                     // 4 x [nl_max + 2][max_cols]
     int *iws;                        // 6 x [max_cols]
-    double *dws;                     // 2 x [max_cols]
+    double *dws;                     // 3 x [max_cols]
+    void *rtr;                       // maximum-random overlap factors of the NAM LW route
+    double *swc;                     // clear-sky band fluxes of the NAM SW route, 2 x [NL+1][max_cols]
     double *sflux_d;                 // [112][max_cols]
     float *sflux_f;
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
@@ -34,7 +36,7 @@ struct wr_ctx {
 
 // kernel ids reported by wr_timing_read
 enum { KID_ZERO = 0, KID_COLPREP = 1, KID_DAYLIST = 2, KID_LW0 = 3, KID_SW0 = 19, KID_POST = 33, KID_DIAG = 34,
-       KID_FEAT = 35, KID_MLP = 36, KID_COUNT = 37 };
+       KID_FEAT = 35, KID_MLP = 36, KID_RTRFAC = 37, KID_COUNT = 38 };
 
 inline void tbeg(wr_ctx *c, int kid, hipStream_t s)
 {
@@ -64,6 +66,7 @@ inline ColScal colscal(wr_ctx *c)
     s.nday = c->iws + 5 * m;
     s.pwv = c->dws + 0 * m;
     s.zcf1 = c->dws + 1 * m;
+    s.zcf0 = c->dws + 2 * m;
     return s;
 }
 inline Acc tile_acc(wr_ctx *c, int n, int nrl)

@@ -22,13 +22,14 @@ struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each
 };
 struct ColScal {
     int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
-    double *pwv, *zcf1;
+    double *pwv, *zcf1, *zcf0;
 };
 
 template <typename R>
 __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                                  ColScal cs, R *__restrict__ sflux)
 {
+#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
@@ -77,6 +78,7 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
     if (z0 <= FTINY) z0 = 0.0;
     if (z0 > ONEMINUS) z0 = 1.0;
     cs.zcf1[t] = 1.0 - z0;
+    cs.zcf0[t] = z0;
     cs.sw_laytrop[t] = swlt;
     // solar source per g-point (phys_rrtm_sw_taumol.cu:59-120)
     for (int b = 0; b < 14; ++b) {
@@ -377,6 +379,38 @@ template <typename R, int B> struct SwLayerRT {
     R rb[SW_NG[B]], rd[SW_NG[B]], tb[SW_NG[B]], td[SW_NG[B]], ld[SW_NG[B]];
 };
 
+// Layer R/T of one g-point from its gas (tg) and Rayleigh (tr) optical depths, plus the cloud
+// (tau, tau*w, tau*w*g) when cld.  FP32 evaluates in FP64 where FP32 loses the answer:
+// near-conservative scattering (w0 > 1 - 1e-6: 1 - w0 and gamma1 - gamma2 cancel; an FP32 pre-test
+// with a 10x wider margin skips the FP64 test for most layers) and near the two-stream singularity
+// k mu = 1 (flagged by the FP32 evaluation); the optical properties are then summed in FP64.
+template <typename R>
+__device__ __forceinline__ void layer_rt_eval(const Tabs<R> *__restrict__ T, R tr, R tg, bool cld, R ctau, R cssa, R casy,
+                                              R cz, R snt, R &rb, R &rd, R &tb, R &td, R &ld)
+{
+    R ztau0 = fmax((R)FTINY, tr + tg + (R)0);
+    R zssa0 = tr + (R)0;
+    R zasy0 = (R)0;
+    if (cld) {
+        ztau0 = ztau0 + ctau;
+        zssa0 = zssa0 + cssa;
+        zasy0 = zasy0 + casy;
+    }
+    if constexpr (sizeof(R) < sizeof(double)) {
+        const double t0 = fmax(FTINY, (double)tr + (double)tg + (cld ? (double)ctau : 0.0));
+        const double s0 = (double)tr + (cld ? (double)cssa : 0.0);
+        bool fp64 = zssa0 > (R)0.99999 * ztau0 && s0 > W0_FP64 * t0;
+        if (!fp64) layer_rt<R>(ztau0, zssa0, zasy0, cz, snt, T->sw_exp_tbl, rb, rd, tb, td, ld, &fp64);
+        if (fp64) {
+            double b, d, e, f, h;
+            layer_rt<double>(t0, s0, cld ? (double)casy : 0.0, (double)cz, (double)snt, T->sw_exp_tbl64, b, d, e, f, h);
+            rb = (R)b, rd = (R)d, tb = (R)e, td = (R)f, ld = (R)h;
+        }
+    } else {
+        layer_rt<R>(ztau0, zssa0, zasy0, cz, snt, T->sw_exp_tbl, rb, rd, tb, td, ld);
+    }
+}
+
 template <typename R, int B, bool SURR = false>
 __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
                                             int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o,
@@ -403,31 +437,9 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
     if (cf1 > 0.0 && L.ct > FTINY) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
 #pragma unroll
-    for (int g = 0; g < NG; ++g) {
-        const bool cld = (mb[g] >> k) & 1;
-        if constexpr (sizeof(R) < sizeof(double)) {
-            // near-conservative scattering (w0 > 1 - 1e-6): 1 - w0 and gamma1 - gamma2 cancel in FP32,
-            // so these layers take the FP64 R/T from the FP32 optical properties, summed in FP64
-            const double t0 = fmax(FTINY, (double)tr[g] + (double)tg[g] + (cld ? (double)ctau : 0.0));
-            const double s0 = (double)tr[g] + (cld ? (double)cssa : 0.0);
-            if (s0 > W0_FP64 * t0) {
-                double rb, rd, tb, td, ld;
-                layer_rt<double>(t0, s0, cld ? (double)casy : 0.0, (double)cz, (double)snt, T->sw_exp_tbl64, rb, rd,
-                                 tb, td, ld);
-                o.rb[g] = (R)rb, o.rd[g] = (R)rd, o.tb[g] = (R)tb, o.td[g] = (R)td, o.ld[g] = (R)ld;
-                continue;
-            }
-        }
-        R ztau0 = fmax((R)FTINY, tr[g] + tg[g] + (R)0);
-        R zssa0 = tr[g] + (R)0;
-        R zasy0 = (R)0;
-        if (cld) {
-            ztau0 = ztau0 + ctau;
-            zssa0 = zssa0 + cssa;
-            zasy0 = zasy0 + casy;
-        }
-        layer_rt<R>(ztau0, zssa0, zasy0, cz, snt, T->sw_exp_tbl, o.rb[g], o.rd[g], o.tb[g], o.td[g], o.ld[g]);
-    }
+    for (int g = 0; g < NG; ++g)
+        layer_rt_eval<R>(T, tr[g], tg[g], (mb[g] >> k) & 1, ctau, cssa, casy, cz, snt, o.rb[g], o.rd[g], o.tb[g],
+                         o.td[g], o.ld[g]);
 }
 
 template <typename R, int B, bool SURR = false>
@@ -461,8 +473,10 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
             return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
         });
     }
-    // bottom-up: reflectance of the medium below each level (swflux sweep 1)
+    // bottom-up: reflectance of the medium below each level (swflux sweep 1); the layer R/T of every
+    // g-point are kept (private memory) for the top-down sweep instead of being recomputed
     R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
+    R lrb[NG][NL], lrd[NG][NL], ltb[NG][NL], ltd[NG][NL], lld[NG][NL];
 #pragma unroll
     for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
     for (int k = 0; k < nrl; ++k) {
@@ -470,6 +484,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
         sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
+            lrb[g][k] = o.rb[g], lrd[g][k] = o.rd[g], ltb[g][k] = o.tb[g], ltd[g][k] = o.td[g], lld[g][k] = o.ld[g];
             const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
             zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
             zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
@@ -491,7 +506,11 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     acc.sw_dn[(size_t)nrl * n + t] += (double)sd;
     for (int s = nrl; s >= 1; --s) {
         SwLayerRT<R, B> o;
-        sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table);
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            o.rb[g] = lrb[g][s - 1], o.rd[g] = lrd[g][s - 1], o.tb[g] = ltb[g][s - 1], o.td[g] = ltd[g][s - 1];
+            o.ld[g] = lld[g][s - 1];
+        }
         su = (R)0, sd = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -523,6 +542,7 @@ struct OutPtrs {
 };
 static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, Acc acc, OutPtrs out)
 {
+#pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;

@@ -25,6 +25,7 @@ template <typename R>
 __device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int kin,
                                          double la, LwLay<R> &L)
 {
+#pragma clang fp contract(off)
     const double tem1g = 100.0 * CON_G;
     const double tem2c = 1.0e-20 * 1.0e3 * CON_AVGD;
     const double pav = rc.plyr(kin), tav = rc.tlyr(kin);

@@ -19,6 +19,7 @@ __device__ __forceinline__ double pow2i(int k)
 
 __device__ __forceinline__ double exp_d(double x)
 {
+#pragma clang fp contract(off)
     if (x != x) return x;
     if (x > 709.782712893384) return __longlong_as_double(0x7ff0000000000000LL);
     if (x < -745.1332191019412) return 0.0;
@@ -45,6 +46,7 @@ __device__ __forceinline__ double exp_d(double x)
 
 __device__ __forceinline__ double log_d(double x)
 {
+#pragma clang fp contract(off)
     if (x != x || x < 0.0) return __longlong_as_double(0x7ff8000000000000LL);
     if (x == 0.0) return __longlong_as_double((long long)0xfff0000000000000ULL);
     if (x == __longlong_as_double(0x7ff0000000000000LL)) return x;

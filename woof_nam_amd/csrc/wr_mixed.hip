@@ -10,5 +10,6 @@ using namespace wr;
 int wr_launch_mixed(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs &out, int col0, int n, int band,
                    hipStream_t s)
 {
+    if (par.mcica == 0) return launch_tile_nam<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
     return launch_tile<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
 }

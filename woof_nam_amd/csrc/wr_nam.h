@@ -1,0 +1,436 @@
+// wr_nam.h -- the NAM route (mcica = 0): LW transfer with maximum-random overlap of fractional
+// clouds (rtrnmr, phys_rrtm_lw_rtrnmr_body.cuh:9-365) and the SW two-stream with cloud-fraction
+// mixing and column-cover weighting (spcvrtc, phys_rrtm_sw_spcvrtc.cu:217-547).
+#pragma once
+#include "wr_kernels.h"
+
+namespace wr {
+
+// Overlap factors of one tile (rtrnmr:50-172): f[q][idx][t], idx 0..nl+2 as in the reference;
+// q = 0 faccld1u, 1 faccld2u, 2 facclr1u, 3 facclr2u, 4 faccmb1u, 5 faccmb2u, 6..11 the same
+// downward; lstu/lstd bit (idx - 1) set for idx = 1..nl.
+template <typename R> struct Rtr {
+    R *f;
+    uint64_t *lstu, *lstd;
+    int n;
+    __device__ __forceinline__ R &at(int q, int idx, int t) const { return f[((size_t)q * (NL + 3) + idx) * n + t]; }
+};
+enum { FCLD1U = 0, FCLD2U, FCLR1U, FCLR2U, FCMB1U, FCMB2U, FCLD1D, FCLD2D, FCLR1D, FCLR2D, FCMB1D, FCMB2D };
+
+// layer cloud fraction cldfrc(kk) of the LW route (1-based surface-first, cldfrc(0) = 1, (nl+1) = 0)
+__device__ __forceinline__ double lw_cldfrc(const RadCol &rc, int kk)
+{
+    if (kk == 0) return 1.0;
+    if (kk > rc.nrl) return 0.0;
+    return rc.cfrac(rc.nrl - kk);
+}
+
+template <typename R>
+__global__ void __launch_bounds__(64) k_rtr_fac(ColIn in, int col0, int n, Rtr<R> rt)
+{
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const RadCol rc{&in, col0 + t, in.nlay + 1};
+    const int nl = rc.nrl;
+    const double eps = 1.0e-6;
+    double cf[NL + 3];
+    for (int k = 0; k <= nl + 1; ++k) cf[k] = lw_cldfrc(rc, k);
+    double cld1[NL + 3], cld2[NL + 3], clr1[NL + 3], clr2[NL + 3], cmb1[NL + 3], cmb2[NL + 3];
+    uint64_t lst = 0;
+    // upward factors
+    for (int k = 0; k <= nl + 2; ++k) cld1[k] = cld2[k] = clr1[k] = clr2[k] = cmb1[k] = cmb2[k] = 0.0;
+    bool lk = cf[1] > eps;  // lstu[k]
+    if (lk) lst |= 1ull;
+    double rat1 = 0.0, rat2 = 0.0;
+    for (int k = 1; k <= nl - 1; ++k) {
+        const bool lk1 = cf[k + 1] > eps && cf[k] <= eps;
+        if (lk1) lst |= 1ull << k;
+        if (cf[k] > eps) {
+            if (cf[k + 1] >= cf[k]) {
+                if (lk) {
+                    if (cf[k] < 1.0) clr2[k + 1] = (cf[k + 1] - cf[k]) / (1.0 - cf[k]);
+                    clr2[k] = 0.0;
+                    cld2[k] = 0.0;
+                } else {
+                    const double fmx = fmax(cf[k], cf[k - 1]);
+                    if (cf[k + 1] > fmx) {
+                        clr1[k + 1] = rat2;
+                        clr2[k + 1] = (cf[k + 1] - fmx) / (1.0 - fmx);
+                    } else if (cf[k + 1] < fmx) {
+                        clr1[k + 1] = (cf[k + 1] - cf[k]) / (cf[k - 1] - cf[k]);
+                    } else {
+                        clr1[k + 1] = rat2;
+                    }
+                }
+                if (clr1[k + 1] > 0.0 || clr2[k + 1] > 0.0) rat1 = 1.0, rat2 = 0.0;
+                else rat1 = 0.0, rat2 = 0.0;
+            } else {
+                if (lk) {
+                    cld2[k + 1] = (cf[k] - cf[k + 1]) / cf[k];
+                    clr2[k] = 0.0;
+                    cld2[k] = 0.0;
+                } else {
+                    const double fmn = fmin(cf[k], cf[k - 1]);
+                    if (cf[k + 1] <= fmn) {
+                        cld1[k + 1] = rat1;
+                        cld2[k + 1] = (fmn - cf[k + 1]) / fmn;
+                    } else {
+                        cld1[k + 1] = (cf[k] - cf[k + 1]) / (cf[k] - fmn);
+                    }
+                }
+                if (cld1[k + 1] > 0.0 || cld2[k + 1] > 0.0) rat1 = 0.0, rat2 = 1.0;
+                else rat1 = 0.0, rat2 = 0.0;
+            }
+            cmb1[k + 1] = clr1[k + 1] * cld2[k] * cf[k - 1];
+            cmb2[k + 1] = cld1[k + 1] * clr2[k] * (1.0 - cf[k - 1]);
+        }
+        lk = lk1;
+    }
+    for (int k = 0; k <= nl + 2; ++k) {
+        rt.at(FCLD1U, k, t) = (R)cld1[k], rt.at(FCLD2U, k, t) = (R)cld2[k], rt.at(FCLR1U, k, t) = (R)clr1[k];
+        rt.at(FCLR2U, k, t) = (R)clr2[k], rt.at(FCMB1U, k, t) = (R)cmb1[k], rt.at(FCMB2U, k, t) = (R)cmb2[k];
+    }
+    rt.lstu[t] = lst;
+    // downward factors
+    for (int k = 0; k <= nl + 2; ++k) cld1[k] = cld2[k] = clr1[k] = clr2[k] = cmb1[k] = cmb2[k] = 0.0;
+    lst = 0;
+    lk = cf[nl] > eps;  // lstd[k]
+    if (lk) lst |= 1ull << (nl - 1);
+    rat1 = 0.0, rat2 = 0.0;
+    for (int k = nl; k >= 2; --k) {
+        const bool lkm = cf[k - 1] > eps && cf[k] <= eps;
+        if (lkm) lst |= 1ull << (k - 2);
+        if (cf[k] > eps) {
+            if (cf[k - 1] >= cf[k]) {
+                if (lk) {
+                    if (cf[k] < 1.0) clr2[k - 1] = (cf[k - 1] - cf[k]) / (1.0 - cf[k]);
+                    clr2[k] = 0.0;
+                    cld2[k] = 0.0;
+                } else {
+                    const double fmx = fmax(cf[k], cf[k + 1]);
+                    if (cf[k - 1] > fmx) {
+                        clr1[k - 1] = rat2;
+                        clr2[k - 1] = (cf[k - 1] - fmx) / (1.0 - fmx);
+                    } else if (cf[k - 1] < fmx) {
+                        clr1[k - 1] = (cf[k - 1] - cf[k]) / (cf[k + 1] - cf[k]);
+                    } else {
+                        clr1[k - 1] = rat2;
+                    }
+                }
+                if (clr1[k - 1] > 0.0 || clr2[k - 1] > 0.0) rat1 = 1.0, rat2 = 0.0;
+                else rat1 = 0.0, rat2 = 0.0;
+            } else {
+                if (lk) {
+                    cld2[k - 1] = (cf[k] - cf[k - 1]) / cf[k];
+                    clr2[k] = 0.0;
+                    cld2[k] = 0.0;
+                } else {
+                    const double fmn = fmin(cf[k], cf[k + 1]);
+                    if (cf[k - 1] <= fmn) {
+                        cld1[k - 1] = rat1;
+                        cld2[k - 1] = (fmn - cf[k - 1]) / fmn;
+                    } else {
+                        cld1[k - 1] = (cf[k] - cf[k - 1]) / (cf[k] - fmn);
+                    }
+                }
+                if (cld1[k - 1] > 0.0 || cld2[k - 1] > 0.0) rat1 = 0.0, rat2 = 1.0;
+                else rat1 = 0.0, rat2 = 0.0;
+            }
+            cmb1[k - 1] = clr1[k - 1] * cld2[k] * cf[k + 1];
+            cmb2[k - 1] = cld1[k - 1] * clr2[k] * (1.0 - cf[k + 1]);
+        }
+        lk = lkm;
+    }
+    for (int k = 0; k <= nl + 2; ++k) {
+        rt.at(FCLD1D, k, t) = (R)cld1[k], rt.at(FCLD2D, k, t) = (R)cld2[k], rt.at(FCLR1D, k, t) = (R)clr1[k];
+        rt.at(FCLR2D, k, t) = (R)clr2[k], rt.at(FCMB1D, k, t) = (R)cmb1[k], rt.at(FCMB2D, k, t) = (R)cmb2[k];
+    }
+    rt.lstd[t] = lst;
+}
+
+// total-sky transmittance, source factor and absorptance of odtot (rtrnmr's cloudy branch)
+template <typename R>
+__device__ __forceinline__ void lw_tot3(const Tabs<R> *__restrict__ T, R odtot, R &totfac, R &atrtot, R &trnt)
+{
+    if (odtot < (R)0.06) {
+        totfac = (R)0.166667 * odtot;
+        atrtot = odtot - (R)0.5 * odtot * odtot;
+        trnt = (R)1 - atrtot;
+    } else {
+        const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
+        const int ittot = (int)((R)10000 * tblind + (R)0.5);
+        totfac = T->tfn_tbl[ittot];
+        trnt = T->exp_tbl[ittot];
+        atrtot = (R)1 - trnt;
+    }
+}
+
+// ---- LW band kernel of the NAM route (rtrnmr) ---------------------------------------------------
+template <typename R, int B>
+__global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                            ColScal cs, Acc acc, Rtr<R> rt)
+{
+    constexpr int NG = LW_NG[B];
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const int lcf1 = cs.lcf1[t];
+    const double pwv = cs.pwv[t];
+    const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
+    const double sfe = in.emis[i];
+    const R semis = (R)((sfe > EPS && sfe <= 1.0) ? sfe : 1.0);
+    const uint64_t lstu = rt.lstu[t], lstd = rt.lstd[t];
+    const R eps = (R)1.0e-6;
+
+    // ---- down sweep, top to surface ----
+    R radtotd[NG], totradd[NG], clrradd[NG], rad[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) radtotd[g] = totradd[g] = clrradd[g] = rad[g] = (R)0;
+    double tz_top = rc.tlyr(0);
+    double la_c = log_d(rc.plyr(0));
+    for (int kin = 0; kin < nrl; ++kin) {
+        const int kk = nrl - kin;  // 1-based surface-first layer
+        double tz_bot, la_n = 0.0;
+        if (kin + 1 < nrl) {
+            la_n = log_d(rc.plyr(kin + 1));
+            tz_bot = rc.tlvl(kin + 1, la_n, la_c, log_d(rc.plvl(kin + 1)));
+        } else {
+            tz_bot = rc.tskn();
+        }
+        LwLay<R> L;
+        lw_layer<R>(T, rc, par, kin, la_c, L);
+        R tau[NG], fr[NG];
+        lw_optics<R, B>(T, L, tau, fr);
+        const R blay = planck<R>(T, B, L.tavel);
+        const R dplnkd = planck<R>(T, B, tz_bot) - blay;
+        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R clfr = (R)lw_cldfrc(rc, kk);
+        const bool top = (lstd >> (kk - 1)) & 1;
+        const R c1 = rt.at(FCLR1D, kk - 1, t), d1 = rt.at(FCLD1D, kk - 1, t), m1 = rt.at(FCMB1D, kk - 1, t);
+        const R m2 = rt.at(FCMB2D, kk - 1, t), c2 = rt.at(FCLR2D, kk - 1, t), d2 = rt.at(FCLD2D, kk - 1, t);
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const R gassrcd = fr[g] * (blay + dplnkd * q.gasfac) * q.atrgas;
+            if (top) {
+                totradd[g] = clfr * radtotd[g];
+                clrradd[g] = radtotd[g] - totradd[g];
+                rad[g] = (R)0;
+            }
+            if (clfr >= eps) {
+                const R odcld = sdif * taucld;
+                R totfac, atrtot, trnt;
+                lw_tot3<R>(T, q.odepth + odcld, totfac, atrtot, trnt);
+                const R totsrcd = fr[g] * (blay + dplnkd * totfac) * atrtot;
+                totradd[g] = totradd[g] * trnt + clfr * totsrcd;
+                clrradd[g] = clrradd[g] * q.trng + ((R)1 - clfr) * gassrcd;
+                radtotd[g] = totradd[g] + clrradd[g];
+                s = s + radtotd[g];
+                const R radmod = rad[g] * (c1 * q.trng + d1 * trnt) - m1 * gassrcd + m2 * totsrcd;
+                rad[g] = -radmod + c2 * (clrradd[g] + radmod) - d2 * (totradd[g] - radmod);
+                totradd[g] = totradd[g] + rad[g];
+                clrradd[g] = clrradd[g] - rad[g];
+            } else {
+                radtotd[g] = radtotd[g] * q.trng + gassrcd;
+                s = s + radtotd[g];
+            }
+        }
+        acc.lw_dn[(size_t)(kk - 1) * n + t] += (double)s;
+        tz_top = tz_bot;
+        la_c = la_n;
+    }
+    // ---- up sweep, surface to top ----
+    const R reflct = (R)1 - semis;
+    R radtotu[NG], totradu[NG], clrradu[NG];
+    double tz_bot = rc.tskn();
+    la_c = log_d(rc.plyr(nrl - 1));
+    for (int kin = nrl - 1; kin >= 0; --kin) {
+        const int kk = nrl - kin;
+        double la_m = 0.0;
+        if (kin > 0) {
+            la_m = log_d(rc.plyr(kin - 1));
+            tz_top = rc.tlvl(kin, la_c, la_m, log_d(rc.plvl(kin)));
+        } else {
+            tz_top = rc.tlyr(0);
+        }
+        LwLay<R> L;
+        lw_layer<R>(T, rc, par, kin, la_c, L);
+        R tau[NG], fr[NG];
+        lw_optics<R, B>(T, L, tau, fr);
+        const R blay = planck<R>(T, B, L.tavel);
+        const R dplnku = planck<R>(T, B, tz_top) - blay;
+        if (kk == 1) {  // surface emission and reflection
+            const R pk0 = planck<R>(T, B, rc.tskn());
+            R s0 = (R)0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const R rad0 = semis * fr[g] * pk0;
+                radtotu[g] = rad0 + reflct * radtotd[g];
+                totradu[g] = clrradu[g] = (R)0;
+                rad[g] = (R)0;
+                s0 = s0 + radtotu[g];
+            }
+            acc.lw_up[t] += (double)s0;
+        }
+        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R clfr = (R)lw_cldfrc(rc, kk);
+        const bool bot = (lstu >> (kk - 1)) & 1;
+        const R c1 = rt.at(FCLR1U, kk + 1, t), d1 = rt.at(FCLD1U, kk + 1, t), m1 = rt.at(FCMB1U, kk + 1, t);
+        const R m2 = rt.at(FCMB2U, kk + 1, t), c2 = rt.at(FCLR2U, kk + 1, t), d2 = rt.at(FCLD2U, kk + 1, t);
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const R gasu = fr[g] * (blay + dplnku * q.gasfac) * q.atrgas;
+            if (bot) {
+                totradu[g] = clfr * radtotu[g];
+                clrradu[g] = radtotu[g] - totradu[g];
+                rad[g] = (R)0;
+            }
+            if (clfr >= eps) {
+                const R odcld = sdif * taucld;
+                R totfac, atrtot, trnt;
+                lw_tot3<R>(T, q.odepth + odcld, totfac, atrtot, trnt);
+                const R totu = fr[g] * (blay + dplnku * totfac) * atrtot;
+                totradu[g] = totradu[g] * trnt + clfr * totu;
+                clrradu[g] = clrradu[g] * q.trng + ((R)1 - clfr) * gasu;
+                radtotu[g] = totradu[g] + clrradu[g];
+                s = s + radtotu[g];
+                const R radmod = rad[g] * (c1 * q.trng + d1 * trnt) - m1 * gasu + m2 * totu;
+                rad[g] = -radmod + c2 * (clrradu[g] + radmod) - d2 * (totradu[g] - radmod);
+                totradu[g] = totradu[g] + rad[g];
+                clrradu[g] = clrradu[g] - rad[g];
+            } else {
+                radtotu[g] = radtotu[g] * q.trng + gasu;
+                s = s + radtotu[g];
+            }
+        }
+        acc.lw_up[(size_t)kk * n + t] += (double)s;
+        tz_bot = tz_top;
+        la_c = la_m;
+    }
+}
+
+// ---- SW band kernel of the NAM route (spcvrtc with cldfrc mixing and zcf1/zcf0 weighting) -------
+template <typename R, int B>
+__global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                            ColScal cs, const R *__restrict__ sflux, Acc acc, double *__restrict__ clr)
+{
+    constexpr int NG = SW_NG[B];
+    constexpr int NS = SW_NS[B];
+    const int tt = blockIdx.x * blockDim.x + threadIdx.x;
+    if (tt >= *cs.nday) return;
+    const int t = cs.dlist[tt];
+    const int i = col0 + t;
+    const RadCol rc{&in, i, in.nlay + 1};
+    const int nrl = rc.nrl, nlp1 = nrl + 1;
+    const double czd = in.cosz[i];
+    const R cz = (R)czd, snt = (R)(1.0 / czd);
+    const double ssolar = (par.solcon / S0) * czd;
+    const double alb = in.alb[i];
+    const R asfc = (R)((SW_IDXSFC[B] != 0) ? alb : 0.5 * (alb + alb));
+    const double cf1 = cs.zcf1[t], cf0 = cs.zcf0[t];
+    const bool total = cf1 > EPS;
+    const int lt = cs.sw_laytrop[t];
+    R zsolar[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
+    double *clr_up = clr, *clr_dn = clr + (size_t)(NL + 1) * n;
+
+    R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
+    R lrb[NG][NL], lrd[NG][NL], ltb[NG][NL], ltd[NG][NL], lld[NG][NL];
+    for (int pass = 0; pass < (total ? 2 : 1); ++pass) {
+        // bottom-up: pass 0 clear sky (R/T kept), pass 1 total sky: cloudy R/T mixed by cldfrc
+#pragma unroll
+        for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
+        for (int k = 0; k < nrl; ++k) {
+            const int kin = nrl - 1 - k;
+            double zc1d = 0.0;
+            if (pass == 1) zc1d = rc.cfrac(kin) / cf1;
+            const R zc1 = (R)zc1d, zc0 = (R)(1.0 - zc1d);
+            if (pass == 0 || zc1 > (R)FTINY) {
+                SwLay<R> L;
+                sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+                R tg[NG], tr[NG];
+                sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+                R ctau = (R)0, cssa = (R)1, casy = (R)0;
+                if (pass == 1) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    R rb, rd, tb, td, ld;
+                    layer_rt_eval<R>(T, tr[g], tg[g], pass == 1, ctau, cssa, casy, cz, snt, rb, rd, tb, td, ld);
+                    if (pass == 0) {
+                        lrb[g][k] = rb, lrd[g][k] = rd, ltb[g][k] = tb, ltd[g][k] = td, lld[g][k] = ld;
+                    } else {
+                        lrb[g][k] = zc0 * lrb[g][k] + zc1 * rb;
+                        lrd[g][k] = zc0 * lrd[g][k] + zc1 * rd;
+                        ltb[g][k] = zc0 * ltb[g][k] + zc1 * tb;
+                        ltd[g][k] = zc0 * ltd[g][k] + zc1 * td;
+                        lld[g][k] = zc0 * lld[g][k] + zc1 * ld;
+                    }
+                }
+            }
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * lrd[g][k]);
+                zrupb[g][k + 2] =
+                    lrb[g][k] + (ltd[g][k] * ((ltb[g][k] - lld[g][k]) * zrupd[g][k + 1] + lld[g][k] * zrupb[g][k + 1])) * zden1;
+                zrupd[g][k + 2] = lrd[g][k] + ltd[g][k] * ltd[g][k] * zrupd[g][k + 1] * zden1;
+            }
+        }
+        // top-down and level fluxes; pass 0 keeps the clear band sums, the last pass adds the band
+        R tdbt[NG], tdn[NG], rdnd[NG];
+        R su = (R)0, sd = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            tdbt[g] = (R)1, tdn[g] = (R)1, rdnd[g] = (R)0;
+            const R zden1 = (R)1 / ((R)1 - rdnd[g] * zrupd[g][nlp1]);
+            const R zfu = (tdbt[g] * zrupb[g][nlp1] + (tdn[g] - tdbt[g]) * zrupd[g][nlp1]) * zden1;
+            const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * zrupb[g][nlp1] * rdnd[g]) * zden1;
+            su = su + zsolar[g] * zfu;
+            sd = sd + zsolar[g] * zfd;
+        }
+        for (int s = nrl; s >= 0; --s) {
+            if (s < nrl) {
+                su = (R)0, sd = (R)0;
+#pragma unroll
+                for (int g = 0; g < NG; ++g) {
+                    const int k = s;  // layer below level s + 1
+                    const R tdbt_s = lld[g][k] * tdbt[g];
+                    R tdn_s, rdnd_s;
+                    if (s == nrl - 1) {
+                        tdn_s = ltb[g][k];
+                        rdnd_s = lrd[g][k];
+                    } else {
+                        const R zden1 = (R)1 / ((R)1 - lrd[g][k] * rdnd[g]);
+                        tdn_s = tdbt[g] * ltb[g][k] +
+                                (ltd[g][k] * ((tdn[g] - tdbt[g]) + tdbt[g] * lrb[g][k] * rdnd[g])) * zden1;
+                        rdnd_s = lrd[g][k] + ltd[g][k] * ltd[g][k] * rdnd[g] * zden1;
+                    }
+                    tdbt[g] = tdbt_s, tdn[g] = tdn_s, rdnd[g] = rdnd_s;
+                    const R zden1 = (R)1 / ((R)1 - rdnd_s * zrupd[g][s + 1]);
+                    const R zfu = (tdbt_s * zrupb[g][s + 1] + (tdn_s - tdbt_s) * zrupd[g][s + 1]) * zden1;
+                    const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zrupb[g][s + 1] * rdnd_s) * zden1;
+                    su = su + zsolar[g] * zfu;
+                    sd = sd + zsolar[g] * zfd;
+                }
+            }
+            const size_t o = (size_t)s * n + t;  // level s (surface-first)
+            if (pass == 0 && total) {
+                clr_up[o] = (double)su, clr_dn[o] = (double)sd;
+            } else if (pass == 0) {
+                acc.sw_up[o] += (double)su;
+                acc.sw_dn[o] += (double)sd;
+            } else {
+                acc.sw_up[o] += cf1 * (double)su + cf0 * clr_up[o];
+                acc.sw_dn[o] += cf1 * (double)sd + cf0 * clr_dn[o];
+            }
+        }
+    }
+}
+
+}  // namespace wr

@@ -28,6 +28,7 @@ template <typename R>
 __device__ __forceinline__ void sw_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int kin,
                                          double la, SwLay<R> &L)
 {
+#pragma clang fp contract(off)
     const double tem1 = 100.0 * CON_G;
     const double tem2 = 1.0e-20 * 1.0e3 * CON_AVGD;
     const double pav = rc.plyr(kin), tav = rc.tlyr(kin);
@@ -369,7 +370,7 @@ template <typename R> __device__ __forceinline__ R expm(R zb, const R *__restric
 // behind the 1 - 1e-6 clamp of zssaw, phys_rrtm_sw_spcvrtc.cu:54,117)
 template <typename R>
 __device__ __forceinline__ void layer_rt(R ztau0, R zssa0, R zasy0, R cz, R snt, const R *__restrict__ tbl, R &rb, R &rd,
-                                         R &tb, R &td, R &zldbt)
+                                         R &tb, R &td, R &zldbt, bool *ill = nullptr)
 {
     const R zssaw = fmin((R)ONEMINUS, zssa0 / ztau0);
     const R zasyw = zasy0 / fmax((R)FTINY, zssa0);
@@ -392,6 +393,12 @@ __device__ __forceinline__ void layer_rt(R ztau0, R zssa0, R zasy0, R cz, R snt,
     const R zrp1 = (R)1 + zrp;
     const R zrm1 = (R)1 - zrp;
     R zrpp = (R)1 - zrp * zrp;
+    // FP32 near the two-stream singularity k mu = 1: numerator and denominator of the direct-beam
+    // terms both vanish there, so the caller re-evaluates the layer in FP64
+    if (ill && fabs(zrpp) < (R)1.0e-2) {
+        *ill = true;
+        return;
+    }
     if (fabs(zrpp) < eps1) zrpp = (zrpp < (R)0) ? -eps1 : eps1;
     const R zrkg1 = zrk + zgam1;
     const R zrkg3 = zrk * zgam3;

@@ -207,7 +207,7 @@ def latency_stats(reqs) -> dict:
 
 
 def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed: int = 0,
-                members: int = 8) -> dict:
+                members: int = 8, keep_outputs: bool = False) -> dict:
     """Drive the server with Poisson arrivals for `duration` seconds (real time).
 
     kinds: name -> (cols dict of device tensors, col_id tensor or None).  An arrival picks a kind
@@ -238,6 +238,9 @@ def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed:
             i += 1
         if server.poll() == 0:
             time.sleep(2e-4)
+        if not keep_outputs:              # the consumer takes finished outputs away
+            for r in server.completed:
+                r.out = None
     torch.cuda.synchronize(server.device)
     ncols = sum(r.ncol for r in reqs)
     t_end = max(r.done for r in reqs) if reqs else 0.0
