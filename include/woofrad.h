@@ -82,6 +82,8 @@ typedef struct {
     double *flw_dn;          /* [nlay+2][ncol] LW downward flux                                  */
     double *fsw_up;          /* [nlay+2][ncol] SW upward flux (0 at night)                       */
     double *fsw_dn;          /* [nlay+2][ncol] SW downward flux                                  */
+    double *fsw_dn_sfc_clr;  /* [ncol] optional (NULL: not written): clear-sky surface SW downward
+                                flux of the NAM route (mcica = 0); 0 on the McICA route and at night */
 } wr_outputs;
 
 /* Optional diagnostics for bit-exact checks (any pointer may be NULL).  Radiation layers in the

@@ -130,6 +130,5 @@ void osw_sfluxzen(const OT *t, SwCol *c);
 void osw_cldprop(const OT *t, SwCol *c, int mcica);
 void osw_taumol(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL]);
 void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL], const uint32_t *mask,
-                 int mcica, double *fup, double *fdn);
-
+                 int mcica, double *fup, double *fdn, double *fdn0_sfc);
 #endif

@@ -325,7 +325,9 @@ static void o_column(const OT *t, const wr_columns *in, const wr_params *p, cons
             o_mcica(col, p->seed, 1, 112, nrl, p->iovr, s->cldf, w->swmask, 4);
         else
             memset(w->swmask, 0, sizeof w->swmask);
-        osw_spcvrtc(t, s, w->staug, w->staur, w->swmask, p->mcica, w->fu, w->fd);
+        double fdn0 = 0.0;
+        osw_spcvrtc(t, s, w->staug, w->staur, w->swmask, p->mcica, w->fu, w->fd, &fdn0);
+        if (out->fsw_dn_sfc_clr) out->fsw_dn_sfc_clr[i] = fdn0;
         for (int L = 0; L <= nrl; ++L) {
             out->fsw_up[(size_t)L * nc + i] = w->fu[nrl - L];
             out->fsw_dn[(size_t)L * nc + i] = w->fd[nrl - L];
@@ -348,6 +350,7 @@ static void o_column(const OT *t, const wr_columns *in, const wr_params *p, cons
         }
     } else {
         for (int L = 0; L <= nrl; ++L) out->fsw_up[(size_t)L * nc + i] = out->fsw_dn[(size_t)L * nc + i] = 0.0;
+        if (out->fsw_dn_sfc_clr) out->fsw_dn_sfc_clr[i] = 0.0;
         for (int k0 = 0; k0 < nm; ++k0) out->hr_sw[(size_t)k0 * nc + i] = 0.0;
         if (dg) {
             for (int k = 0; k < nrl; ++k) {

@@ -667,7 +667,7 @@ static const int SW_IDXSFC[14] = {1, 1, 1, 1, 1, 1, 1, 1, 0, 2, 2, 2, 2, 1}; /* 
  * mcica = 0: the reference's spcvrtc (cldfrc mixing, cf1 weighting); mcica = 1: the same two-stream
  * with the per-g 0/1 McICA mask in place of cldfrc and no cf1 weighting. */
 void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL], const uint32_t *mask,
-                 int mcica, double *fup, double *fdn)
+                 int mcica, double *fup, double *fdn, double *fdn0_sfc)
 {
     const int nlay = c->nlay, nlp1 = nlay + 1;
     const double *tbl = t->sw_exp_tbl;
@@ -777,5 +777,12 @@ void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*tau
         }
         fup[k] = su;
         fdn[k] = sd;
+    }
+    /* clear-sky surface downward flux (NAM route; the McICA route reports 0) */
+    if (fdn0_sfc) {
+        double s0 = 0.0;
+        if (!mcica)
+            for (int b = 0; b < 14; ++b) s0 = s0 + fxdn0[b][0];
+        *fdn0_sfc = s0;
     }
 }

@@ -30,7 +30,7 @@ class WrParams(C.Structure):
 
 class WrOutputs(C.Structure):
     _fields_ = [("hr_lw", _dp), ("hr_sw", _dp), ("flw_up", _dp), ("flw_dn", _dp),
-                ("fsw_up", _dp), ("fsw_dn", _dp)]
+                ("fsw_up", _dp), ("fsw_dn", _dp), ("fsw_dn_sfc_clr", _dp)]
 
 
 class WrDiag(C.Structure):
@@ -45,7 +45,7 @@ class WrSurrogate(C.Structure):
 
 
 INPUT_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi", "t_sfc", "emis", "alb", "cosz")
-OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn")
+OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn", "fsw_dn_sfc_clr")
 DIAG_FIELDS = ("lw_jp", "lw_jt", "lw_jt1", "lw_laytrop", "sw_jp", "sw_jt", "sw_jt1", "sw_laytrop",
                "lw_mask", "sw_mask")
 
@@ -66,7 +66,8 @@ def make_params(mcica: int = 1, iovr: int = 1, seed: int = 0, solcon: float = 13
 
 def output_shapes(ncol: int, nlay: int) -> dict:
     return {"hr_lw": (nlay, ncol), "hr_sw": (nlay, ncol), "flw_up": (nlay + 2, ncol),
-            "flw_dn": (nlay + 2, ncol), "fsw_up": (nlay + 2, ncol), "fsw_dn": (nlay + 2, ncol)}
+            "flw_dn": (nlay + 2, ncol), "fsw_up": (nlay + 2, ncol), "fsw_dn": (nlay + 2, ncol),
+            "fsw_dn_sfc_clr": (ncol,)}
 
 
 def diag_shapes(ncol: int, nlay: int) -> dict:

@@ -462,7 +462,7 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     const RunPar par = to_par(p);
     OutPtrs op;
     op.hr_lw = o->hr_lw, op.hr_sw = o->hr_sw, op.flw_up = o->flw_up, op.flw_dn = o->flw_dn;
-    op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn;
+    op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn, op.fsw_dn_sfc_clr = o->fsw_dn_sfc_clr;
     // equal tiles of at most max_cols columns (no small ragged tail tile)
     const int ntiles = (in->ncol + c->max_cols - 1) / c->max_cols;
     const int per = (in->ncol + ntiles - 1) / ntiles;
@@ -525,7 +525,7 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     }
     const size_t m = (size_t)max_cols;
     if (rc == WR_OK) {
-        if (hipMalloc((void **)&c->acc, sizeof(double) * 4 * (NL + 1) * m) != hipSuccess ||
+        if (hipMalloc((void **)&c->acc, sizeof(double) * (4 * (NL + 1) + 1) * m) != hipSuccess ||
             hipMalloc((void **)&c->iws, sizeof(int) * 6 * m) != hipSuccess ||
             hipMalloc((void **)&c->dws, sizeof(double) * 3 * m) != hipSuccess ||
             hipMalloc(&c->rtr, (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m) != hipSuccess ||

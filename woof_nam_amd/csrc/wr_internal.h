@@ -15,7 +15,7 @@ struct wr_ctx {
     wr::Tabs<double> host_d;         // host images (device pointers inside)
     wr::Tabs<float> host_f;
     // tile work space
-    double *acc;                     // 4 x [nl_max + 2][max_cols]
+    double *acc;                     // 4 x [NL + 1][max_cols] + [max_cols]
     int *iws;                        // 6 x [max_cols]
     double *dws;                     // 3 x [max_cols]
     void *rtr;                       // maximum-random overlap factors of the NAM LW route
@@ -77,6 +77,7 @@ inline Acc tile_acc(wr_ctx *c, int n, int nrl)
     a.lw_dn = c->acc + per;
     a.sw_up = c->acc + 2 * per;
     a.sw_dn = c->acc + 3 * per;
+    a.sw_clr0 = c->acc + 4 * per;
     return a;
 }
 }  // namespace wr

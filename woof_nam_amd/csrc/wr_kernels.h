@@ -17,8 +17,8 @@
 
 namespace wr {
 
-struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each
-    double *lw_up, *lw_dn, *sw_up, *sw_dn;
+struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each; sw_clr0 [n]
+    double *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
 };
 struct ColScal {
     int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
@@ -538,7 +538,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
 
 // ---- post: FP64 fluxes -> heating rates, top-first outputs -------------------------------------------
 struct OutPtrs {
-    double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn;
+    double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn, *fsw_dn_sfc_clr;
 };
 static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, Acc acc, OutPtrs out)
 {
@@ -578,9 +578,11 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
             if (k < nm) out.hr_sw[(size_t)(nm - 1 - k) * nc + i] = h;
             fu0 = fu1, fd0 = fd1;
         }
+        if (out.fsw_dn_sfc_clr) out.fsw_dn_sfc_clr[i] = acc.sw_clr0[t];
     } else {
         for (int L = 0; L <= nrl; ++L) out.fsw_up[(size_t)L * nc + i] = 0.0, out.fsw_dn[(size_t)L * nc + i] = 0.0;
         for (int k0 = 0; k0 < nm; ++k0) out.hr_sw[(size_t)k0 * nc + i] = 0.0;
+        if (out.fsw_dn_sfc_clr) out.fsw_dn_sfc_clr[i] = 0.0;
     }
 }
 

@@ -45,7 +45,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
     const int nrl = in.nlay + 1;
     ColScal cs = colscal(c);
     Acc acc = tile_acc(c, n, nrl);
-    const size_t nacc = (size_t)4 * (nrl + 1) * n;
+    const size_t nacc = (size_t)(4 * (nrl + 1) + 1) * n;
     tbeg(c, KID_ZERO, s);
     hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, c->acc, nacc);
     tend(c, s);
@@ -101,7 +101,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     const int nrl = in.nlay + 1;
     ColScal cs = colscal(c);
     Acc acc = tile_acc(c, n, nrl);
-    const size_t nacc = (size_t)4 * (nrl + 1) * n;
+    const size_t nacc = (size_t)(4 * (nrl + 1) + 1) * n;
     Rtr<R> rt;
     rt.f = (R *)c->rtr;
     rt.lstu = (uint64_t *)((char *)c->rtr + sizeof(double) * 12 * (NL + 3) * (size_t)c->max_cols);

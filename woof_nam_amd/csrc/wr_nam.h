@@ -420,6 +420,7 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
                 }
             }
             const size_t o = (size_t)s * n + t;  // level s (surface-first)
+            if (pass == 0 && s == 0) acc.sw_clr0[t] += (double)sd;
             if (pass == 0 && total) {
                 clr_up[o] = (double)su, clr_dn[o] = (double)sd;
             } else if (pass == 0) {
