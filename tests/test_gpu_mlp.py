@@ -116,12 +116,13 @@ def test_surrogate_path_runs_and_falls_back(gpu_device):
     ref, _ = rad.run(dev, precision="fp64")
     torch.cuda.synchronize()
     flag = info["fallback"].cpu().numpy()
-    assert flag[7] == 1 and flag.sum() < 20, flag.sum()
+    assert flag[7] == 1 and flag.sum() < 20, (int(flag[7]), int(flag.sum()))
     for k in abi.OUTPUT_FIELDS:
         assert torch.isfinite(out[k]).all(), k
         tol = 1e-3 / 86400 if k.startswith("hr") else 1e-2      # 1e-3 K/day, 1e-2 W/m2
-        assert float((out[k][:, 7] - mix[k][:, 7]).abs().max()) <= tol, k
+        d = float((out[k][..., 7] - mix[k][..., 7]).abs().max())
+        assert d <= tol, (k, d, tol)
     hr = float(torch.sqrt(torch.mean((out["hr_lw"] - ref["hr_lw"]) ** 2))) * 86400
     print("surrogate LW HR RMS K/day (short training):", hr)
-    assert hr < 5.0
+    assert hr < 5.0, hr
     rad.close()

@@ -6,11 +6,12 @@ loading fails with an ImportError naming the missing file.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from pathlib import Path
 
 from . import abi
 
-LIB_PATH = Path(__file__).resolve().parent / "libwoofrad.so"
+LIB_PATH = Path(os.environ.get("WR_LIB", Path(__file__).resolve().parent / "libwoofrad.so"))
 TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 
 # symbols include/woofrad.h declares (the CPU test suite checks every one is exported)

@@ -46,26 +46,29 @@ UNIT_FLAGS = {
 
 
 def _compile(src: Path, obj: Path) -> None:
-    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), "-c", str(src), "-o", str(obj)]
+    extra = os.environ.get("WR_CFLAGS_EXTRA", "").split()      # tuning builds, e.g. -DWR_BAND_WAVES=4
+    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), *extra, "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")
 
 
-def build(verbose: bool = False) -> Path:
-    OBJ.mkdir(exist_ok=True)
-    objs = [OBJ / (Path(u).stem + ".o") for u in UNITS]
+def build(verbose: bool = False, out: Path | None = None) -> Path:
+    out = Path(out or os.environ.get("WR_LIB_OUT", LIB))
+    objdir = OBJ if out == LIB else out.parent / (out.stem + "_obj")
+    objdir.mkdir(exist_ok=True)
+    objs = [objdir / (Path(u).stem + ".o") for u in UNITS]
     with ThreadPoolExecutor(max_workers=min(len(UNITS), 8)) as ex:
         list(ex.map(lambda p: _compile(CSRC / p[0], p[1]), zip(UNITS, objs)))
-    tmp = LIB.with_suffix(".so.tmp")
+    tmp = out.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{r.stderr}")
-    os.replace(tmp, LIB)
+    os.replace(tmp, out)
     if verbose:
-        print(f"built {LIB}")
-    return LIB
+        print(f"built {out}")
+    return out
 
 
 if __name__ == "__main__":

@@ -15,6 +15,11 @@
 #include "wr_lw.h"
 #include "wr_sw.h"
 
+// minimum waves per SIMD the band kernels are compiled for (register budget 512 / WR_BAND_WAVES)
+#ifndef WR_BAND_WAVES
+#define WR_BAND_WAVES 1
+#endif
+
 namespace wr {
 
 struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each; sw_clr0 [n]
@@ -176,7 +181,7 @@ template <typename R> __device__ __forceinline__ GasT<R> lw_gas(const Tabs<R> *_
         q.gasfac = (R)0.166667 * od;
     } else {
         const R tblind = od / ((R)(1.0 / 0.278) + od);
-        const int itgas = (int)((R)10000 * tblind + (R)0.5);
+        const int itgas = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
         q.trng = T->exp_tbl[itgas];
         q.atrgas = (R)1 - q.trng;
         q.gasfac = T->tfn_tbl[itgas];
@@ -192,7 +197,7 @@ template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__re
         atrtot = odtot - (R)0.5 * odtot * odtot;
     } else {
         const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
-        const int ittot = (int)((R)10000 * tblind + (R)0.5);
+        const int ittot = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
         totfac = T->tfn_tbl[ittot];
         atrtot = (R)1 - T->exp_tbl[ittot];
     }
@@ -234,7 +239,7 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCo
 }
 
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, WR_BAND_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, Acc acc, Surr sg)
 {
     constexpr int NG = LW_NG[B];
@@ -443,7 +448,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
 }
 
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, WR_BAND_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg)
 {
     constexpr int NG = SW_NG[B];
@@ -473,10 +478,10 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
             return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
         });
     }
-    // bottom-up: reflectance of the medium below each level (swflux sweep 1); the layer R/T of every
-    // g-point are kept (private memory) for the top-down sweep instead of being recomputed
+    // bottom-up: reflectance of the medium below each level (swflux sweep 1); the layer R/T are
+    // recomputed in the top-down sweep (keeping them per g-point and layer in private memory needs
+    // tens of KB of scratch per lane, which every HIP queue running the kernel must back)
     R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
-    R lrb[NG][NL], lrd[NG][NL], ltb[NG][NL], ltd[NG][NL], lld[NG][NL];
 #pragma unroll
     for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
     for (int k = 0; k < nrl; ++k) {
@@ -484,7 +489,6 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
         sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            lrb[g][k] = o.rb[g], lrd[g][k] = o.rd[g], ltb[g][k] = o.tb[g], ltd[g][k] = o.td[g], lld[g][k] = o.ld[g];
             const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
             zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
             zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
@@ -506,11 +510,7 @@ __global__ void __launch_bounds__(64) k_sw(const Tabs<R> *__restrict__ T, ColIn 
     acc.sw_dn[(size_t)nrl * n + t] += (double)sd;
     for (int s = nrl; s >= 1; --s) {
         SwLayerRT<R, B> o;
-#pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            o.rb[g] = lrb[g][s - 1], o.rd[g] = lrd[g][s - 1], o.tb[g] = ltb[g][s - 1], o.td[g] = ltd[g][s - 1];
-            o.ld[g] = lld[g][s - 1];
-        }
+        sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table);
         su = (R)0, sd = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {

@@ -159,7 +159,7 @@ __device__ __forceinline__ void lw_tot3(const Tabs<R> *__restrict__ T, R odtot, 
         trnt = (R)1 - atrtot;
     } else {
         const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
-        const int ittot = (int)((R)10000 * tblind + (R)0.5);
+        const int ittot = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
         totfac = T->tfn_tbl[ittot];
         trnt = T->exp_tbl[ittot];
         atrtot = (R)1 - trnt;
@@ -316,6 +316,39 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
 }
 
 // ---- SW band kernel of the NAM route (spcvrtc with cldfrc mixing and zcf1/zcf0 weighting) -------
+// R/T of layer k (surface-first): pass 0 clear sky; pass 1 total sky, the cloudy R/T mixed with the
+// clear ones by the layer cloud fraction zc1 = cfrac / zcf1 (phys_rrtm_sw_spcvrtc.cu, cloudy block)
+template <typename R, int B>
+__device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
+                                          int lt, int pass, double cf1, R cz, R snt, SwLayerRT<R, B> &o)
+{
+    constexpr int NG = SW_NG[B];
+    const int kin = rc.nrl - 1 - k;
+    SwLay<R> L;
+    sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+    R tg[NG], tr[NG];
+    sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+#pragma unroll
+    for (int g = 0; g < NG; ++g)
+        layer_rt_eval<R>(T, tr[g], tg[g], false, (R)0, (R)1, (R)0, cz, snt, o.rb[g], o.rd[g], o.tb[g], o.td[g], o.ld[g]);
+    if (pass == 0) return;
+    const double zc1d = rc.cfrac(kin) / cf1;
+    if (!(zc1d > FTINY)) return;
+    const R zc1 = (R)zc1d, zc0 = (R)(1.0 - zc1d);
+    R ctau = (R)0, cssa = (R)1, casy = (R)0;
+    sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
+#pragma unroll
+    for (int g = 0; g < NG; ++g) {
+        R rb, rd, tb, td, ld;
+        layer_rt_eval<R>(T, tr[g], tg[g], true, ctau, cssa, casy, cz, snt, rb, rd, tb, td, ld);
+        o.rb[g] = zc0 * o.rb[g] + zc1 * rb;
+        o.rd[g] = zc0 * o.rd[g] + zc1 * rd;
+        o.tb[g] = zc0 * o.tb[g] + zc1 * tb;
+        o.td[g] = zc0 * o.td[g] + zc1 * td;
+        o.ld[g] = zc0 * o.ld[g] + zc1 * ld;
+    }
+}
+
 template <typename R, int B>
 __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                             ColScal cs, const R *__restrict__ sflux, Acc acc, double *__restrict__ clr)
@@ -342,47 +375,21 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
     double *clr_up = clr, *clr_dn = clr + (size_t)(NL + 1) * n;
 
     R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
-    R lrb[NG][NL], lrd[NG][NL], ltb[NG][NL], ltd[NG][NL], lld[NG][NL];
     for (int pass = 0; pass < (total ? 2 : 1); ++pass) {
-        // bottom-up: pass 0 clear sky (R/T kept), pass 1 total sky: cloudy R/T mixed by cldfrc
+        // bottom-up (swflux sweep 1)
 #pragma unroll
         for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
         for (int k = 0; k < nrl; ++k) {
-            const int kin = nrl - 1 - k;
-            double zc1d = 0.0;
-            if (pass == 1) zc1d = rc.cfrac(kin) / cf1;
-            const R zc1 = (R)zc1d, zc0 = (R)(1.0 - zc1d);
-            if (pass == 0 || zc1 > (R)FTINY) {
-                SwLay<R> L;
-                sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
-                R tg[NG], tr[NG];
-                sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
-                R ctau = (R)0, cssa = (R)1, casy = (R)0;
-                if (pass == 1) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
-#pragma unroll
-                for (int g = 0; g < NG; ++g) {
-                    R rb, rd, tb, td, ld;
-                    layer_rt_eval<R>(T, tr[g], tg[g], pass == 1, ctau, cssa, casy, cz, snt, rb, rd, tb, td, ld);
-                    if (pass == 0) {
-                        lrb[g][k] = rb, lrd[g][k] = rd, ltb[g][k] = tb, ltd[g][k] = td, lld[g][k] = ld;
-                    } else {
-                        lrb[g][k] = zc0 * lrb[g][k] + zc1 * rb;
-                        lrd[g][k] = zc0 * lrd[g][k] + zc1 * rd;
-                        ltb[g][k] = zc0 * ltb[g][k] + zc1 * tb;
-                        ltd[g][k] = zc0 * ltd[g][k] + zc1 * td;
-                        lld[g][k] = zc0 * lld[g][k] + zc1 * ld;
-                    }
-                }
-            }
+            SwLayerRT<R, B> o;
+            swr_layer<R, B>(T, rc, par, k, lt, pass, cf1, cz, snt, o);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * lrd[g][k]);
-                zrupb[g][k + 2] =
-                    lrb[g][k] + (ltd[g][k] * ((ltb[g][k] - lld[g][k]) * zrupd[g][k + 1] + lld[g][k] * zrupb[g][k + 1])) * zden1;
-                zrupd[g][k + 2] = lrd[g][k] + ltd[g][k] * ltd[g][k] * zrupd[g][k + 1] * zden1;
+                const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
+                zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
+                zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
             }
         }
-        // top-down and level fluxes; pass 0 keeps the clear band sums, the last pass adds the band
+        // top-down and level fluxes (layer R/T recomputed); pass 0 keeps the clear band sums
         R tdbt[NG], tdn[NG], rdnd[NG];
         R su = (R)0, sd = (R)0;
 #pragma unroll
@@ -396,20 +403,20 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
         }
         for (int s = nrl; s >= 0; --s) {
             if (s < nrl) {
+                SwLayerRT<R, B> o;
+                swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o);
                 su = (R)0, sd = (R)0;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
-                    const int k = s;  // layer below level s + 1
-                    const R tdbt_s = lld[g][k] * tdbt[g];
+                    const R tdbt_s = o.ld[g] * tdbt[g];
                     R tdn_s, rdnd_s;
                     if (s == nrl - 1) {
-                        tdn_s = ltb[g][k];
-                        rdnd_s = lrd[g][k];
+                        tdn_s = o.tb[g];
+                        rdnd_s = o.rd[g];
                     } else {
-                        const R zden1 = (R)1 / ((R)1 - lrd[g][k] * rdnd[g]);
-                        tdn_s = tdbt[g] * ltb[g][k] +
-                                (ltd[g][k] * ((tdn[g] - tdbt[g]) + tdbt[g] * lrb[g][k] * rdnd[g])) * zden1;
-                        rdnd_s = lrd[g][k] + ltd[g][k] * ltd[g][k] * rdnd[g] * zden1;
+                        const R zden1 = (R)1 / ((R)1 - o.rd[g] * rdnd[g]);
+                        tdn_s = tdbt[g] * o.tb[g] + (o.td[g] * ((tdn[g] - tdbt[g]) + tdbt[g] * o.rb[g] * rdnd[g])) * zden1;
+                        rdnd_s = o.rd[g] + o.td[g] * o.td[g] * rdnd[g] * zden1;
                     }
                     tdbt[g] = tdbt_s, tdn[g] = tdn_s, rdnd[g] = rdnd_s;
                     const R zden1 = (R)1 / ((R)1 - rdnd_s * zrupd[g][s + 1]);
