@@ -62,8 +62,10 @@ def test_hook_matches_column_call(gpu_device):
     pd = args["PD"].double().reshape(-1)[c0:c1]
     direct = torch.from_numpy(sg1).to(gpu_device)[:, None] * pdtop + torch.from_numpy(sg2).to(gpu_device)[:, None] \
         * pd[None, :] + gen.PT
+    # the driver hands PDSG1/DSG2 over as float32 vectors (as the reference's driver does), so the
+    # cumulative interface pressures agree with the float64 sigma formula to float32 rounding
     err = float((cols["p_lev"] - direct).abs().max())
-    assert err < 1.0e-6, f"interface pressure mismatch {err:.3e} Pa"
+    assert err < 1.0e-2, f"interface pressure mismatch {err:.3e} Pa"
     glat = args["GLAT"].reshape(-1)[c0:c1]
     glon = args["GLON"].reshape(-1)[c0:c1]
     cols["cosz"] = solar.coszen(args["JDAT"], 3 * 60.0, 3 * 60.0, glat, glon)

@@ -463,9 +463,10 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     OutPtrs op;
     op.hr_lw = o->hr_lw, op.hr_sw = o->hr_sw, op.flw_up = o->flw_up, op.flw_dn = o->flw_dn;
     op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn, op.fsw_dn_sfc_clr = o->fsw_dn_sfc_clr;
-    // equal tiles of at most max_cols columns (no small ragged tail tile)
-    const int ntiles = (in->ncol + c->max_cols - 1) / c->max_cols;
-    const int per = (in->ncol + ntiles - 1) / ntiles;
+    // full tiles of max_cols columns and one remainder: the band kernels run one 64-column wave per
+    // workgroup at low occupancy, so tiles that are multiples of 65,536 columns (one wave per SIMD on
+    // 1,024 SIMDs) leave no partly filled round of waves except in the remainder tile
+    const int per = c->max_cols;
     for (int col0 = 0; col0 < in->ncol; col0 += per) {
         const int n = (in->ncol - col0 < per) ? in->ncol - col0 : per;
         if (mode == MODE_SURR)

@@ -15,9 +15,12 @@
 #include "wr_lw.h"
 #include "wr_sw.h"
 
-// minimum waves per SIMD the band kernels are compiled for (register budget 512 / WR_BAND_WAVES)
-#ifndef WR_BAND_WAVES
-#define WR_BAND_WAVES 1
+// minimum waves per SIMD the LW and SW band kernels are compiled for (register budget 512 / waves)
+#ifndef WR_LW_WAVES
+#define WR_LW_WAVES 1
+#endif
+#ifndef WR_SW_WAVES
+#define WR_SW_WAVES 1
 #endif
 
 namespace wr {
@@ -239,7 +242,7 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCo
 }
 
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, WR_BAND_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, Acc acc, Surr sg)
 {
     constexpr int NG = LW_NG[B];
@@ -448,7 +451,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
 }
 
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, WR_BAND_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg)
 {
     constexpr int NG = SW_NG[B];
