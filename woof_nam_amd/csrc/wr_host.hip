@@ -526,14 +526,25 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     }
     const size_t m = (size_t)max_cols;
     if (rc == WR_OK) {
-        if (hipMalloc((void **)&c->acc, sizeof(double) * (4 * (NL + 1) + 1) * m) != hipSuccess ||
+        if (hipMalloc((void **)&c->acc, sizeof(double) * (60 * (NL + 1) + 14) * m) != hipSuccess ||
             hipMalloc((void **)&c->iws, sizeof(int) * 6 * m) != hipSuccess ||
             hipMalloc((void **)&c->dws, sizeof(double) * 3 * m) != hipSuccess ||
             hipMalloc(&c->rtr, (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m) != hipSuccess ||
-            hipMalloc((void **)&c->swc, sizeof(double) * 2 * (NL + 1) * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_d, sizeof(double) * 112 * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_f, sizeof(float) * 112 * m) != hipSuccess)
             rc = WR_ERR_NOMEM;
+    }
+    if (rc == WR_OK) {
+        // WR_BAND_STREAMS=0 keeps the band kernels on the caller's stream
+        const char *e = getenv("WR_BAND_STREAMS");
+        int na = e ? atoi(e) : WR_NAUX;
+        c->naux = na < 0 ? 0 : (na > WR_NAUX ? WR_NAUX : na);
+        for (int q = 0; q < c->naux && rc == WR_OK; ++q)
+            if (hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming) != hipSuccess)
+                rc = WR_ERR_HIP;
+        if (rc == WR_OK && c->naux > 0 && hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess)
+            rc = WR_ERR_HIP;
     }
     if (rc == WR_OK && hipDeviceSynchronize() != hipSuccess) rc = WR_ERR_HIP;
     if (rc != WR_OK) {
@@ -707,7 +718,6 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->iws);
     (void)hipFree(c->dws);
     (void)hipFree(c->rtr);
-    (void)hipFree(c->swc);
     (void)hipFree(c->sflux_d);
     (void)hipFree(c->sflux_f);
     (void)hipFree(c->sx);
@@ -715,6 +725,11 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->sh1);
     (void)hipFree(c->sh2);
     (void)hipFree(c->sflag);
+    for (int q = 0; q < WR_NAUX; ++q) {
+        if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
+        if (c->join[q]) (void)hipEventDestroy(c->join[q]);
+    }
+    if (c->fork) (void)hipEventDestroy(c->fork);
     if (c->ev)
         for (int j = 0; j < 2 * c->ev_cap; ++j)
             if (c->ev[j]) (void)hipEventDestroy(c->ev[j]);

@@ -5,6 +5,8 @@
 #include "../../include/woofrad.h"
 #include "wr_kernels.h"
 
+#define WR_NAUX 4  // auxiliary streams the band kernels of a tile are spread over
+
 struct wr_ctx {
     int device;
     int max_cols;
@@ -15,11 +17,10 @@ struct wr_ctx {
     wr::Tabs<double> host_d;         // host images (device pointers inside)
     wr::Tabs<float> host_f;
     // tile work space
-    double *acc;                     // 4 x [NL + 1][max_cols] + [max_cols]
+    double *acc;                     // band flux planes: 60 x [NL + 1][max_cols] + 14 x [max_cols]
     int *iws;                        // 6 x [max_cols]
     double *dws;                     // 3 x [max_cols]
     void *rtr;                       // maximum-random overlap factors of the NAM LW route
-    double *swc;                     // clear-sky band fluxes of the NAM SW route, 2 x [NL+1][max_cols]
     double *sflux_d;                 // [112][max_cols]
     float *sflux_f;
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
@@ -28,6 +29,10 @@ struct wr_ctx {
     float *sx, *sy;                  // features [S][8], outputs [504][S]
     void *sh1, *sh2;                 // hidden activations, bfloat16 [S][128]
     int *sflag;                      // [max_cols] per-column table-optics flag
+    // band-kernel streams (default priority, non-blocking) and their fork/join events
+    int naux;
+    hipStream_t aux[WR_NAUX];
+    hipEvent_t fork, join[WR_NAUX];
     // optional per-launch event timing (wr_timing)
     int timing, ev_cap, ev_n;
     hipEvent_t *ev;                  // 2 x ev_cap
@@ -68,17 +73,6 @@ inline ColScal colscal(wr_ctx *c)
     s.zcf1 = c->dws + 1 * m;
     s.zcf0 = c->dws + 2 * m;
     return s;
-}
-inline Acc tile_acc(wr_ctx *c, int n, int nrl)
-{
-    const size_t per = (size_t)(nrl + 1) * n;
-    Acc a;
-    a.lw_up = c->acc;
-    a.lw_dn = c->acc + per;
-    a.sw_up = c->acc + 2 * per;
-    a.sw_dn = c->acc + 3 * per;
-    a.sw_clr0 = c->acc + 4 * per;
-    return a;
 }
 }  // namespace wr
 

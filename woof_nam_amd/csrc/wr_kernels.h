@@ -25,9 +25,33 @@
 
 namespace wr {
 
-struct Acc {  // FP64 level accumulators of one tile, [nrl+1][n] each; sw_clr0 [n]
+// FP64 level fluxes of ONE band for one tile ([nrl+1][n] planes, written once per level; sw_clr0 [n])
+struct Acc {
     double *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
 };
+// All band planes of a tile: LW band b at planes 2b, 2b+1; SW band b at planes 32+2b, 33+2b; the
+// clear-sky surface SW of SW band b at base + 60 P + b n.  k_post sums the bands in band order, which
+// reproduces the sequential accumulation of the reference bit for bit.
+struct BandAcc {
+    double *base;
+    long long P;  // (nrl + 1) * n
+    int n;
+    unsigned lw_mask, sw_mask;  // bands present (bit b)
+};
+__host__ __device__ inline Acc band_acc(const BandAcc &ba, int id)  // id 0..15 LW, 16..29 SW
+{
+    Acc a = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    if (id < 16) {
+        a.lw_up = ba.base + (size_t)(2 * id) * ba.P;
+        a.lw_dn = ba.base + (size_t)(2 * id + 1) * ba.P;
+    } else {
+        const int b = id - 16;
+        a.sw_up = ba.base + (size_t)(32 + 2 * b) * ba.P;
+        a.sw_dn = ba.base + (size_t)(33 + 2 * b) * ba.P;
+        a.sw_clr0 = ba.base + (size_t)60 * ba.P + (size_t)b * ba.n;
+    }
+    return a;
+}
 struct ColScal {
     int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
     double *pwv, *zcf1, *zcf0;
@@ -161,11 +185,6 @@ static __global__ void __launch_bounds__(1024) k_daylist(const int *__restrict__
     if (tid == 1023) *nday = part[1023];
 }
 
-static __global__ void k_zero(double *__restrict__ a, size_t n)
-{
-    for (size_t j = blockIdx.x * (size_t)blockDim.x + threadIdx.x; j < n; j += (size_t)gridDim.x * blockDim.x) a[j] = 0.0;
-}
-
 // ---- LW band kernel (rtrnmc, McICA) -------------------------------------------------------------
 constexpr double LW_A0[16] = {1.66, 1.55, 1.58, 1.66, 1.54, 1.454, 1.89, 1.33, 1.668, 1.66, 1.66, 1.66, 1.66, 1.66, 1.66, 1.66};
 constexpr double LW_A1[16] = {0.00, 0.25, 0.22, 0.00, 0.13, 0.446, -0.10, 0.40, -0.006, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00, 0.00};
@@ -264,6 +283,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
     for (int g = 0; g < NG; ++g)
         mb[g] = mcica_g(col, seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
 
+    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
     // ---- down sweep, top to surface ----
     R rad[NG];
 #pragma unroll
@@ -312,7 +332,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             }
             s = s + rad[g];
         }
-        acc.lw_dn[(size_t)k * n + t] += (double)s;
+        acc.lw_dn[(size_t)k * n + t] = (double)s;
         tz_top = tz_bot;
         la_c = la_n;
     }
@@ -349,7 +369,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
                 rad[g] = rad0 + reflct * rad[g];
                 s0 = s0 + rad[g];
             }
-            acc.lw_up[t] += (double)s0;
+            acc.lw_up[t] = (double)s0;
         }
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         R s = (R)0;
@@ -373,7 +393,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             }
             s = s + rad[g];
         }
-        acc.lw_up[(size_t)(k + 1) * n + t] += (double)s;
+        acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
         tz_bot = tz_top;
         la_c = la_m;
     }
@@ -509,8 +529,9 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
         su = su + zsolar[g] * zfu;
         sd = sd + zsolar[g] * zfd;
     }
-    acc.sw_up[(size_t)nrl * n + t] += (double)su;
-    acc.sw_dn[(size_t)nrl * n + t] += (double)sd;
+    acc.sw_up[(size_t)nrl * n + t] = (double)su;
+    acc.sw_dn[(size_t)nrl * n + t] = (double)sd;
+    acc.sw_clr0[t] = 0.0;  // clear-sky surface flux: NAM route only
     for (int s = nrl; s >= 1; --s) {
         SwLayerRT<R, B> o;
         sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table);
@@ -534,8 +555,8 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
             su = su + zsolar[g] * zfu;
             sd = sd + zsolar[g] * zfd;
         }
-        acc.sw_up[(size_t)(s - 1) * n + t] += (double)su;
-        acc.sw_dn[(size_t)(s - 1) * n + t] += (double)sd;
+        acc.sw_up[(size_t)(s - 1) * n + t] = (double)su;
+        acc.sw_dn[(size_t)(s - 1) * n + t] = (double)sd;
     }
 }
 
@@ -543,7 +564,7 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
 struct OutPtrs {
     double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn, *fsw_dn_sfc_clr;
 };
-static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, Acc acc, OutPtrs out)
+static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, BandAcc ba, OutPtrs out)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -553,11 +574,25 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
     const int nrl = rc.nrl, nm = in.nlay;
     const size_t nc = (size_t)in.ncol;
     const double flxfac = 0.5 * FLUXFAC;
-    double fprev = acc.lw_up[t] * flxfac - acc.lw_dn[t] * flxfac;
-    out.flw_up[(size_t)nrl * nc + i] = acc.lw_up[t] * flxfac;
-    out.flw_dn[(size_t)nrl * nc + i] = acc.lw_dn[t] * flxfac;
+    // band sums in band order (the order the reference accumulates them)
+    auto lwsum = [&](int plane0, int k) {
+        double v = 0.0;
+        for (int b = 0; b < 16; ++b)
+            if ((ba.lw_mask >> b) & 1) v = v + ba.base[(size_t)(2 * b + plane0) * ba.P + (size_t)k * n + t];
+        return v;
+    };
+    auto swsum = [&](int plane0, int k) {
+        double v = 0.0;
+        for (int b = 0; b < 14; ++b)
+            if ((ba.sw_mask >> b) & 1) v = v + ba.base[(size_t)(32 + 2 * b + plane0) * ba.P + (size_t)k * n + t];
+        return v;
+    };
+    const double up0 = lwsum(0, 0), dn0 = lwsum(1, 0);
+    double fprev = up0 * flxfac - dn0 * flxfac;
+    out.flw_up[(size_t)nrl * nc + i] = up0 * flxfac;
+    out.flw_dn[(size_t)nrl * nc + i] = dn0 * flxfac;
     for (int k = 1; k <= nrl; ++k) {
-        const double fu = acc.lw_up[(size_t)k * n + t] * flxfac, fd = acc.lw_dn[(size_t)k * n + t] * flxfac;
+        const double fu = lwsum(0, k) * flxfac, fd = lwsum(1, k) * flxfac;
         out.flw_up[(size_t)(nrl - k) * nc + i] = fu;
         out.flw_dn[(size_t)(nrl - k) * nc + i] = fd;
         const double fnet = fu - fd;
@@ -568,11 +603,11 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
         fprev = fnet;
     }
     if (cs.day[t]) {
-        double fu0 = acc.sw_up[t], fd0 = acc.sw_dn[t];
+        double fu0 = swsum(0, 0), fd0 = swsum(1, 0);
         out.fsw_up[(size_t)nrl * nc + i] = fu0;
         out.fsw_dn[(size_t)nrl * nc + i] = fd0;
         for (int k = 0; k < nrl; ++k) {
-            const double fu1 = acc.sw_up[(size_t)(k + 1) * n + t], fd1 = acc.sw_dn[(size_t)(k + 1) * n + t];
+            const double fu1 = swsum(0, k + 1), fd1 = swsum(1, k + 1);
             out.fsw_up[(size_t)(nrl - 1 - k) * nc + i] = fu1;
             out.fsw_dn[(size_t)(nrl - 1 - k) * nc + i] = fd1;
             const int kin = nrl - 1 - k;
@@ -581,7 +616,12 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
             if (k < nm) out.hr_sw[(size_t)(nm - 1 - k) * nc + i] = h;
             fu0 = fu1, fd0 = fd1;
         }
-        if (out.fsw_dn_sfc_clr) out.fsw_dn_sfc_clr[i] = acc.sw_clr0[t];
+        if (out.fsw_dn_sfc_clr) {
+            double v = 0.0;
+            for (int b = 0; b < 14; ++b)
+                if ((ba.sw_mask >> b) & 1) v = v + ba.base[(size_t)60 * ba.P + (size_t)b * n + t];
+            out.fsw_dn_sfc_clr[i] = v;
+        }
     } else {
         for (int L = 0; L <= nrl; ++L) out.fsw_up[(size_t)L * nc + i] = 0.0, out.fsw_dn[(size_t)L * nc + i] = 0.0;
         for (int k0 = 0; k0 < nm; ++k0) out.hr_sw[(size_t)k0 * nc + i] = 0.0;

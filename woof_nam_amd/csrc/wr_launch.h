@@ -1,128 +1,188 @@
 // wr_launch.h -- the tile launch sequence, shared by the FP64 and the mixed-precision units.
+//
+// A tile runs the column kernels on the caller's stream, then the 30 band kernels (16 LW, 14 SW),
+// then the FP64 post.  Every band kernel writes its own flux planes (BandAcc), so the band kernels
+// are independent of one another: outside graph capture they are spread over the context's
+// auxiliary streams (fork and join through events), which fills the CUs that a finishing band
+// kernel leaves idle with the next band's waves.  k_post adds the planes in band order, so the
+// result does not depend on which stream ran which band.
 #pragma once
 #include "wr_internal.h"
 #include "wr_nam.h"
 
 namespace wr {
 
-// band: -1 = all bands; 0..15 = LW band 1..16 only; 16..29 = SW band 16..29 only (diagnostics)
-template <typename R, int B, bool SURR = false>
-void lw_bands(wr_ctx *c, const Tabs<R> *tabs, const ColIn &in, const RunPar &par, int col0, int n, ColScal cs, Acc acc,
-              int band, hipStream_t s, const Surr &sg = Surr{})
+// Band kernel id: 0..15 = LW band 1..16, 16..29 = SW band 16..29.
+// Static longest-processing-time plan of the 30 band kernels over WR_NAUX streams (measured
+// mixed-path kernel times on a 262,144-column tile; -1 ends a row).
+constexpr int BAND_PLAN[WR_NAUX][9] = {
+    {2, 17, 11, 20, 19, 27, 15, 22, -1},
+    {4, 0, 7, 23, 24, 16, 25, -1, -1},
+    {8, 1, 10, 5, 9, 18, 14, 13, -1},
+    {6, 3, 29, 21, 12, 26, 28, -1, -1},
+};
+
+constexpr bool band_plan_complete()
 {
-    if constexpr (B < 16) {
-        if (band < 0 || band == B) {
-            tbeg(c, KID_LW0 + B, s);
-            hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, cs, acc,
-                               sg);
-            tend(c, s);
+    int seen[30] = {};
+    for (int q = 0; q < WR_NAUX; ++q)
+        for (int r = 0; r < 9 && BAND_PLAN[q][r] >= 0; ++r) {
+            const int id = BAND_PLAN[q][r];
+            if (id >= 30 || seen[id]++) return false;
         }
-        lw_bands<R, B + 1, SURR>(c, tabs, in, par, col0, n, cs, acc, band, s, sg);
+    for (int id = 0; id < 30; ++id)
+        if (seen[id] != 1) return false;
+    return true;
+}
+static_assert(band_plan_complete(), "every band kernel appears exactly once in BAND_PLAN");
+
+inline BandAcc tile_planes(wr_ctx *c, int n, int nrl, int band)
+{
+    BandAcc ba;
+    ba.base = c->acc;
+    ba.P = (long long)(nrl + 1) * n;
+    ba.n = n;
+    ba.lw_mask = band < 0 ? 0xffffu : (band < 16 ? 1u << band : 0u);
+    ba.sw_mask = band < 0 ? 0x3fffu : (band >= 16 ? 1u << (band - 16) : 0u);
+    return ba;
+}
+
+// Everything a band launch needs besides the band id.
+template <typename R>
+struct BandJob {
+    wr_ctx *c;
+    const Tabs<R> *tabs;
+    const R *sflux;
+    ColIn in;
+    RunPar par;
+    int col0, n;
+    ColScal cs;
+    BandAcc ba;
+    Surr sg;
+    Rtr<R> rt;
+};
+
+// McICA route band kernels (k_lw / k_sw), dispatched from a runtime id to the template instance.
+template <typename R, bool SURR, int B = 0>
+void launch_band(const BandJob<R> &j, int id, hipStream_t s)
+{
+    if constexpr (B < 30) {
+        if (id != B) return launch_band<R, SURR, B + 1>(j, id, s);
+        const Acc acc = band_acc(j.ba, B);
+        tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
+        if constexpr (B < 16)
+            hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
+                               j.n, j.cs, acc, j.sg);
+        else
+            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par,
+                               j.col0, j.n, j.cs, j.sflux, acc, j.sg);
+        tend(j.c, s);
     }
 }
 
-template <typename R, int B, bool SURR = false>
-void sw_bands(wr_ctx *c, const Tabs<R> *tabs, const R *sflux, const ColIn &in, const RunPar &par, int col0, int n,
-              ColScal cs, Acc acc, int band, hipStream_t s, const Surr &sg = Surr{})
+// NAM route band kernels (k_lwr / k_swr).
+template <typename R, int B = 0>
+void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s)
 {
-    if constexpr (B < 14) {
-        if (band < 0 || band == 16 + B) {
-            tbeg(c, KID_SW0 + B, s);
-            hipLaunchKernelGGL((k_sw<R, B, SURR>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, cs,
-                               sflux, acc, sg);
-            tend(c, s);
-        }
-        sw_bands<R, B + 1, SURR>(c, tabs, sflux, in, par, col0, n, cs, acc, band, s, sg);
+    if constexpr (B < 30) {
+        if (id != B) return launch_band_nam<R, B + 1>(j, id, s);
+        const Acc acc = band_acc(j.ba, B);
+        tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
+        if constexpr (B < 16)
+            hipLaunchKernelGGL((k_lwr<R, B>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
+                               j.cs, acc, j.rt);
+        else
+            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
+                               j.n, j.cs, j.sflux, acc);
+        tend(j.c, s);
     }
 }
 
-// One tile of n columns from col0: accumulators cleared, column scalars, day list, 16 LW and 14 SW
-// band kernels, FP64 post.  Everything on stream s, no host synchronisation.
+// True when the band kernels may leave stream s: auxiliary streams exist and s is not being
+// captured into a graph (a captured tile keeps one stream; graph replays overlap across slots).
+inline bool spread_bands(wr_ctx *c, hipStream_t s)
+{
+    if (c->naux <= 0) return false;
+    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    return st == hipStreamCaptureStatusNone;
+}
+
+// Launches the band kernels of a tile: one band (band >= 0) or all of them, in band order on s, or
+// by BAND_PLAN over the auxiliary streams with fork and join events around them.
+template <typename F>
+int run_bands(wr_ctx *c, int band, hipStream_t s, F &&one)
+{
+    if (band >= 0) {
+        one(band, s);
+        return WR_OK;
+    }
+    if (!spread_bands(c, s)) {
+        for (int id = 0; id < 30; ++id) one(id, s);
+        return WR_OK;
+    }
+    if (hipEventRecord(c->fork, s) != hipSuccess) return WR_ERR_HIP;
+    for (int q = 0; q < c->naux; ++q)
+        if (hipStreamWaitEvent(c->aux[q], c->fork, 0) != hipSuccess) return WR_ERR_HIP;
+    for (int q = 0; q < WR_NAUX; ++q) {
+        hipStream_t a = c->aux[q % c->naux];
+        for (int r = 0; r < 9 && BAND_PLAN[q][r] >= 0; ++r) one(BAND_PLAN[q][r], a);
+    }
+    for (int q = 0; q < c->naux; ++q) {
+        if (hipEventRecord(c->join[q], c->aux[q]) != hipSuccess) return WR_ERR_HIP;
+        if (hipStreamWaitEvent(s, c->join[q], 0) != hipSuccess) return WR_ERR_HIP;
+    }
+    return WR_OK;
+}
+
+// One tile of n columns from col0 (McICA route): column scalars, day list, band kernels, FP64 post.
+// No host synchronisation.
 template <typename R, bool SURR = false>
 int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const RunPar &par, const OutPtrs &out,
                 int col0, int n, int band, hipStream_t s, const Surr &sg = Surr{})
 {
     const int nrl = in.nlay + 1;
-    ColScal cs = colscal(c);
-    Acc acc = tile_acc(c, n, nrl);
-    const size_t nacc = (size_t)(4 * (nrl + 1) + 1) * n;
-    tbeg(c, KID_ZERO, s);
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, c->acc, nacc);
-    tend(c, s);
+    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), sg, Rtr<R>{}};
     tbeg(c, KID_COLPREP, s);
-    hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, cs, sflux);
+    hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
     tbeg(c, KID_DAYLIST, s);
-    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)cs.day, n, cs.dlist, cs.nday);
+    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)j.cs.day, n, j.cs.dlist, j.cs.nday);
     tend(c, s);
-    lw_bands<R, 0, SURR>(c, tabs, in, par, col0, n, cs, acc, band, s, sg);
-    sw_bands<R, 0, SURR>(c, tabs, (const R *)sflux, in, par, col0, n, cs, acc, band, s, sg);
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q) { launch_band<R, SURR>(j, id, q); });
+    if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
-    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, cs, acc, out);
+    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
     tend(c, s);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }
 
 // The NAM route (mcica = 0): overlap factors once per tile, then rtrnmr and cldfrc-mixed spcvrtc
 // band kernels.
-template <typename R, int B>
-void lwr_bands(wr_ctx *c, const Tabs<R> *tabs, const ColIn &in, const RunPar &par, int col0, int n, ColScal cs, Acc acc,
-               Rtr<R> rt, int band, hipStream_t s)
-{
-    if constexpr (B < 16) {
-        if (band < 0 || band == B) {
-            tbeg(c, KID_LW0 + B, s);
-            hipLaunchKernelGGL((k_lwr<R, B>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, cs, acc, rt);
-            tend(c, s);
-        }
-        lwr_bands<R, B + 1>(c, tabs, in, par, col0, n, cs, acc, rt, band, s);
-    }
-}
-
-template <typename R, int B>
-void swr_bands(wr_ctx *c, const Tabs<R> *tabs, const R *sflux, const ColIn &in, const RunPar &par, int col0, int n,
-               ColScal cs, Acc acc, int band, hipStream_t s)
-{
-    if constexpr (B < 14) {
-        if (band < 0 || band == 16 + B) {
-            tbeg(c, KID_SW0 + B, s);
-            hipLaunchKernelGGL((k_swr<R, B>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, cs, sflux, acc,
-                               c->swc);
-            tend(c, s);
-        }
-        swr_bands<R, B + 1>(c, tabs, sflux, in, par, col0, n, cs, acc, band, s);
-    }
-}
-
 template <typename R>
 int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const RunPar &par, const OutPtrs &out,
                     int col0, int n, int band, hipStream_t s)
 {
     const int nrl = in.nlay + 1;
-    ColScal cs = colscal(c);
-    Acc acc = tile_acc(c, n, nrl);
-    const size_t nacc = (size_t)(4 * (nrl + 1) + 1) * n;
     Rtr<R> rt;
     rt.f = (R *)c->rtr;
     rt.lstu = (uint64_t *)((char *)c->rtr + sizeof(double) * 12 * (NL + 3) * (size_t)c->max_cols);
     rt.lstd = rt.lstu + c->max_cols;
     rt.n = n;
-    tbeg(c, KID_ZERO, s);
-    hipLaunchKernelGGL(k_zero, dim3(1024), dim3(256), 0, s, c->acc, nacc);
-    tend(c, s);
+    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), Surr{}, rt};
     tbeg(c, KID_COLPREP, s);
-    hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, cs, sflux);
+    hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
     tbeg(c, KID_DAYLIST, s);
-    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)cs.day, n, cs.dlist, cs.nday);
+    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)j.cs.day, n, j.cs.dlist, j.cs.nday);
     tend(c, s);
     tbeg(c, KID_RTRFAC, s);
     hipLaunchKernelGGL((k_rtr_fac<R>), dim3((n + 63) / 64), dim3(64), 0, s, in, col0, n, rt);
     tend(c, s);
-    lwr_bands<R, 0>(c, tabs, in, par, col0, n, cs, acc, rt, band, s);
-    swr_bands<R, 0>(c, tabs, (const R *)sflux, in, par, col0, n, cs, acc, band, s);
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q) { launch_band_nam<R>(j, id, q); });
+    if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
-    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, cs, acc, out);
+    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
     tend(c, s);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }

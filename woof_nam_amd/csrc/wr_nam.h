@@ -185,6 +185,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
     const uint64_t lstu = rt.lstu[t], lstd = rt.lstd[t];
     const R eps = (R)1.0e-6;
 
+    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
     // ---- down sweep, top to surface ----
     R radtotd[NG], totradd[NG], clrradd[NG], rad[NG];
 #pragma unroll
@@ -239,7 +240,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
                 s = s + radtotd[g];
             }
         }
-        acc.lw_dn[(size_t)(kk - 1) * n + t] += (double)s;
+        acc.lw_dn[(size_t)(kk - 1) * n + t] = (double)s;
         tz_top = tz_bot;
         la_c = la_n;
     }
@@ -274,7 +275,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
                 rad[g] = (R)0;
                 s0 = s0 + radtotu[g];
             }
-            acc.lw_up[t] += (double)s0;
+            acc.lw_up[t] = (double)s0;
         }
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         const R clfr = (R)lw_cldfrc(rc, kk);
@@ -309,7 +310,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
                 s = s + radtotu[g];
             }
         }
-        acc.lw_up[(size_t)kk * n + t] += (double)s;
+        acc.lw_up[(size_t)kk * n + t] = (double)s;
         tz_bot = tz_top;
         la_c = la_m;
     }
@@ -351,7 +352,7 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
 
 template <typename R, int B>
 __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, const R *__restrict__ sflux, Acc acc, double *__restrict__ clr)
+                                            ColScal cs, const R *__restrict__ sflux, Acc acc)
 {
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
@@ -372,7 +373,6 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
     R zsolar[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
-    double *clr_up = clr, *clr_dn = clr + (size_t)(NL + 1) * n;
 
     R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
     for (int pass = 0; pass < (total ? 2 : 1); ++pass) {
@@ -389,7 +389,8 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
                 zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
             }
         }
-        // top-down and level fluxes (layer R/T recomputed); pass 0 keeps the clear band sums
+        // top-down and level fluxes (layer R/T recomputed); pass 0 leaves the clear-sky fluxes in the
+        // band planes, which pass 1 reads back and mixes with the cloudy ones
         R tdbt[NG], tdn[NG], rdnd[NG];
         R su = (R)0, sd = (R)0;
 #pragma unroll
@@ -427,15 +428,13 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
                 }
             }
             const size_t o = (size_t)s * n + t;  // level s (surface-first)
-            if (pass == 0 && s == 0) acc.sw_clr0[t] += (double)sd;
-            if (pass == 0 && total) {
-                clr_up[o] = (double)su, clr_dn[o] = (double)sd;
-            } else if (pass == 0) {
-                acc.sw_up[o] += (double)su;
-                acc.sw_dn[o] += (double)sd;
+            if (pass == 0 && s == 0) acc.sw_clr0[t] = (double)sd;
+            if (pass == 0) {
+                acc.sw_up[o] = (double)su;
+                acc.sw_dn[o] = (double)sd;
             } else {
-                acc.sw_up[o] += cf1 * (double)su + cf0 * clr_up[o];
-                acc.sw_dn[o] += cf1 * (double)sd + cf0 * clr_dn[o];
+                acc.sw_up[o] = cf1 * (double)su + cf0 * acc.sw_up[o];
+                acc.sw_dn[o] = cf1 * (double)sd + cf0 * acc.sw_dn[o];
             }
         }
     }
