@@ -535,6 +535,18 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
             rc = WR_ERR_NOMEM;
     }
     if (rc == WR_OK) {
+        // SW level-cache pool: room for every 64-column block of a tile in the largest layout
+        // (2 values x 16 g x 65 levels, FP64), at most WR_POOL_GB (default 6) GB per stream
+        const size_t lane_max = cache_slot_bytes(SW_NV, 16, c->nl_max + 1, (int)sizeof(double));
+        const size_t need = (m + 63) / 64 * lane_max;
+        const char *pe = getenv("WR_POOL_GB");
+        const double gb = pe ? atof(pe) : 6.0;
+        size_t cap = (size_t)(gb * 1e9);
+        if (cap < lane_max) cap = lane_max;
+        c->pool_bytes = need < cap ? need : cap;
+        if (hipMalloc((void **)&c->pool[0], c->pool_bytes) != hipSuccess) rc = WR_ERR_NOMEM;
+    }
+    if (rc == WR_OK) {
         // WR_BAND_STREAMS=0 keeps the band kernels on the caller's stream
         const char *e = getenv("WR_BAND_STREAMS");
         int na = e ? atoi(e) : WR_NAUX;
@@ -726,6 +738,7 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->sh2);
     (void)hipFree(c->sflag);
     for (int q = 0; q < WR_NAUX; ++q) {
+        (void)hipFree(c->pool[q]);
         if (c->aux[q]) (void)hipStreamDestroy(c->aux[q]);
         if (c->join[q]) (void)hipEventDestroy(c->join[q]);
     }

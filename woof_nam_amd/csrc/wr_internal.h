@@ -29,6 +29,10 @@ struct wr_ctx {
     float *sx, *sy;                  // features [S][8], outputs [504][S]
     void *sh1, *sh2;                 // hidden activations, bfloat16 [S][128]
     int *sflag;                      // [max_cols] per-column table-optics flag
+    // per-stream cache pools of the band kernels (pool[0]: the caller's stream and aux stream 0;
+    // pool[q]: aux stream q, allocated on first use), pool_bytes each
+    char *pool[WR_NAUX];
+    size_t pool_bytes;
     // band-kernel streams (default priority, non-blocking) and their fork/join events
     int naux;
     hipStream_t aux[WR_NAUX];

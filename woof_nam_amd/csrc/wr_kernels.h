@@ -15,7 +15,9 @@
 #include "wr_lw.h"
 #include "wr_sw.h"
 
-// minimum waves per SIMD the LW and SW band kernels are compiled for (register budget 512 / waves)
+// minimum waves per SIMD the LW and SW band kernels are compiled for (register budget 512 / waves).
+// One: the band kernels then keep everything in registers (AGPRs included) and need no scratch
+// (private segment) that the runtime would have to back on every HIP queue running them.
 #ifndef WR_LW_WAVES
 #define WR_LW_WAVES 1
 #endif
@@ -260,6 +262,35 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCo
     L.delp = rc.plvl(kin + 1) - rc.plvl(kin);
 }
 
+// ---- per-wave level cache of the SW kernels ----------------------------------------------------------
+// The SW adding method keeps two reflectances per g-point and level from the bottom-up sweep for the
+// top-down sweep.  They live in a slot of the context's pool, not in per-lane private arrays (which
+// would need kilobytes of scratch per lane, backed by the runtime on every HIP queue): one slot per
+// resident 64-lane workgroup (the kernels loop over column blocks, so the slot count bounds the
+// memory), laid out [value][g][level][lane] so that every access is one coalesced 64-lane row.
+// Entries per value and g: nk = nrl + 1 (layers 0..nrl-1, or levels 0..nrl).
+template <typename R> struct LaneCache {
+    R *p;  // this lane's element 0
+    int nk;
+    __device__ __forceinline__ R &at(int v, int ng, int g, int k) const
+    {
+        return p[(((size_t)v * ng + g) * nk + k) * 64];
+    }
+};
+__host__ __device__ constexpr size_t cache_slot_bytes(int nv, int ng, int nk, int rbytes)
+{
+    return (size_t)64 * nv * ng * nk * rbytes;
+}
+template <typename R>
+__device__ __forceinline__ LaneCache<R> lane_cache(char *pool, int nv, int ng, int nk)
+{
+    LaneCache<R> c;
+    c.p = (R *)(pool + (size_t)blockIdx.x * cache_slot_bytes(nv, ng, nk, (int)sizeof(R))) + threadIdx.x;
+    c.nk = nk;
+    return c;
+}
+constexpr int SW_NV = 2;  // per level and g: zrupb, zrupd (reflectances of the medium below)
+
 template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, Acc acc, Surr sg)
@@ -470,15 +501,15 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
                          o.td[g], o.ld[g]);
 }
 
-template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg)
+// One day column of one SW band (spcvrtc, McICA): bottom-up adding pass (reflectances of each level
+// to the lane cache), then the top-down pass with the layer R/T recomputed and the level fluxes.
+template <typename R, int B, bool SURR>
+__device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
+                                          int n, const ColScal &cs, const R *__restrict__ sflux, const Acc &acc,
+                                          const Surr &sg, int t, const LaneCache<R> &cc)
 {
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
-    const int tt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tt >= *cs.nday) return;
-    const int t = cs.dlist[tt];
     const int i = col0 + t;
     const RadCol rc{&in, i, in.nlay + 1};
     const int nrl = rc.nrl, nlp1 = nrl + 1;
@@ -501,20 +532,26 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
             return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
         });
     }
-    // bottom-up: reflectance of the medium below each level (swflux sweep 1); the layer R/T are
-    // recomputed in the top-down sweep (keeping them per g-point and layer in private memory needs
-    // tens of KB of scratch per lane, which every HIP queue running the kernel must back)
-    R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
+    // bottom-up: reflectance of the medium below each level (swflux sweep 1); zrupb/zrupd of level
+    // L (1-based, surface L = 1) sit at cache entry L - 1, the latest level also in pb/pd
+    R pb[NG], pd[NG];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
+    for (int g = 0; g < NG; ++g) {
+        pb[g] = asfc, pd[g] = asfc;
+        cc.at(0, NG, g, 0) = asfc;
+        cc.at(1, NG, g, 0) = asfc;
+    }
     for (int k = 0; k < nrl; ++k) {
         SwLayerRT<R, B> o;
         sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
-            zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
-            zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
+            const R zden1 = (R)1 / ((R)1 - pd[g] * o.rd[g]);
+            const R nb = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * pd[g] + o.ld[g] * pb[g])) * zden1;
+            const R nd = o.rd[g] + o.td[g] * o.td[g] * pd[g] * zden1;
+            pb[g] = nb, pd[g] = nd;
+            cc.at(0, NG, g, k + 1) = nb;
+            cc.at(1, NG, g, k + 1) = nd;
         }
     }
     // top-down: direct beam, diffuse transmission and reflection from above, level fluxes
@@ -523,9 +560,9 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
         tdbt[g] = (R)1, tdn[g] = (R)1, rdnd[g] = (R)0;
-        const R zden1 = (R)1 / ((R)1 - rdnd[g] * zrupd[g][nlp1]);
-        const R zfu = (tdbt[g] * zrupb[g][nlp1] + (tdn[g] - tdbt[g]) * zrupd[g][nlp1]) * zden1;
-        const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * zrupb[g][nlp1] * rdnd[g]) * zden1;
+        const R zden1 = (R)1 / ((R)1 - rdnd[g] * pd[g]);
+        const R zfu = (tdbt[g] * pb[g] + (tdn[g] - tdbt[g]) * pd[g]) * zden1;
+        const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * pb[g] * rdnd[g]) * zden1;
         su = su + zsolar[g] * zfu;
         sd = sd + zsolar[g] * zfd;
     }
@@ -549,14 +586,28 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
                 rdnd_s = o.rd[g] + o.td[g] * o.td[g] * rdnd[g] * zden1;
             }
             tdbt[g] = tdbt_s, tdn[g] = tdn_s, rdnd[g] = rdnd_s;
-            const R zden1 = (R)1 / ((R)1 - rdnd_s * zrupd[g][s]);
-            const R zfu = (tdbt_s * zrupb[g][s] + (tdn_s - tdbt_s) * zrupd[g][s]) * zden1;
-            const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zrupb[g][s] * rdnd_s) * zden1;
+            const R zb = cc.at(0, NG, g, s - 1), zd = cc.at(1, NG, g, s - 1);  // level s
+            const R zden1 = (R)1 / ((R)1 - rdnd_s * zd);
+            const R zfu = (tdbt_s * zb + (tdn_s - tdbt_s) * zd) * zden1;
+            const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zb * rdnd_s) * zden1;
             su = su + zsolar[g] * zfu;
             sd = sd + zsolar[g] * zfd;
         }
         acc.sw_up[(size_t)(s - 1) * n + t] = (double)su;
         acc.sw_dn[(size_t)(s - 1) * n + t] = (double)sd;
+    }
+}
+
+// SW band kernel: blocks of 64 day-list entries, a grid no larger than the pool's slot count.
+template <typename R, int B, bool SURR = false>
+__global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                           ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg, char *__restrict__ pool)
+{
+    const LaneCache<R> cc = lane_cache<R>(pool, SW_NV, SW_NG[B], in.nlay + 2);
+    const int nday = *cs.nday;
+    for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
+        const int tt = blk * 64 + threadIdx.x;
+        if (tt < nday) sw_column<R, B, SURR>(T, in, par, col0, n, cs, sflux, acc, sg, cs.dlist[tt], cc);
     }
 }
 

@@ -63,49 +63,67 @@ struct BandJob {
 };
 
 // McICA route band kernels (k_lw / k_sw), dispatched from a runtime id to the template instance.
+// Grid of a band kernel: one 64-lane block per 64 columns, at most one block per pool slot (the
+// kernels loop over the remaining blocks).
+template <typename R>
+inline unsigned band_grid(const BandJob<R> &j, int nv, int ng)
+{
+    const size_t slot = cache_slot_bytes(nv, ng, j.in.nlay + 2, (int)sizeof(R));
+    const size_t nslot = j.c->pool_bytes / slot;
+    const size_t nblk = (size_t)(j.n + 63) / 64;
+    return (unsigned)(nblk < nslot ? nblk : nslot);
+}
+
 template <typename R, bool SURR, int B = 0>
-void launch_band(const BandJob<R> &j, int id, hipStream_t s)
+void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
-        if (id != B) return launch_band<R, SURR, B + 1>(j, id, s);
+        if (id != B) return launch_band<R, SURR, B + 1>(j, id, s, pool);
         const Acc acc = band_acc(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16)
             hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
                                j.n, j.cs, acc, j.sg);
         else
-            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par,
-                               j.col0, j.n, j.cs, j.sflux, acc, j.sg);
+            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3(band_grid(j, SW_NV, SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+                               j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, j.sg, pool);
         tend(j.c, s);
     }
 }
 
 // NAM route band kernels (k_lwr / k_swr).
 template <typename R, int B = 0>
-void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s)
+void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
-        if (id != B) return launch_band_nam<R, B + 1>(j, id, s);
+        if (id != B) return launch_band_nam<R, B + 1>(j, id, s, pool);
         const Acc acc = band_acc(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16)
             hipLaunchKernelGGL((k_lwr<R, B>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
                                j.cs, acc, j.rt);
         else
-            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
-                               j.n, j.cs, j.sflux, acc);
+            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3(band_grid(j, SW_NV, SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+                               j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
         tend(j.c, s);
     }
 }
 
-// True when the band kernels may leave stream s: auxiliary streams exist and s is not being
-// captured into a graph (a captured tile keeps one stream; graph replays overlap across slots).
+// True when the band kernels may leave stream s: auxiliary streams exist, s is not being captured
+// into a graph (a captured tile keeps one stream; graph replays overlap across slots) and every
+// auxiliary stream has its cache pool (allocated here on first use, outside any capture).
 inline bool spread_bands(wr_ctx *c, hipStream_t s)
 {
     if (c->naux <= 0) return false;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
-    return st == hipStreamCaptureStatusNone;
+    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
+    for (int q = 1; q < c->naux; ++q)
+        if (!c->pool[q] && hipMalloc((void **)&c->pool[q], c->pool_bytes) != hipSuccess) {
+            c->pool[q] = nullptr;
+            (void)hipGetLastError();
+            return false;
+        }
+    return true;
 }
 
 // Launches the band kernels of a tile: one band (band >= 0) or all of them, in band order on s, or
@@ -114,19 +132,21 @@ template <typename F>
 int run_bands(wr_ctx *c, int band, hipStream_t s, F &&one)
 {
     if (band >= 0) {
-        one(band, s);
+        one(band, s, c->pool[0]);
         return WR_OK;
     }
     if (!spread_bands(c, s)) {
-        for (int id = 0; id < 30; ++id) one(id, s);
+        for (int id = 0; id < 30; ++id) one(id, s, c->pool[0]);
         return WR_OK;
     }
     if (hipEventRecord(c->fork, s) != hipSuccess) return WR_ERR_HIP;
     for (int q = 0; q < c->naux; ++q)
         if (hipStreamWaitEvent(c->aux[q], c->fork, 0) != hipSuccess) return WR_ERR_HIP;
+    // aux stream 0 shares pool 0 with the caller's stream: band kernels on s never overlap the
+    // auxiliary ones (fork and join events order them)
     for (int q = 0; q < WR_NAUX; ++q) {
-        hipStream_t a = c->aux[q % c->naux];
-        for (int r = 0; r < 9 && BAND_PLAN[q][r] >= 0; ++r) one(BAND_PLAN[q][r], a);
+        const int a = q % c->naux;
+        for (int r = 0; r < 9 && BAND_PLAN[q][r] >= 0; ++r) one(BAND_PLAN[q][r], c->aux[a], c->pool[a]);
     }
     for (int q = 0; q < c->naux; ++q) {
         if (hipEventRecord(c->join[q], c->aux[q]) != hipSuccess) return WR_ERR_HIP;
@@ -149,7 +169,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
     tbeg(c, KID_DAYLIST, s);
     hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)j.cs.day, n, j.cs.dlist, j.cs.nday);
     tend(c, s);
-    int rc = run_bands(c, band, s, [&](int id, hipStream_t q) { launch_band<R, SURR>(j, id, q); });
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
     hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
@@ -179,7 +199,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     tbeg(c, KID_RTRFAC, s);
     hipLaunchKernelGGL((k_rtr_fac<R>), dim3((n + 63) / 64), dim3(64), 0, s, in, col0, n, rt);
     tend(c, s);
-    int rc = run_bands(c, band, s, [&](int id, hipStream_t q) { launch_band_nam<R>(j, id, q); });
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band_nam<R>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
     hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);

@@ -350,15 +350,16 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
     }
 }
 
+// One day column of one SW band on the NAM route: clear-sky pass, then (cloudy columns) the
+// total-sky pass; each pass is the bottom-up adding sweep (level reflectances to the lane cache) and
+// the top-down sweep with the layer R/T recomputed.
 template <typename R, int B>
-__global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, const R *__restrict__ sflux, Acc acc)
+__device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
+                                           int n, const ColScal &cs, const R *__restrict__ sflux, const Acc &acc, int t,
+                                           const LaneCache<R> &cc)
 {
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
-    const int tt = blockIdx.x * blockDim.x + threadIdx.x;
-    if (tt >= *cs.nday) return;
-    const int t = cs.dlist[tt];
     const int i = col0 + t;
     const RadCol rc{&in, i, in.nlay + 1};
     const int nrl = rc.nrl, nlp1 = nrl + 1;
@@ -374,19 +375,27 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
 #pragma unroll
     for (int g = 0; g < NG; ++g) zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
 
-    R zrupb[NG][NL + 2], zrupd[NG][NL + 2];
     for (int pass = 0; pass < (total ? 2 : 1); ++pass) {
-        // bottom-up (swflux sweep 1)
+        // bottom-up (swflux sweep 1); zrupb/zrupd of level L (surface L = 1) at cache entry L - 1,
+        // the latest level also in pb/pd
+        R pb[NG], pd[NG];
 #pragma unroll
-        for (int g = 0; g < NG; ++g) zrupb[g][1] = asfc, zrupd[g][1] = asfc;
+        for (int g = 0; g < NG; ++g) {
+            pb[g] = asfc, pd[g] = asfc;
+            cc.at(0, NG, g, 0) = asfc;
+            cc.at(1, NG, g, 0) = asfc;
+        }
         for (int k = 0; k < nrl; ++k) {
             SwLayerRT<R, B> o;
             swr_layer<R, B>(T, rc, par, k, lt, pass, cf1, cz, snt, o);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
-                const R zden1 = (R)1 / ((R)1 - zrupd[g][k + 1] * o.rd[g]);
-                zrupb[g][k + 2] = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * zrupd[g][k + 1] + o.ld[g] * zrupb[g][k + 1])) * zden1;
-                zrupd[g][k + 2] = o.rd[g] + o.td[g] * o.td[g] * zrupd[g][k + 1] * zden1;
+                const R zden1 = (R)1 / ((R)1 - pd[g] * o.rd[g]);
+                const R nb = o.rb[g] + (o.td[g] * ((o.tb[g] - o.ld[g]) * pd[g] + o.ld[g] * pb[g])) * zden1;
+                const R nd = o.rd[g] + o.td[g] * o.td[g] * pd[g] * zden1;
+                pb[g] = nb, pd[g] = nd;
+                cc.at(0, NG, g, k + 1) = nb;
+                cc.at(1, NG, g, k + 1) = nd;
             }
         }
         // top-down and level fluxes (layer R/T recomputed); pass 0 leaves the clear-sky fluxes in the
@@ -396,9 +405,9 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             tdbt[g] = (R)1, tdn[g] = (R)1, rdnd[g] = (R)0;
-            const R zden1 = (R)1 / ((R)1 - rdnd[g] * zrupd[g][nlp1]);
-            const R zfu = (tdbt[g] * zrupb[g][nlp1] + (tdn[g] - tdbt[g]) * zrupd[g][nlp1]) * zden1;
-            const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * zrupb[g][nlp1] * rdnd[g]) * zden1;
+            const R zden1 = (R)1 / ((R)1 - rdnd[g] * pd[g]);
+            const R zfu = (tdbt[g] * pb[g] + (tdn[g] - tdbt[g]) * pd[g]) * zden1;
+            const R zfd = tdbt[g] + (tdn[g] - tdbt[g] + tdbt[g] * pb[g] * rdnd[g]) * zden1;
             su = su + zsolar[g] * zfu;
             sd = sd + zsolar[g] * zfd;
         }
@@ -420,9 +429,10 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
                         rdnd_s = o.rd[g] + o.td[g] * o.td[g] * rdnd[g] * zden1;
                     }
                     tdbt[g] = tdbt_s, tdn[g] = tdn_s, rdnd[g] = rdnd_s;
-                    const R zden1 = (R)1 / ((R)1 - rdnd_s * zrupd[g][s + 1]);
-                    const R zfu = (tdbt_s * zrupb[g][s + 1] + (tdn_s - tdbt_s) * zrupd[g][s + 1]) * zden1;
-                    const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zrupb[g][s + 1] * rdnd_s) * zden1;
+                    const R zb = cc.at(0, NG, g, s), zd = cc.at(1, NG, g, s);  // level s + 1
+                    const R zden1 = (R)1 / ((R)1 - rdnd_s * zd);
+                    const R zfu = (tdbt_s * zb + (tdn_s - tdbt_s) * zd) * zden1;
+                    const R zfd = tdbt_s + (tdn_s - tdbt_s + tdbt_s * zb * rdnd_s) * zden1;
                     su = su + zsolar[g] * zfu;
                     sd = sd + zsolar[g] * zfd;
                 }
@@ -437,6 +447,19 @@ __global__ void __launch_bounds__(64) k_swr(const Tabs<R> *__restrict__ T, ColIn
                 acc.sw_dn[o] = cf1 * (double)sd + cf0 * acc.sw_dn[o];
             }
         }
+    }
+}
+
+// SW band kernel of the NAM route: blocks of 64 day-list entries looped over a pool-sized grid.
+template <typename R, int B>
+__global__ void __launch_bounds__(64, WR_SW_WAVES) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                            ColScal cs, const R *__restrict__ sflux, Acc acc, char *__restrict__ pool)
+{
+    const LaneCache<R> cc = lane_cache<R>(pool, SW_NV, SW_NG[B], in.nlay + 2);
+    const int nday = *cs.nday;
+    for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
+        const int tt = blk * 64 + threadIdx.x;
+        if (tt < nday) swr_column<R, B>(T, in, par, col0, n, cs, sflux, acc, cs.dlist[tt], cc);
     }
 }
 
