@@ -34,6 +34,25 @@ IN_SW = (6 * NL + (NL + 1) + 3) * 8          # + albedo, cos z (day columns only
 OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 
 
+# Measured HBM bytes per column (LW) or per day column (SW) of the band kernels: rocprofv3 FETCH_SIZE
+# and WRITE_SIZE on a 160,000-column FP64 call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
+# device copy); the table and its commands are in profiles/hbm_traffic_fp64_grid400x400.md.
+TRAFFIC_FP64 = {
+    "k_lw_b01": 11696, "k_lw_b02": 11502, "k_lw_b03": 8067, "k_lw_b04": 9063, "k_lw_b05": 9232, "k_lw_b06": 9938,
+    "k_lw_b07": 8900, "k_lw_b08": 8564, "k_lw_b09": 7829, "k_lw_b10": 9130, "k_lw_b11": 9877, "k_lw_b12": 7601,
+    "k_lw_b13": 8091, "k_lw_b14": 7654, "k_lw_b15": 8459, "k_lw_b16": 7467,
+    "k_sw_b16": 20020, "k_sw_b17": 41438, "k_sw_b18": 24399, "k_sw_b19": 24383, "k_sw_b20": 28747,
+    "k_sw_b21": 28760, "k_sw_b22": 11269, "k_sw_b23": 28757, "k_sw_b24": 25337, "k_sw_b25": 21012,
+    "k_sw_b26": 20036, "k_sw_b27": 25326, "k_sw_b28": 20984, "k_sw_b29": 35641,
+}
+
+
+def kernel_traffic(name: str, precision: str, cols_per_launch: float, days_per_launch: float):
+    if precision != "fp64" or name not in TRAFFIC_FP64:
+        return None
+    return TRAFFIC_FP64[name] * (cols_per_launch if name.startswith("k_lw") else days_per_launch)
+
+
 def kernel_bytes(name: str, ncol: int, nday: int) -> float:
     if name.startswith("k_lw"):
         return IN_LW * ncol
@@ -159,6 +178,7 @@ def main(argv=None):
     k_ms = tot[dom] / cnt[dom]
     bytes_per_launch = kernel_bytes(dom, ncol, nday) * a.steps / cnt[dom]
     achieved = bytes_per_launch / (k_ms * 1e-3) / 1e9
+    traffic = kernel_traffic(dom, a.precision, ncol * a.steps / cnt[dom], nday * a.steps / cnt[dom])
     shares = sorted(((v / sum(tot.values()), k) for k, v in tot.items()), reverse=True)[:6]
     print("# kernel time shares: " + ", ".join(f"{k} {100 * f:.1f}%" for f, k in shares), file=sys.stderr, flush=True)
 
@@ -174,8 +194,10 @@ def main(argv=None):
         "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                        "per_call": "HIP events around each call on its stream"},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "launch_ms": k_ms, "algorithmic_bytes_per_launch": bytes_per_launch,
+                     "band_streams": "the 30 band kernels of a tile run on 4 streams at once: launch_ms includes "
+                                     "the time a kernel shares the CUs with the others",
                      "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
     }
 
