@@ -35,8 +35,8 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 
 
 # Measured HBM bytes per column (LW) or per day column (SW) of the band kernels: rocprofv3 FETCH_SIZE
-# and WRITE_SIZE on a 160,000-column FP64 call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
-# device copy); the table and its commands are in profiles/hbm_traffic_fp64_grid400x400.md.
+# and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
+# device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
     "k_lw_b01": 11696, "k_lw_b02": 11502, "k_lw_b03": 8067, "k_lw_b04": 9063, "k_lw_b05": 9232, "k_lw_b06": 9938,
     "k_lw_b07": 8900, "k_lw_b08": 8564, "k_lw_b09": 7829, "k_lw_b10": 9130, "k_lw_b11": 9877, "k_lw_b12": 7601,
@@ -47,10 +47,21 @@ TRAFFIC_FP64 = {
 }
 
 
+TRAFFIC_MIXED = {
+    "k_lw_b01": 11623, "k_lw_b02": 11439, "k_lw_b03": 20854, "k_lw_b04": 13159, "k_lw_b05": 18798,
+    "k_lw_b06": 9900, "k_lw_b07": 13138, "k_lw_b08": 10878, "k_lw_b09": 11874, "k_lw_b10": 9564, "k_lw_b11": 9872,
+    "k_lw_b12": 9915, "k_lw_b13": 9330, "k_lw_b14": 7647, "k_lw_b15": 8613, "k_lw_b16": 7648, "k_sw_b16": 14055,
+    "k_sw_b17": 21639, "k_sw_b18": 16425, "k_sw_b19": 16416, "k_sw_b20": 19103, "k_sw_b21": 19103,
+    "k_sw_b22": 9247, "k_sw_b23": 19081, "k_sw_b24": 17360, "k_sw_b25": 15011, "k_sw_b26": 14032,
+    "k_sw_b27": 17397, "k_sw_b28": 14982, "k_sw_b29": 21601,
+}
+
+
 def kernel_traffic(name: str, precision: str, cols_per_launch: float, days_per_launch: float):
-    if precision != "fp64" or name not in TRAFFIC_FP64:
+    table = TRAFFIC_FP64 if precision == "fp64" else TRAFFIC_MIXED
+    if name not in table:
         return None
-    return TRAFFIC_FP64[name] * (cols_per_launch if name.startswith("k_lw") else days_per_launch)
+    return table[name] * (cols_per_launch if name.startswith("k_lw") else days_per_launch)
 
 
 def kernel_bytes(name: str, ncol: int, nday: int) -> float:
