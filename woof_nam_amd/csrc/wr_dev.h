@@ -37,30 +37,43 @@ constexpr double HEATFAC = CON_G * 1.0e-2 / CON_CP;  // K/s per (W/m2 / hPa)
 constexpr double FLUXFAC = 3.141592653589793 * 2.0e4;
 constexpr double S0 = 1368.22;
 
+// Device code reads and writes HBM through pointers held in structs (tables, column fields, work
+// space).  Declared in the global address space in the device pass, so that their accesses compile
+// to global loads and stores: through a generic pointer they become flat accesses, which also
+// count against the LDS/scalar wait counter and so serialise the waits of every scalar load
+// behind the outstanding table gathers.  The host pass sees plain pointers of the same size.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define WR_G __attribute__((address_space(1)))
+#else
+#define WR_G
+#endif
+// a device allocation stored into a WR_G member (host code)
+template <typename T> inline WR_G T *gptr(T *p) { return (WR_G T *)p; }
+
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
 // ---- tables --------------------------------------------------------------------------------
 template <typename R> struct LwBand {
-    const R *absa, *absb, *selfref, *forref, *fracrefa, *fracrefb;
-    const R *ka_m[3], *kb_m[3];
-    const R *x1, *x2;
+    const WR_G R *absa, *absb, *selfref, *forref, *fracrefa, *fracrefb;
+    const WR_G R *ka_m[3], *kb_m[3];
+    const WR_G R *x1, *x2;
     R refrat_planck_a, refrat_planck_b, refrat_m_a, refrat_m_b, refrat_m_a3;
 };
 template <typename R> struct SwBand {
-    const R *absa, *absb, *selfref, *forref;
-    const R *rayl_v, *rayla, *raylb, *abso3a, *abso3b, *absch4, *absh2o, *absco2;
+    const WR_G R *absa, *absb, *selfref, *forref;
+    const WR_G R *rayl_v, *rayla, *raylb, *abso3a, *abso3b, *absch4, *absh2o, *absco2;
     R rayl, strrat;
 };
 template <typename R> struct Tabs {
     // index arithmetic stays FP64 in every precision
-    const double *preflog, *tref, *chi_mls, *sw_preflog, *sw_tref;
-    const R *totplnk, *delwave, *absliq1, *absice1;
-    const int32_t *ipat, *layreffr, *ix1, *ix2, *ibx;
-    const R *exp_tbl, *tfn_tbl, *tau_tbl, *sw_exp_tbl;
-    const double *sw_exp_tbl64;  // FP64 SW exp table (layers that take the FP64 R/T branch)
-    const R *sf01, *sf02, *sf03, *sw_strrat, *sw_specwt;
-    const R *extliq1, *ssaliq1, *asyliq1, *extice3, *ssaice3, *asyice3;
+    const WR_G double *preflog, *tref, *chi_mls, *sw_preflog, *sw_tref;
+    const WR_G R *totplnk, *delwave, *absliq1, *absice1;
+    const WR_G int32_t *ipat, *layreffr, *ix1, *ix2, *ibx;
+    const WR_G R *exp_tbl, *tfn_tbl, *tau_tbl, *sw_exp_tbl;
+    const WR_G double *sw_exp_tbl64;  // FP64 SW exp table (layers that take the FP64 R/T branch)
+    const WR_G R *sf01, *sf02, *sf03, *sw_strrat, *sw_specwt;
+    const WR_G R *extliq1, *ssaliq1, *asyliq1, *extice3, *ssaice3, *asyice3;
     R scalekur;
     LwBand<R> lw[16];
     SwBand<R> sw[14];
@@ -68,9 +81,9 @@ template <typename R> struct Tabs {
 
 // ---- column inputs ---------------------------------------------------------------------------
 struct ColIn {
-    const double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
-    const uint64_t *col_id;
-    const uint64_t *seed_dev;  // optional device copy of the McICA seed
+    const WR_G double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
+    const WR_G uint64_t *col_id;
+    const WR_G uint64_t *seed_dev;  // optional device copy of the McICA seed
     int ncol;  // stride of the [k][column] fields
     int nlay;  // model layers
 };
@@ -85,26 +98,26 @@ struct RunPar {
 // Radiation-grid accessors of one column (grrad_setup phys_rrtm_grrad_setup.cu:102,
 // grrad_gases .cu:56): layer kin top-first, kin = 0 is the layer added above the model top.
 struct RadCol {
-    const ColIn *in;
+    ColIn in;  // the call's field pointers, held by value (uniform, in scalar registers)
     int i;    // column index into the fields
     int nrl;  // radiation layers = model layers + 1
-    __device__ __forceinline__ double fld(const double *a, int k) const { return a[(size_t)k * in->ncol + i]; }
+    __device__ __forceinline__ double fld(const WR_G double *a, int k) const { return a[(size_t)k * in.ncol + i]; }
     __device__ __forceinline__ double plvl(int l) const
     {
         if (l == 0) return 1.0e-6;
-        const double v = 10.0 * (fld(in->p_lev, l - 1) * 1.0e-3);
+        const double v = 10.0 * (fld(in.p_lev, l - 1) * 1.0e-3);
         return (l == 1 && v <= 1.0e-6) ? 2.0e-6 : v;
     }
     __device__ __forceinline__ double plyr(int l) const
     {
         if (l == 0) return 0.5 * plvl(1);
-        const double prsl = 0.5 * (fld(in->p_lev, l - 1) + fld(in->p_lev, l));
+        const double prsl = 0.5 * (fld(in.p_lev, l - 1) + fld(in.p_lev, l));
         return 10.0 * (prsl * 1.0e-3);
     }
-    __device__ __forceinline__ double tlyr(int l) const { return fld(in->t_lay, l == 0 ? 0 : l - 1); }
-    __device__ __forceinline__ double qlyr(int l) const { return fmax(1.0e-7, fld(in->q_lay, l == 0 ? 0 : l - 1)); }
-    __device__ __forceinline__ double olyr(int l) const { return fmax(1.0e-10, fld(in->o3_lay, l == 0 ? 0 : l - 1)); }
-    __device__ __forceinline__ double tskn() const { return in->t_sfc[i]; }
+    __device__ __forceinline__ double tlyr(int l) const { return fld(in.t_lay, l == 0 ? 0 : l - 1); }
+    __device__ __forceinline__ double qlyr(int l) const { return fmax(1.0e-7, fld(in.q_lay, l == 0 ? 0 : l - 1)); }
+    __device__ __forceinline__ double olyr(int l) const { return fmax(1.0e-10, fld(in.o3_lay, l == 0 ? 0 : l - 1)); }
+    __device__ __forceinline__ double tskn() const { return in.t_sfc[i]; }
     // level temperature by log-pressure interpolation; la(l) = log(plyr(l)), lb(l) = log(plvl(l))
     __device__ __forceinline__ double tlvl(int l, double la_l, double la_lm1, double lb_l) const
     {
@@ -117,7 +130,7 @@ struct RadCol {
     __device__ __forceinline__ double cfrac(int l) const
     {
         if (l == 0) return 0.0;
-        const double ct = fld(in->cldf, l - 1);
+        const double ct = fld(in.cldf, l - 1);
         return (ct < 0.001) ? 0.0 : ct;
     }
     // progcld2 with the cloud fraction given (phys_rrtm_progcld2.cu:92-255): (frac, cwp, cip)
@@ -126,8 +139,8 @@ struct RadCol {
         ct = 0.0, cwp = 0.0, cip = 0.0;
         if (l == 0) return;
         const int k0 = l - 1;
-        ct = fld(in->cldf, k0);
-        const double qc = fld(in->qc, k0), qi = fld(in->qi, k0);
+        ct = fld(in.cldf, k0);
+        const double qc = fld(in.qc, k0), qi = fld(in.qi, k0);
         double qcwat, qcice;
         if (tlyr(l) - 273.15 > -40.0) {
             qcwat = qc;

@@ -78,7 +78,7 @@ struct Blob {
         bytes.resize(off);
         return off;
     }
-    template <typename T> const T *conv(const TEntry &e)
+    template <typename T> const WR_G T *conv(const TEntry &e)
     {
         const size_t off = align();
         bytes.resize(off + e.count * sizeof(T));
@@ -90,13 +90,13 @@ struct Blob {
             const int32_t *s = (const int32_t *)e.data;
             for (uint64_t j = 0; j < e.count; ++j) dst[j] = (T)s[j];
         }
-        return (const T *)(base + off);
+        return (const WR_G T *)(base + off);
     }
-    template <typename T> T *reserve(size_t n)
+    template <typename T> WR_G T *reserve(size_t n)
     {
         const size_t off = align();
         bytes.resize(off + n * sizeof(T), 0);
-        return (T *)(base + off);
+        return (WR_G T *)(base + off);
     }
 };
 
@@ -108,7 +108,7 @@ struct Binder {
         auto it = f.e.find(nm);
         return it == f.e.end() ? nullptr : &it->second;
     }
-    template <typename T> const T *need(const std::string &nm)
+    template <typename T> const WR_G T *need(const std::string &nm)
     {
         const TEntry *e = find(nm);
         if (!e) {
@@ -117,7 +117,7 @@ struct Binder {
         }
         return b.conv<T>(*e);
     }
-    template <typename T> const T *opt(const std::string &nm)
+    template <typename T> const WR_G T *opt(const std::string &nm)
     {
         const TEntry *e = find(nm);
         return e ? b.conv<T>(*e) : nullptr;
@@ -210,7 +210,7 @@ template <typename R> int bind_tables(const TFile &f, Blob &b, Tabs<R> &T)
     };
     for (const Slot &s : slots) {
         LwBand<R> &B = T.lw[s.band - 1];
-        const R *p = d.need<R>(bname("lw", s.band, s.key, true));
+        const WR_G R *p = d.need<R>(bname("lw", s.band, s.key, true));
         if (s.kind == 0) B.ka_m[s.idx] = p;
         if (s.kind == 1) B.kb_m[s.idx] = p;
         if (s.kind == 2) B.x1 = p;
@@ -307,7 +307,7 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const size_t nc = (size_t)in.ncol;
     const int day = cs.day[t];
@@ -432,9 +432,10 @@ int check_args(const wr_ctx *c, const wr_columns *in, const wr_params *p, const 
 ColIn to_colin(const wr_columns *in)
 {
     ColIn ci;
-    ci.p_lev = in->p_lev, ci.t_lay = in->t_lay, ci.q_lay = in->q_lay, ci.o3_lay = in->o3_lay;
-    ci.cldf = in->cldf, ci.qc = in->qc, ci.qi = in->qi, ci.t_sfc = in->t_sfc, ci.emis = in->emis;
-    ci.alb = in->alb, ci.cosz = in->cosz, ci.col_id = in->col_id, ci.seed_dev = in->seed_dev;
+    ci.p_lev = gptr(in->p_lev), ci.t_lay = gptr(in->t_lay), ci.q_lay = gptr(in->q_lay), ci.o3_lay = gptr(in->o3_lay);
+    ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
+    ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
+    ci.seed_dev = gptr(in->seed_dev);
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     return ci;
 }
@@ -461,8 +462,8 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     const ColIn ci = to_colin(in);
     const RunPar par = to_par(p);
     OutPtrs op;
-    op.hr_lw = o->hr_lw, op.hr_sw = o->hr_sw, op.flw_up = o->flw_up, op.flw_dn = o->flw_dn;
-    op.fsw_up = o->fsw_up, op.fsw_dn = o->fsw_dn, op.fsw_dn_sfc_clr = o->fsw_dn_sfc_clr;
+    op.hr_lw = gptr(o->hr_lw), op.hr_sw = gptr(o->hr_sw), op.flw_up = gptr(o->flw_up), op.flw_dn = gptr(o->flw_dn);
+    op.fsw_up = gptr(o->fsw_up), op.fsw_dn = gptr(o->fsw_dn), op.fsw_dn_sfc_clr = gptr(o->fsw_dn_sfc_clr);
     // full tiles of max_cols columns and one remainder: the band kernels run one 64-column wave per
     // workgroup at low occupancy, so tiles that are multiples of 65,536 columns (one wave per SIMD on
     // 1,024 SIMDs) leave no partly filled round of waves except in the remainder tile

@@ -67,15 +67,15 @@ inline ColScal colscal(wr_ctx *c)
 {
     const size_t m = (size_t)c->max_cols;
     ColScal s;
-    s.lw_laytrop = c->iws + 0 * m;
-    s.lcf1 = c->iws + 1 * m;
-    s.day = c->iws + 2 * m;
-    s.sw_laytrop = c->iws + 3 * m;
-    s.dlist = c->iws + 4 * m;
-    s.nday = c->iws + 5 * m;
-    s.pwv = c->dws + 0 * m;
-    s.zcf1 = c->dws + 1 * m;
-    s.zcf0 = c->dws + 2 * m;
+    s.lw_laytrop = gptr(c->iws + 0 * m);
+    s.lcf1 = gptr(c->iws + 1 * m);
+    s.day = gptr(c->iws + 2 * m);
+    s.sw_laytrop = gptr(c->iws + 3 * m);
+    s.dlist = gptr(c->iws + 4 * m);
+    s.nday = gptr(c->iws + 5 * m);
+    s.pwv = gptr(c->dws + 0 * m);
+    s.zcf1 = gptr(c->dws + 1 * m);
+    s.zcf0 = gptr(c->dws + 2 * m);
     return s;
 }
 }  // namespace wr

@@ -29,13 +29,13 @@ namespace wr {
 
 // FP64 level fluxes of ONE band for one tile ([nrl+1][n] planes, written once per level; sw_clr0 [n])
 struct Acc {
-    double *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
+    WR_G double *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
 };
 // All band planes of a tile: LW band b at planes 2b, 2b+1; SW band b at planes 32+2b, 33+2b; the
 // clear-sky surface SW of SW band b at base + 60 P + b n.  k_post sums the bands in band order, which
 // reproduces the sequential accumulation of the reference bit for bit.
 struct BandAcc {
-    double *base;
+    WR_G double *base;
     long long P;  // (nrl + 1) * n
     int n;
     unsigned lw_mask, sw_mask;  // bands present (bit b)
@@ -55,8 +55,8 @@ __host__ __device__ inline Acc band_acc(const BandAcc &ba, int id)  // id 0..15 
     return a;
 }
 struct ColScal {
-    int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
-    double *pwv, *zcf1, *zcf0;
+    WR_G int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
+    WR_G double *pwv, *zcf1, *zcf0;
 };
 
 template <typename R>
@@ -67,7 +67,7 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const double tem1g = 100.0 * CON_G;
     const double tem2c = 1.0e-20 * 1.0e3 * CON_AVGD;
@@ -123,7 +123,7 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
             for (int j = 0; j < njb; ++j) sflux[(size_t)(ns + j) * n + t] = T->scalekur * T->sf01[(ibd - 1) * 16 + j];
         } else {
             int ks, mfs;
-            const R *tab;
+            const WR_G R *tab;
             if (jb == 17 || jb == 28) {
                 ks = nrl;
                 for (int k = swlt; k <= nrl - 1; ++k)
@@ -231,8 +231,8 @@ template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__re
 // (j < 504 outputs, surface-first layer k, tile column t), de-normalised as y * tstd[j] + tmean[j];
 // flag[t] != 0 sends column t back to the table optics (inputs outside the training envelope).
 struct Surr {
-    const float *Y, *tmean, *tstd;
-    const int *flag;
+    const WR_G float *Y, *tmean, *tstd;
+    const WR_G int *flag;
     long long S;
 };
 
@@ -241,12 +241,14 @@ __device__ __forceinline__ float surr_out(const Surr &sg, int j, size_t s)
     return sg.Y[(size_t)j * sg.S + s] * sg.tstd[j] + sg.tmean[j];
 }
 
-template <typename R, int B> __device__ __forceinline__ void surr_lw(const Surr &sg, size_t s, R *tau, R *fr)
+template <typename R, int B, int NGL = LW_NG[B]>
+__device__ __forceinline__ void surr_lw(const Surr &sg, size_t s, R *tau, R *fr, int g0 = 0)
 {
 #pragma unroll
-    for (int g = 0; g < LW_NG[B]; ++g) {
-        tau[g] = (R)__expf(surr_out(sg, LW_NS[B] + g, s));
-        fr[g] = (R)fminf(1.0f, fmaxf(0.0f, surr_out(sg, 140 + LW_NS[B] + g, s)));
+    for (int jj = 0; jj < NGL; ++jj) {
+        const int g = g0 + jj;
+        tau[jj] = (R)__expf(surr_out(sg, LW_NS[B] + g, s));
+        fr[jj] = (R)fminf(1.0f, fmaxf(0.0f, surr_out(sg, 140 + LW_NS[B] + g, s)));
     }
 }
 
@@ -270,7 +272,7 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCo
 // memory), laid out [value][g][level][lane] so that every access is one coalesced 64-lane row.
 // Entries per value and g: nk = nrl + 1 (layers 0..nrl-1, or levels 0..nrl).
 template <typename R> struct LaneCache {
-    R *p;  // this lane's element 0
+    WR_G R *p;  // this lane's element 0
     int nk;
     __device__ __forceinline__ R &at(int v, int ng, int g, int k) const
     {
@@ -285,22 +287,42 @@ template <typename R>
 __device__ __forceinline__ LaneCache<R> lane_cache(char *pool, int nv, int ng, int nk)
 {
     LaneCache<R> c;
-    c.p = (R *)(pool + (size_t)blockIdx.x * cache_slot_bytes(nv, ng, nk, (int)sizeof(R))) + threadIdx.x;
+    c.p = (WR_G R *)(pool + (size_t)blockIdx.x * cache_slot_bytes(nv, ng, nk, (int)sizeof(R))) + threadIdx.x;
     c.nk = nk;
     return c;
 }
 constexpr int SW_NV = 2;  // per level and g: zrupb, zrupd (reflectances of the medium below)
 
+// Lanes per column of the LW band kernels: two in the FP32 instantiations (each lane takes half of
+// the band's g-points, which halves the registers a lane holds and doubles the waves in flight; the
+// two partial g-sums meet through a lane swap), one in FP64 (sums over g in the reference's order).
+#ifndef WR_LW_GSPLIT
+#define WR_LW_GSPLIT 1
+#endif
+template <typename R> constexpr int lw_gsplit() { return sizeof(R) == 4 ? WR_LW_GSPLIT : 1; }
+
+template <int GS, typename R> __device__ __forceinline__ R gsum(R s)
+{
+    if constexpr (GS == 2) return s + __shfl_xor(s, 1);
+    return s;
+}
+
 template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, Acc acc, Surr sg)
 {
+    constexpr int GS = lw_gsplit<R>();
     constexpr int NG = LW_NG[B];
+    constexpr int NGL = NG / GS;
+    static_assert(NG % GS == 0, "g-points split evenly over the lanes of a column");
     constexpr int NS = LW_NS[B];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = gt / GS;
+    const int g0 = (gt % GS) * NGL;
+    const bool lead = (gt % GS) == 0;  // writes the column's level fluxes
+    if (t >= n) return;  // both lanes of a column leave together
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = cs.lcf1[t];
     const int table = SURR ? sg.flag[t] : 1;  // 0: learned gas optics for this column
@@ -310,15 +332,15 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
     const R semis = (R)((sfe > EPS && sfe <= 1.0) ? sfe : 1.0);
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
     const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
-    uint64_t mb[NG];
-    for (int g = 0; g < NG; ++g)
-        mb[g] = mcica_g(col, seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+    uint64_t mb[NGL];
+    for (int jj = 0; jj < NGL; ++jj)
+        mb[jj] = mcica_g(col, seed, 0, NS + g0 + jj, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
 
-    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
+    if (lead) acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
     // ---- down sweep, top to surface ----
-    R rad[NG];
+    R rad[NGL];
 #pragma unroll
-    for (int g = 0; g < NG; ++g) rad[g] = (R)0;
+    for (int jj = 0; jj < NGL; ++jj) rad[jj] = (R)0;
     double tz_top = rc.tlyr(0);
     double la_c = log_d(rc.plyr(0));
     for (int kin = 0; kin < nrl; ++kin) {
@@ -331,25 +353,25 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             tz_bot = rc.tskn();
         }
         LwLay<R> L;
-        R tau[NG], fr[NG];
+        R tau[NGL], fr[NGL];
         if (SURR && !table) {
             lw_layer_cloud<R>(rc, kin, L);
-            surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
+            surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, rc, par, kin, la_c, L);
-            lw_optics<R, B>(T, L, tau, fr);
+            lw_optics<R, B, NGL>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         R s = (R)0;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
-            const R plfrac = fr[g];
+        for (int jj = 0; jj < NGL; ++jj) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + (R)0);
+            const R plfrac = fr[jj];
             const R bbdgas = plfrac * (blay + dplnkd * q.gasfac);
             const R gassrcd = bbdgas * q.atrgas;
-            const R clfm = ((mb[g] >> k) & 1) ? (R)1 : (R)0;
+            const R clfm = ((mb[jj] >> k) & 1) ? (R)1 : (R)0;
             if (clfm >= (R)EPS) {
                 const R odcld = sdif * taucld;
                 const R efclrfr = (R)1 - ((R)1 - M<R>::exp(-odcld)) * clfm;
@@ -357,13 +379,14 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
                 lw_tot<R>(T, q.odepth + odcld, totfac, atrtot);
                 const R bbdtot = plfrac * (blay + dplnkd * totfac);
                 const R totsrcd = bbdtot * atrtot;
-                rad[g] = rad[g] * q.trng * efclrfr + gassrcd + clfm * (totsrcd - gassrcd);
+                rad[jj] = rad[jj] * q.trng * efclrfr + gassrcd + clfm * (totsrcd - gassrcd);
             } else {
-                rad[g] = rad[g] * q.trng + gassrcd;
+                rad[jj] = rad[jj] * q.trng + gassrcd;
             }
-            s = s + rad[g];
+            s = s + rad[jj];
         }
-        acc.lw_dn[(size_t)k * n + t] = (double)s;
+        s = gsum<GS>(s);
+        if (lead) acc.lw_dn[(size_t)k * n + t] = (double)s;
         tz_top = tz_bot;
         la_c = la_n;
     }
@@ -381,13 +404,13 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             tz_top = rc.tlyr(0);
         }
         LwLay<R> L;
-        R tau[NG], fr[NG];
+        R tau[NGL], fr[NGL];
         if (SURR && !table) {
             lw_layer_cloud<R>(rc, kin, L);
-            surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
+            surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, rc, par, kin, la_c, L);
-            lw_optics<R, B>(T, L, tau, fr);
+            lw_optics<R, B, NGL>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
@@ -395,22 +418,23 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             const R pk0 = planck<R>(T, B, rc.tskn());
             R s0 = (R)0;
 #pragma unroll
-            for (int g = 0; g < NG; ++g) {
-                const R rad0 = semis * fr[g] * pk0;
-                rad[g] = rad0 + reflct * rad[g];
-                s0 = s0 + rad[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const R rad0 = semis * fr[jj] * pk0;
+                rad[jj] = rad0 + reflct * rad[jj];
+                s0 = s0 + rad[jj];
             }
-            acc.lw_up[t] = (double)s0;
+            s0 = gsum<GS>(s0);
+            if (lead) acc.lw_up[t] = (double)s0;
         }
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         R s = (R)0;
 #pragma unroll
-        for (int g = 0; g < NG; ++g) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
-            const R plfrac = fr[g];
+        for (int jj = 0; jj < NGL; ++jj) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + (R)0);
+            const R plfrac = fr[jj];
             const R bbugas = plfrac * (blay + dplnku * q.gasfac);
             const R gassrcu = bbugas * q.atrgas;
-            const R clfm = ((mb[g] >> k) & 1) ? (R)1 : (R)0;
+            const R clfm = ((mb[jj] >> k) & 1) ? (R)1 : (R)0;
             if (clfm > (R)EPS) {
                 const R odcld = sdif * taucld;
                 const R efclrfr = (R)1 - ((R)1 - M<R>::exp(-odcld)) * clfm;
@@ -418,13 +442,14 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
                 lw_tot<R>(T, q.odepth + odcld, totfac, atrtot);
                 const R bbutot = plfrac * (blay + dplnku * totfac);
                 const R totsrcu = bbutot * atrtot;
-                rad[g] = rad[g] * q.trng * efclrfr + gassrcu + clfm * (totsrcu - gassrcu);
+                rad[jj] = rad[jj] * q.trng * efclrfr + gassrcu + clfm * (totsrcu - gassrcu);
             } else {
-                rad[g] = rad[g] * q.trng + gassrcu;
+                rad[jj] = rad[jj] * q.trng + gassrcu;
             }
-            s = s + rad[g];
+            s = s + rad[jj];
         }
-        acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
+        s = gsum<GS>(s);
+        if (lead) acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
         tz_bot = tz_top;
         la_c = la_m;
     }
@@ -511,7 +536,7 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl, nlp1 = nrl + 1;
     const double czd = in.cosz[i];
     const R cz = (R)czd, snt = (R)(1.0 / czd);
@@ -613,7 +638,7 @@ __global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restric
 
 // ---- post: FP64 fluxes -> heating rates, top-first outputs -------------------------------------------
 struct OutPtrs {
-    double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn, *fsw_dn_sfc_clr;
+    WR_G double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn, *fsw_dn_sfc_clr;
 };
 static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, BandAcc ba, OutPtrs out)
 {
@@ -621,7 +646,7 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl, nm = in.nlay;
     const size_t nc = (size_t)in.ncol;
     const double flxfac = 0.5 * FLUXFAC;

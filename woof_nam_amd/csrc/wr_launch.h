@@ -39,7 +39,7 @@ static_assert(band_plan_complete(), "every band kernel appears exactly once in B
 inline BandAcc tile_planes(wr_ctx *c, int n, int nrl, int band)
 {
     BandAcc ba;
-    ba.base = c->acc;
+    ba.base = gptr(c->acc);
     ba.P = (long long)(nrl + 1) * n;
     ba.n = n;
     ba.lw_mask = band < 0 ? 0xffffu : (band < 16 ? 1u << band : 0u);
@@ -82,8 +82,8 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
         const Acc acc = band_acc(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16)
-            hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
-                               j.n, j.cs, acc, j.sg);
+            hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
+                               j.par, j.col0, j.n, j.cs, acc, j.sg);
         else
             hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3(band_grid(j, SW_NV, SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, j.sg, pool);
@@ -185,8 +185,8 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
 {
     const int nrl = in.nlay + 1;
     Rtr<R> rt;
-    rt.f = (R *)c->rtr;
-    rt.lstu = (uint64_t *)((char *)c->rtr + sizeof(double) * 12 * (NL + 3) * (size_t)c->max_cols);
+    rt.f = gptr((R *)c->rtr);
+    rt.lstu = gptr((uint64_t *)((char *)c->rtr + sizeof(double) * 12 * (NL + 3) * (size_t)c->max_cols));
     rt.lstd = rt.lstu + c->max_cols;
     rt.n = n;
     BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), Surr{}, rt};

@@ -2,7 +2,8 @@
 //
 // lw_layer:  lwrad_prep (phys_rrtm_lw_lwrad.cu:98-306) + setcoef (phys_rrtm_lw_setcoef.cu:82-244)
 //            for one surface-first layer, from the raw column inputs.
-// lw_optics: taugb01..16 (phys_rrtm_lw_taugbNN.cu) for all g-points of band B in one layer.
+// lw_optics: taugb01..16 (phys_rrtm_lw_taugbNN.cu) for the g-points of band B in one layer (all, or
+//            a contiguous half when two lanes share a column).
 // The operations are written in the order of the reference kernels; the FP64 instantiation is
 // compiled without contraction, so it reproduces a sequential IEEE evaluation bit for bit.
 #pragma once
@@ -112,7 +113,7 @@ __device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const Ra
 }
 
 // reference ratio chi_mls(a, lev) / chi_mls(b, lev) (setcoef rfrate, FP64)
-__device__ __forceinline__ double chirat(const double *__restrict__ chi, int a, int b, int lev)
+__device__ __forceinline__ double chirat(const WR_G double *__restrict__ chi, int a, int b, int lev)
 {
     return chi[(lev - 1) * 7 + (a - 1)] / chi[(lev - 1) * 7 + (b - 1)];
 }
@@ -121,7 +122,7 @@ template <typename R> __device__ __forceinline__ R planck(const Tabs<R> *__restr
 {
     const int ind = imin(180, imax(1, (int)(temp - 159.0)));
     const R fr = (R)(temp - (double)((int)temp));
-    const R *tp = T->totplnk + b * 181;
+    const WR_G R *tp = T->totplnk + b * 181;
     return T->delwave[b] * (tp[ind - 1] + fr * (tp[ind] - tp[ind - 1]));
 }
 
@@ -201,20 +202,20 @@ template <typename R> __device__ __forceinline__ Eta<R> eta(R a, R ratio, R b, R
 #define T3(tab, m, j, nj, g, ng) (tab)[(((m)-1) * (nj) + ((j)-1)) * (ng) + (g)]
 
 template <typename R>
-__device__ __forceinline__ R maj6(const R *__restrict__ tab, int ng, int g, const Sel<R> &s, R fa, R fb)
+__device__ __forceinline__ R maj6(const WR_G R *__restrict__ tab, int ng, int g, const Sel<R> &s, R fa, R fb)
 {
     return (s.fk0 * fa) * T2(tab, s.id[0], g, ng) + (s.fk0 * fb) * T2(tab, s.id[1], g, ng) +
            (s.fk1 * fa) * T2(tab, s.id[2], g, ng) + (s.fk1 * fb) * T2(tab, s.id[3], g, ng) +
            (s.fk2 * fa) * T2(tab, s.id[4], g, ng) + (s.fk2 * fb) * T2(tab, s.id[5], g, ng);
 }
 template <typename R>
-__device__ __forceinline__ R maj4(const R *__restrict__ tab, int ng, int g, int i0, R fs, R fa, R fb)
+__device__ __forceinline__ R maj4(const WR_G R *__restrict__ tab, int ng, int g, int i0, R fs, R fa, R fb)
 {
     return (((R)1 - fs) * fa) * T2(tab, i0, g, ng) + (((R)1 - fs) * fb) * T2(tab, i0 + 5, g, ng) +
            (fs * fa) * T2(tab, i0 + 1, g, ng) + (fs * fb) * T2(tab, i0 + 6, g, ng);
 }
 template <typename R>
-__device__ __forceinline__ R eta_minor(const R *__restrict__ tab, int nj, int ng, int g, int indm, const Eta<R> &em, R mf)
+__device__ __forceinline__ R eta_minor(const WR_G R *__restrict__ tab, int nj, int ng, int g, int indm, const Eta<R> &em, R mf)
 {
     const R m1 = T3(tab, indm, em.js, nj, g, ng) + em.fs * (T3(tab, indm, em.js + 1, nj, g, ng) - T3(tab, indm, em.js, nj, g, ng));
     const R m2 = T3(tab, indm + 1, em.js, nj, g, ng) +
@@ -222,8 +223,9 @@ __device__ __forceinline__ R eta_minor(const R *__restrict__ tab, int nj, int ng
     return m1 + mf * (m2 - m1);
 }
 
-template <typename R, int B>
-__device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr)
+// g-points g0 .. g0 + NGL - 1 of band B (all of them by default); tau/fr hold them from index 0
+template <typename R, int B, int NGL = LW_NG[B]>
+__device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0 = 0)
 {
     constexpr int ng = LW_NG[B];
     const LwBand<R> &Bt = T->lw[B];
@@ -234,7 +236,7 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
     const int indm = L.indminor, indmp = indm + 1;
     const R sf = L.selffac, sfr = L.selffrac, ff = L.forfac, ffr = L.forfrac, mf = L.minorfrac;
     const R *ca = L.colamt;
-    const double *chi = T->chi_mls;
+    const WR_G double *chi = T->chi_mls;
 #define SELF(g) (sf * (T2(Bt.selfref, inds, g, ng) + sfr * (T2(Bt.selfref, indsp, g, ng) - T2(Bt.selfref, inds, g, ng))))
 #define FOR(g) (ff * (T2(Bt.forref, indf, g, ng) + ffr * (T2(Bt.forref, indfp, g, ng) - T2(Bt.forref, indf, g, ng))))
 #define MIN1(tab, g) (T2(tab, indm, g, ng) + mf * (T2(tab, indmp, g, ng) - T2(tab, indm, g, ng)))
@@ -251,33 +253,37 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
         if (lower) {
             const R corradj = (pp < (R)250) ? ((R)1 - (R)0.15 * ((R)250 - pp) / (R)154.4) : (R)1;
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R taun2 = scalen2 * MIN1(Bt.ka_m[0], g);
-                tau[g] = corradj * (ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + taun2);
-                fr[g] = Bt.fracrefa[g];
+                tau[jj] = corradj * (ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + taun2);
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
             const R corradj = (R)1 - (R)0.15 * (pp / (R)95.6);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R taun2 = scalen2 * MIN1(Bt.kb_m[0], g);
-                tau[g] = corradj * (ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g) + taun2);
-                fr[g] = Bt.fracrefb[g];
+                tau[jj] = corradj * (ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g) + taun2);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 1) {
         if (lower) {
             const R corradj = (R)1 - (R)0.05 * (L.pavel - (R)100) / (R)900;
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = corradj * (ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g));
-                fr[g] = Bt.fracrefa[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = corradj * (ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g));
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g);
-                fr[g] = Bt.fracrefb[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 2) {
@@ -290,24 +296,26 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             const Sel<R> s0 = sel_one<R>(e0.specparm, e0.fs, l0 * 9 + e0.js);
             const Sel<R> s1 = sel_one<R>(e1.specparm, e1.fs, l1 * 9 + e1.js);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R absn2o = eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, em, mf);
                 const R tmaj = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10);
                 const R tmaj1 = e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11);
-                tau[g] = tmaj + tmaj1 + SELF(g) + FOR(g) + adjcoln2o * absn2o;
-                fr[g] = FRACA(ep, g);
+                tau[jj] = tmaj + tmaj1 + SELF(g) + FOR(g) + adjcoln2o * absn2o;
+                fr[jj] = FRACA(ep, g);
             }
         } else {
             const Eta<R> e0 = eta<R>(ca[0], (R)chirat(chi, 1, 2, jp), ca[1], (R)4), e1 = eta<R>(ca[0], (R)chirat(chi, 1, 2, jpp), ca[1], (R)4);
             const Eta<R> em = eta<R>(ca[0], Bt.refrat_m_b, ca[1], (R)4), ep = eta<R>(ca[0], Bt.refrat_planck_b, ca[1], (R)4);
             const int i0 = u0 * 5 + e0.js, i1 = u1 * 5 + e1.js;
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R absn2o = eta_minor<R>(Bt.kb_m[0], 5, ng, g, indm, em, mf);
                 const R tmaj = e0.speccomb * maj4<R>(Bt.absb, ng, g, i0, e0.fs, f00, f10);
                 const R tmaj1 = e1.speccomb * maj4<R>(Bt.absb, ng, g, i1, e1.fs, f01, f11);
-                tau[g] = tmaj + tmaj1 + FOR(g) + adjcoln2o * absn2o;
-                fr[g] = FRACB(ep, g);
+                tau[jj] = tmaj + tmaj1 + FOR(g) + adjcoln2o * absn2o;
+                fr[jj] = FRACB(ep, g);
             }
         }
     } else if constexpr (B == 3) {
@@ -317,18 +325,20 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             const Sel<R> s0 = sel_one<R>(e0.specparm, e0.fs, l0 * 9 + e0.js);
             const Sel<R> s1 = sel_one<R>(e1.specparm, e1.fs, l1 * 9 + e1.js);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R tmaj = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10);
                 const R tmaj1 = e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11);
-                tau[g] = tmaj + tmaj1 + SELF(g) + FOR(g);
-                fr[g] = FRACA(ep, g);
+                tau[jj] = tmaj + tmaj1 + SELF(g) + FOR(g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
             const Eta<R> e0 = eta<R>(ca[2], (R)chirat(chi, 3, 2, jp), ca[1], (R)4), e1 = eta<R>(ca[2], (R)chirat(chi, 3, 2, jpp), ca[1], (R)4);
             const Eta<R> ep = eta<R>(ca[2], Bt.refrat_planck_b, ca[1], (R)4);
             const int i0 = u0 * 5 + e0.js, i1 = u1 * 5 + e1.js;
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 R tg = e0.speccomb * maj4<R>(Bt.absb, ng, g, i0, e0.fs, f00, f10) + e1.speccomb * maj4<R>(Bt.absb, ng, g, i1, e1.fs, f01, f11);
                 if (g == 7) tg = tg * (R)0.92;
                 if (g == 8) tg = tg * (R)0.88;
@@ -337,8 +347,8 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
                 if (g == 11) tg = tg * (R)0.99;
                 if (g == 12) tg = tg * (R)0.88;
                 if (g == 13) tg = tg * (R)0.943;
-                tau[g] = tg;
-                fr[g] = FRACB(ep, g);
+                tau[jj] = tg;
+                fr[jj] = FRACB(ep, g);
             }
         }
     } else if constexpr (B == 4) {
@@ -349,21 +359,23 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R abso3 = eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, em, mf);
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g) + abso3 * ca[2] + wxk * Bt.x1[g];
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
             const Eta<R> e0 = eta<R>(ca[2], (R)chirat(chi, 3, 2, jp), ca[1], (R)4), e1 = eta<R>(ca[2], (R)chirat(chi, 3, 2, jpp), ca[1], (R)4);
             const Eta<R> ep = eta<R>(ca[2], Bt.refrat_planck_b, ca[1], (R)4);
             const int i0 = u0 * 5 + e0.js, i1 = u1 * 5 + e1.js;
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = e0.speccomb * maj4<R>(Bt.absb, ng, g, i0, e0.fs, f00, f10) + e1.speccomb * maj4<R>(Bt.absb, ng, g, i1, e1.fs, f01, f11) +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = e0.speccomb * maj4<R>(Bt.absb, ng, g, i0, e0.fs, f00, f10) + e1.speccomb * maj4<R>(Bt.absb, ng, g, i1, e1.fs, f01, f11) +
                          wxk * Bt.x1[g];
-                fr[g] = FRACB(ep, g);
+                fr[jj] = FRACB(ep, g);
             }
         }
     } else if constexpr (B == 5) {
@@ -373,16 +385,18 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             const R ratco2 = ca[1] / temp;
             const R adjcolco2 = (ratco2 > (R)3) ? ((R)2 + M<R>::pow(ratco2 - (R)2, (R)0.77)) * temp : ca[1];
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + adjcolco2 * MIN1(Bt.ka_m[0], g) +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + adjcolco2 * MIN1(Bt.ka_m[0], g) +
                          wx2 * Bt.x1[g] + wx3 * Bt.x2[g];
-                fr[g] = Bt.fracrefa[g];
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = wx2 * Bt.x1[g] + wx3 * Bt.x2[g];
-                fr[g] = Bt.fracrefa[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = wx2 * Bt.x1[g] + wx3 * Bt.x2[g];
+                fr[jj] = Bt.fracrefa[g];
             }
         }
     } else if constexpr (B == 6) {
@@ -395,16 +409,18 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R absco2 = eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, em, mf);
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g) + adjcolco2 * absco2;
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
             const R adjcolco2 = (ratco2 > (R)3) ? ((R)2 + M<R>::pow(ratco2 - (R)2, (R)0.79)) * temp : ca[1];
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 R tg = ca[2] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + adjcolco2 * MIN1(Bt.kb_m[0], g);
                 if (g == 5) tg = tg * (R)0.92;
                 if (g == 6) tg = tg * (R)0.88;
@@ -412,8 +428,8 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
                 if (g == 8) tg = tg * (R)1.1;
                 if (g == 9) tg = tg * (R)0.99;
                 if (g == 10) tg = tg * (R)0.855;
-                tau[g] = tg;
-                fr[g] = Bt.fracrefb[g];
+                tau[jj] = tg;
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 7) {
@@ -423,20 +439,22 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
         const R w3 = L.wx[2], w4 = L.wx[3];
         if (lower) {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[0] * (f00 * Bt.absa[l0 * ng + g] + f10 * Bt.absa[(l0 + 1) * ng + g] + f01 * Bt.absa[l1 * ng + g] +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[0] * (f00 * Bt.absa[l0 * ng + g] + f10 * Bt.absa[(l0 + 1) * ng + g] + f01 * Bt.absa[l1 * ng + g] +
                                   f11 * Bt.absa[(l1 + 1) * ng + g]) +
                          SELF(g) + FOR(g) + adjcolco2 * MIN1(Bt.ka_m[0], g) + ca[2] * MIN1(Bt.ka_m[1], g) +
                          ca[3] * MIN1(Bt.ka_m[2], g) + w3 * Bt.x1[g] + w4 * Bt.x2[g];
-                fr[g] = Bt.fracrefa[g];
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[2] * (f00 * Bt.absb[u0 * ng + g] + f10 * Bt.absb[(u0 + 1) * ng + g] + f01 * Bt.absb[u1 * ng + g] +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[2] * (f00 * Bt.absb[u0 * ng + g] + f10 * Bt.absb[(u0 + 1) * ng + g] + f01 * Bt.absb[u1 * ng + g] +
                                   f11 * Bt.absb[(u1 + 1) * ng + g]) +
                          adjcolco2 * MIN1(Bt.kb_m[0], g) + ca[3] * MIN1(Bt.kb_m[2], g) + w3 * Bt.x1[g] + w4 * Bt.x2[g];
-                fr[g] = Bt.fracrefb[g];
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 8) {
@@ -449,48 +467,54 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R absn2o = eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, em, mf);
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g) + adjcoln2o * absn2o;
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[4] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + adjcoln2o * MIN1(Bt.kb_m[0], g);
-                fr[g] = Bt.fracrefb[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[4] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + adjcoln2o * MIN1(Bt.kb_m[0], g);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 9) {
         if (lower) {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g);
-                fr[g] = Bt.fracrefa[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g);
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g);
-                fr[g] = Bt.fracrefb[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 10) {
         const R scaleo2 = ca[5] * L.scaleminor;
         if (lower) {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R tauo2 = scaleo2 * MIN1(Bt.ka_m[0], g);
-                tau[g] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + tauo2;
-                fr[g] = Bt.fracrefa[g];
+                tau[jj] = ca[0] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g) + tauo2;
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R tauo2 = scaleo2 * MIN1(Bt.kb_m[0], g);
-                tau[g] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g) + tauo2;
-                fr[g] = Bt.fracrefb[g];
+                tau[jj] = ca[0] * ABS4(Bt.absb, u0 + 1, u1 + 1, g) + FOR(g) + tauo2;
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 11) {
@@ -500,14 +524,15 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g);
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) tau[g] = (R)0, fr[g] = (R)0;
+            for (int jj = 0; jj < NGL; ++jj) tau[jj] = (R)0, fr[jj] = (R)0;
         }
     } else if constexpr (B == 12) {
         if (lower) {
@@ -520,32 +545,36 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R absco2 = eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, emc2, mf);
                 const R absco = eta_minor<R>(Bt.ka_m[1], 9, ng, g, indm, emc, mf);
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g) + adjcolco2 * absco2 + ca[6] * absco;
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[2] * MIN1(Bt.kb_m[0], g);
-                fr[g] = Bt.fracrefb[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[2] * MIN1(Bt.kb_m[0], g);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 13) {
         if (lower) {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[1] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g);
-                fr[g] = Bt.fracrefa[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[1] * ABS4(Bt.absa, l0 + 1, l1 + 1, g) + SELF(g) + FOR(g);
+                fr[jj] = Bt.fracrefa[g];
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[1] * ABS4(Bt.absb, u0 + 1, u1 + 1, g);
-                fr[g] = Bt.fracrefb[g];
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[1] * ABS4(Bt.absb, u0 + 1, u1 + 1, g);
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     } else if constexpr (B == 14) {
@@ -556,15 +585,16 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
                 const R taun2 = scalen2 * eta_minor<R>(Bt.ka_m[0], 9, ng, g, indm, em, mf);
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g) + taun2;
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) tau[g] = (R)0, fr[g] = (R)0;
+            for (int jj = 0; jj < NGL; ++jj) tau[jj] = (R)0, fr[jj] = (R)0;
         }
     } else if constexpr (B == 15) {
         if (lower) {
@@ -573,17 +603,19 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
             Sel<R> s0, s1;
             sel_joint<R>(e0.specparm, e0.fs, l0 * 9 + e0.js, e1.specparm, e1.fs, l1 * 9 + e1.js, s0, s1);
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = e0.speccomb * maj6<R>(Bt.absa, ng, g, s0, f00, f10) + e1.speccomb * maj6<R>(Bt.absa, ng, g, s1, f01, f11) +
                          SELF(g) + FOR(g);
-                fr[g] = FRACA(ep, g);
+                fr[jj] = FRACA(ep, g);
             }
         } else {
 #pragma unroll
-            for (int g = 0; g < ng; ++g) {
-                tau[g] = ca[4] * (f00 * T2(Bt.absb, 1, g, ng) + f10 * T2(Bt.absb, 2, g, ng) + f01 * T2(Bt.absb, 1, g, ng) +
+            for (int jj = 0; jj < NGL; ++jj) {
+                const int g = g0 + jj;
+                tau[jj] = ca[4] * (f00 * T2(Bt.absb, 1, g, ng) + f10 * T2(Bt.absb, 2, g, ng) + f01 * T2(Bt.absb, 1, g, ng) +
                                   f11 * T2(Bt.absb, 2, g, ng));
-                fr[g] = Bt.fracrefb[g];
+                fr[jj] = Bt.fracrefb[g];
             }
         }
     }

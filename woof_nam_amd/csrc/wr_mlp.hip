@@ -221,7 +221,7 @@ __global__ void __launch_bounds__(64) k_optics(const Tabs<double> *__restrict__ 
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= in.ncol) return;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     int swlt = nrl;
     for (int k = 0; k < nrl; ++k)
@@ -347,9 +347,10 @@ int wr_layer_optics(wr_ctx *c, const wr_columns *in, const wr_params *p, float *
     if (!in->p_lev || !in->t_lay || !in->q_lay || !in->o3_lay || !in->cldf || !in->qc || !in->qi || !in->t_sfc)
         return WR_ERR_ARG;
     ColIn ci;
-    ci.p_lev = in->p_lev, ci.t_lay = in->t_lay, ci.q_lay = in->q_lay, ci.o3_lay = in->o3_lay;
-    ci.cldf = in->cldf, ci.qc = in->qc, ci.qi = in->qi, ci.t_sfc = in->t_sfc, ci.emis = in->emis;
-    ci.alb = in->alb, ci.cosz = in->cosz, ci.col_id = in->col_id, ci.seed_dev = in->seed_dev;
+    ci.p_lev = gptr(in->p_lev), ci.t_lay = gptr(in->t_lay), ci.q_lay = gptr(in->q_lay), ci.o3_lay = gptr(in->o3_lay);
+    ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
+    ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
+    ci.seed_dev = gptr(in->seed_dev);
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     RunPar par;
     par.mcica = p->mcica, par.iovr = p->iovr, par.seed = p->seed;

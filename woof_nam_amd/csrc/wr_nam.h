@@ -10,8 +10,8 @@ namespace wr {
 // q = 0 faccld1u, 1 faccld2u, 2 facclr1u, 3 facclr2u, 4 faccmb1u, 5 faccmb2u, 6..11 the same
 // downward; lstu/lstd bit (idx - 1) set for idx = 1..nl.
 template <typename R> struct Rtr {
-    R *f;
-    uint64_t *lstu, *lstd;
+    WR_G R *f;
+    WR_G uint64_t *lstu, *lstd;
     int n;
     __device__ __forceinline__ R &at(int q, int idx, int t) const { return f[((size_t)q * (NL + 3) + idx) * n + t]; }
 };
@@ -31,7 +31,7 @@ __global__ void __launch_bounds__(64) k_rtr_fac(ColIn in, int col0, int n, Rtr<R
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    const RadCol rc{&in, col0 + t, in.nlay + 1};
+    const RadCol rc{in, col0 + t, in.nlay + 1};
     const int nl = rc.nrl;
     const double eps = 1.0e-6;
     double cf[NL + 3];
@@ -175,7 +175,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = cs.lcf1[t];
     const double pwv = cs.pwv[t];
@@ -361,7 +361,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl, nlp1 = nrl + 1;
     const double czd = in.cosz[i];
     const R cz = (R)czd, snt = (R)(1.0 / czd);

@@ -18,7 +18,7 @@ __global__ void __launch_bounds__(64) k_features(ColIn in, int col0, int n, wr_s
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = col0 + t;
-    const RadCol rc{&in, i, in.nlay + 1};
+    const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     int out = 0;
     for (int k = 0; k < nrl; ++k) {
@@ -73,6 +73,6 @@ int wr_launch_surr(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs 
         if (hipGetLastError() != hipSuccess) return WR_ERR_HIP;
     }
     Surr sg;
-    sg.Y = c->sy, sg.tmean = w.tmean, sg.tstd = w.tstd, sg.flag = c->sflag, sg.S = S;
+    sg.Y = gptr(c->sy), sg.tmean = gptr(w.tmean), sg.tstd = gptr(w.tstd), sg.flag = gptr(c->sflag), sg.S = S;
     return launch_tile<float, true>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, -1, s, sg);
 }

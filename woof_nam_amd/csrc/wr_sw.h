@@ -109,14 +109,14 @@ __device__ __forceinline__ void ind8(int id0, int id1, int js, int nrow, int d1,
     id[6] = imin(nrow, id[4] + d1);
     id[7] = imin(nrow, id[4] + d2);
 }
-template <typename R> __device__ __forceinline__ R sum8(const R *__restrict__ A, int ng, int g, const int *id, const R *fc)
+template <typename R> __device__ __forceinline__ R sum8(const WR_G R *__restrict__ A, int ng, int g, const int *id, const R *fc)
 {
     return fc[0] * S2(A, id[0], g, ng) + fc[1] * S2(A, id[1], g, ng) + fc[2] * S2(A, id[2], g, ng) +
            fc[3] * S2(A, id[3], g, ng) + fc[4] * S2(A, id[4], g, ng) + fc[5] * S2(A, id[5], g, ng) +
            fc[6] * S2(A, id[6], g, ng) + fc[7] * S2(A, id[7], g, ng);
 }
 template <typename R>
-__device__ __forceinline__ R abs4(const R *__restrict__ A, int ng, int g, int id0, int id1, int nrow, R f00, R f01, R f10, R f11)
+__device__ __forceinline__ R abs4(const WR_G R *__restrict__ A, int ng, int g, int id0, int id1, int nrow, R f00, R f01, R f10, R f11)
 {
     const int i01 = imax(1, imin(nrow, id0 + 1)), i02 = imin(nrow, i01 + 1);
     const int i11 = imax(1, imin(nrow, id1 + 1)), i12 = imin(nrow, i11 + 1);
@@ -331,7 +331,7 @@ __device__ __forceinline__ void sw_cloud(const Tabs<R> *__restrict__ T, int B, d
         const double factor = refliq - 1.5;
         const int index = imax(1, imin(57, (int)factor));
         const R fint = (R)(factor - (double)index);
-        const R *e = T->extliq1 + B * 58, *s = T->ssaliq1 + B * 58, *a = T->asyliq1 + B * 58;
+        const WR_G R *e = T->extliq1 + B * 58, *s = T->ssaliq1 + B * 58, *a = T->asyliq1 + B * 58;
         const R extco = fmax((R)0, e[index - 1] + fint * (e[index] - e[index - 1]));
         const R ssaco = fmax((R)0, fmin((R)1, s[index - 1] + fint * (s[index] - s[index - 1])));
         const R asyco = fmax((R)0, fmin((R)1, a[index - 1] + fint * (a[index] - a[index - 1])));
@@ -344,7 +344,7 @@ __device__ __forceinline__ void sw_cloud(const Tabs<R> *__restrict__ T, int B, d
         const double factor = (dgeice - 2.0) / 3.0;
         const int index = imax(1, imin(45, (int)factor));
         const R fint = (R)(factor - (double)index);
-        const R *e = T->extice3 + B * 46, *s = T->ssaice3 + B * 46, *a = T->asyice3 + B * 46;
+        const WR_G R *e = T->extice3 + B * 46, *s = T->ssaice3 + B * 46, *a = T->asyice3 + B * 46;
         const R extco = fmax((R)0, e[index - 1] + fint * (e[index] - e[index - 1]));
         const R ssaco = fmax((R)0, fmin((R)1, s[index - 1] + fint * (s[index] - s[index - 1])));
         const R asyco = fmax((R)0, fmin((R)1, a[index - 1] + fint * (a[index] - a[index - 1])));
@@ -358,7 +358,7 @@ __device__ __forceinline__ void sw_cloud(const Tabs<R> *__restrict__ T, int B, d
     asy = asyliq + asyice + (R)0 + (R)0;
 }
 
-template <typename R> __device__ __forceinline__ R expm(R zb, const R *__restrict__ tbl)
+template <typename R> __device__ __forceinline__ R expm(R zb, const WR_G R *__restrict__ tbl)
 {
     if (zb <= (R)0.06) return (R)1 - zb + (R)0.5 * zb * zb;
     const R ftind = zb / ((R)(1.0 / 0.278) + zb);
@@ -369,7 +369,7 @@ template <typename R> __device__ __forceinline__ R expm(R zb, const R *__restric
 // delta scaling + PIFM + layer R/T (zssaw < zcrit branch; the conservative branch is unreachable
 // behind the 1 - 1e-6 clamp of zssaw, phys_rrtm_sw_spcvrtc.cu:54,117)
 template <typename R>
-__device__ __forceinline__ void layer_rt(R ztau0, R zssa0, R zasy0, R cz, R snt, const R *__restrict__ tbl, R &rb, R &rd,
+__device__ __forceinline__ void layer_rt(R ztau0, R zssa0, R zasy0, R cz, R snt, const WR_G R *__restrict__ tbl, R &rb, R &rd,
                                          R &tb, R &td, R &zldbt, bool *ill = nullptr)
 {
     const R zssaw = fmin((R)ONEMINUS, zssa0 / ztau0);
