@@ -53,6 +53,24 @@ template <typename T> inline WR_G T *gptr(T *p) { return (WR_G T *)p; }
 __device__ __forceinline__ int imin(int a, int b) { return a < b ? a : b; }
 __device__ __forceinline__ int imax(int a, int b) { return a > b ? a : b; }
 
+// Entry i of the LW/SW look-up tables (transmittance e, Pade source factor tfn, optical depth tau at
+// tblind = i / 10000; the FP64 tables are built by k_luts), evaluated in FP32 where it is needed: the
+// FP32 kernels spend the few operations instead of three dependent table gathers per g-point.  The
+// FP64 kernels read the tables, which the CPU oracle reproduces bit for bit.
+struct LutF {
+    float e, tfn, tau;
+};
+__device__ __forceinline__ LutF lut_f32(int i, bool lw_cap)
+{
+    if (i <= 0) return {1.0f, 0.0f, 0.0f};
+    if (i >= NTBL) return {lw_cap ? 1.0e-20f : 0.0f, 1.0f, 1.0e10f};
+    const float tau = (float)(1.0 / 0.278) * ((float)i / (float)(NTBL - i));
+    const float e = (lw_cap && tau >= 87.0f) ? 1.0e-20f : M<float>::exp(-tau);
+    const float tfn = (tau < 0.06f) ? tau / 6.0f : 1.0f - 2.0f * ((1.0f / tau) - (e / (1.0f - e)));
+    return {e, tfn, tau};
+}
+
+
 // ---- tables --------------------------------------------------------------------------------
 template <typename R> struct LwBand {
     const WR_G R *absa, *absb, *selfref, *forref, *fracrefa, *fracrefb;

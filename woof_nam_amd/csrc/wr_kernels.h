@@ -206,10 +206,15 @@ template <typename R> __device__ __forceinline__ GasT<R> lw_gas(const Tabs<R> *_
     } else {
         const R tblind = od / ((R)(1.0 / 0.278) + od);
         const int itgas = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
-        q.trng = T->exp_tbl[itgas];
+        if constexpr (sizeof(R) == 4) {
+            const LutF l = lut_f32(itgas, true);
+            q.trng = l.e, q.gasfac = l.tfn, od = l.tau;
+        } else {
+            q.trng = T->exp_tbl[itgas];
+            q.gasfac = T->tfn_tbl[itgas];
+            od = T->tau_tbl[itgas];
+        }
         q.atrgas = (R)1 - q.trng;
-        q.gasfac = T->tfn_tbl[itgas];
-        od = T->tau_tbl[itgas];
     }
     q.odepth = od;
     return q;
@@ -222,8 +227,13 @@ template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__re
     } else {
         const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
         const int ittot = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
-        totfac = T->tfn_tbl[ittot];
-        atrtot = (R)1 - T->exp_tbl[ittot];
+        if constexpr (sizeof(R) == 4) {
+            const LutF l = lut_f32(ittot, true);
+            totfac = l.tfn, atrtot = (R)1 - l.e;
+        } else {
+            totfac = T->tfn_tbl[ittot];
+            atrtot = (R)1 - T->exp_tbl[ittot];
+        }
     }
 }
 

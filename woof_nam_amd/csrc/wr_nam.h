@@ -160,8 +160,13 @@ __device__ __forceinline__ void lw_tot3(const Tabs<R> *__restrict__ T, R odtot, 
     } else {
         const R tblind = odtot / ((R)(1.0 / 0.278) + odtot);
         const int ittot = imin(NTBL, (int)((R)10000 * tblind + (R)0.5));
-        totfac = T->tfn_tbl[ittot];
-        trnt = T->exp_tbl[ittot];
+        if constexpr (sizeof(R) == 4) {
+            const LutF l = lut_f32(ittot, true);
+            totfac = l.tfn, trnt = l.e;
+        } else {
+            totfac = T->tfn_tbl[ittot];
+            trnt = T->exp_tbl[ittot];
+        }
         atrtot = (R)1 - trnt;
     }
 }

@@ -363,6 +363,7 @@ template <typename R> __device__ __forceinline__ R expm(R zb, const WR_G R *__re
     if (zb <= (R)0.06) return (R)1 - zb + (R)0.5 * zb * zb;
     const R ftind = zb / ((R)(1.0 / 0.278) + zb);
     const int itind = imax(0, imin(10000, (int)((R)0.5 + (R)10000 * ftind)));
+    if constexpr (sizeof(R) == 4) return lut_f32(itind, false).e;
     return tbl[itind];
 }
 
