@@ -115,6 +115,14 @@ struct RunPar {
 
 // Radiation-grid accessors of one column (grrad_setup phys_rrtm_grrad_setup.cu:102,
 // grrad_gases .cu:56): layer kin top-first, kin = 0 is the layer added above the model top.
+// The inputs of one radiation layer l as the layer physics uses them: interface pressures plvl(l),
+// plvl(l+1), layer pressure, temperature, water vapour, ozone and progcld2's cloud triple.  The band
+// kernels load the next layer's values one iteration ahead, so that their HBM latency overlaps the
+// current layer's work.
+struct LayV {
+    double pt, pb, pl, t, q, o3, ct, cwp, cip;
+};
+
 struct RadCol {
     ColIn in;  // the call's field pointers, held by value (uniform, in scalar registers)
     int i;    // column index into the fields
@@ -178,7 +186,19 @@ struct RadCol {
             cip = 0.0;
         }
     }
+    __device__ __forceinline__ LayV lay(int l) const
+    {
+        LayV v;
+        v.pt = plvl(l), v.pb = plvl(l + 1), v.pl = plyr(l), v.t = tlyr(l), v.q = qlyr(l), v.o3 = olyr(l);
+        cloud(l, v.ct, v.cwp, v.cip);
+        return v;
+    }
 };
+// tlvl of RadCol from the values of the layers below (t, la_l) and above (tm, la_lm1) an interface
+__device__ __forceinline__ double tlvl_v(double t, double tm, double lb_l, double la_l, double la_lm1)
+{
+    return t + (tm - t) * (lb_l - la_l) / (la_lm1 - la_l);
+}
 
 // ---- McICA sub-columns: phys_rrtm_lw_mcica_subcol.cu:105-168 draw order and max-random rule,
 // Philox4x32-10 stream keyed (seed, column), counter (column, g | kind << 16, k / 4) ------------

@@ -263,15 +263,13 @@ __device__ __forceinline__ void surr_lw(const Surr &sg, size_t s, R *tau, R *fr,
 }
 
 // the layer quantities the LW transfer needs besides the gas optics
-template <typename R> __device__ __forceinline__ void lw_layer_cloud(const RadCol &rc, int kin, LwLay<R> &L)
+template <typename R> __device__ __forceinline__ void lw_layer_cloud(const LayV &v, LwLay<R> &L)
 {
-    double ct, cwp, cip;
-    rc.cloud(kin, ct, cwp, cip);
-    L.ct = ct;
-    L.clwp = cwp * 1.5;
-    L.ciwp = cip * 1.5;
-    L.tavel = rc.tlyr(kin);
-    L.delp = rc.plvl(kin + 1) - rc.plvl(kin);
+    L.ct = v.ct;
+    L.clwp = v.cwp * 1.5;
+    L.ciwp = v.cip * 1.5;
+    L.tavel = v.t;
+    L.delp = v.pb - v.pt;
 }
 
 // ---- per-wave level cache of the SW kernels ----------------------------------------------------------
@@ -351,24 +349,29 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
     R rad[NGL];
 #pragma unroll
     for (int jj = 0; jj < NGL; ++jj) rad[jj] = (R)0;
-    double tz_top = rc.tlyr(0);
-    double la_c = log_d(rc.plyr(0));
+    // layer inputs: cur = layer kin, nxt = kin + 1 (loaded an iteration ahead), nn = kin + 2
+    LayV cur = rc.lay(0), nxt = cur;
+    if (nrl > 1) nxt = rc.lay(1);
+    double tz_top = cur.t;
+    double la_c = log_d(cur.pl);
     for (int kin = 0; kin < nrl; ++kin) {
         const int k = nrl - 1 - kin;
+        LayV nn = nxt;
+        if (kin + 2 < nrl) nn = rc.lay(kin + 2);
         double tz_bot, la_n = 0.0;
         if (kin + 1 < nrl) {
-            la_n = log_d(rc.plyr(kin + 1));
-            tz_bot = rc.tlvl(kin + 1, la_n, la_c, log_d(rc.plvl(kin + 1)));
+            la_n = log_d(nxt.pl);
+            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
         } else {
             tz_bot = rc.tskn();
         }
         LwLay<R> L;
         R tau[NGL], fr[NGL];
         if (SURR && !table) {
-            lw_layer_cloud<R>(rc, kin, L);
+            lw_layer_cloud<R>(cur, L);
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
-            lw_layer<R>(T, rc, par, kin, la_c, L);
+            lw_layer<R>(T, cur, par, la_c, L);
             lw_optics<R, B, NGL>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
@@ -399,27 +402,33 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
         if (lead) acc.lw_dn[(size_t)k * n + t] = (double)s;
         tz_top = tz_bot;
         la_c = la_n;
+        cur = nxt, nxt = nn;
     }
     // ---- up sweep, surface to top ----
     const R reflct = (R)1 - semis;
     double tz_bot = rc.tskn();
-    la_c = log_d(rc.plyr(nrl - 1));
+    // layer inputs: cur = layer kin, nxt = kin - 1 (loaded an iteration ahead), nn = kin - 2
+    cur = rc.lay(nrl - 1), nxt = cur;
+    if (nrl > 1) nxt = rc.lay(nrl - 2);
+    la_c = log_d(cur.pl);
     for (int kin = nrl - 1; kin >= 0; --kin) {
         const int k = nrl - 1 - kin;
+        LayV nn = nxt;
+        if (kin >= 2) nn = rc.lay(kin - 2);
         double la_m = 0.0;
         if (kin > 0) {
-            la_m = log_d(rc.plyr(kin - 1));
-            tz_top = rc.tlvl(kin, la_c, la_m, log_d(rc.plvl(kin)));
+            la_m = log_d(nxt.pl);
+            tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_m);
         } else {
-            tz_top = rc.tlyr(0);
+            tz_top = cur.t;
         }
         LwLay<R> L;
         R tau[NGL], fr[NGL];
         if (SURR && !table) {
-            lw_layer_cloud<R>(rc, kin, L);
+            lw_layer_cloud<R>(cur, L);
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
-            lw_layer<R>(T, rc, par, kin, la_c, L);
+            lw_layer<R>(T, cur, par, la_c, L);
             lw_optics<R, B, NGL>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
@@ -462,6 +471,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
         if (lead) acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
         tz_bot = tz_top;
         la_c = la_m;
+        cur = nxt, nxt = nn;
     }
 }
 

@@ -23,15 +23,15 @@ template <typename R> struct LwLay {
 
 // lwrad_prep + setcoef of surface-first layer k (top-first kin); la = log(plyr(kin)) already known.
 template <typename R>
-__device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int kin,
-                                         double la, LwLay<R> &L)
+__device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const LayV &v, const RunPar &par, double la,
+                                         LwLay<R> &L)
 {
 #pragma clang fp contract(off)
     const double tem1g = 100.0 * CON_G;
     const double tem2c = 1.0e-20 * 1.0e3 * CON_AVGD;
-    const double pav = rc.plyr(kin), tav = rc.tlyr(kin);
-    const double dlp = rc.plvl(kin + 1) - rc.plvl(kin);
-    const double qv = rc.qlyr(kin), ov = rc.olyr(kin);
+    const double pav = v.pl, tav = v.t;
+    const double dlp = v.pb - v.pt;
+    const double qv = v.q, ov = v.o3;
     const double h2ovmr = fmax(0.0, qv * AMDW / (1.0 - qv));
     const double o3vmr = fmax(0.0, ov * AMDO3);
     const double tem0 = (1.0 - h2ovmr) * CON_AMD + h2ovmr * CON_AMW;
@@ -50,11 +50,9 @@ __device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const Ra
     for (int m = 1; m < 7; ++m) summol = summol + ca[m];
     const double colbrd = cdry - summol;
     const double h2o = ca[0] / fmax(cdry, 1e-30);  // lw_core.py:238
-    double ct, cwp, cip;
-    rc.cloud(kin, ct, cwp, cip);
-    L.ct = ct;
-    L.clwp = cwp * 1.5;  // CPATHFAC4LW (np3d = 3), phys_rrtm_grrad_lw.cu:58
-    L.ciwp = cip * 1.5;
+    L.ct = v.ct;
+    L.clwp = v.cwp * 1.5;  // CPATHFAC4LW (np3d = 3), phys_rrtm_grrad_lw.cu:58
+    L.ciwp = v.cip * 1.5;
     L.tavel = tav;
     L.delp = dlp;
     // setcoef
@@ -110,6 +108,13 @@ __device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const Ra
     L.coldry = (R)cdry;
     L.colbrd = (R)colbrd;
     L.pavel = (R)pav;
+}
+
+template <typename R>
+__device__ __forceinline__ void lw_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int kin,
+                                         double la, LwLay<R> &L)
+{
+    lw_layer<R>(T, rc.lay(kin), par, la, L);
 }
 
 // reference ratio chi_mls(a, lev) / chi_mls(b, lev) (setcoef rfrate, FP64)

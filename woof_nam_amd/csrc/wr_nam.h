@@ -195,19 +195,24 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
     R radtotd[NG], totradd[NG], clrradd[NG], rad[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) radtotd[g] = totradd[g] = clrradd[g] = rad[g] = (R)0;
-    double tz_top = rc.tlyr(0);
-    double la_c = log_d(rc.plyr(0));
+    // layer inputs: cur = layer kin, nxt = kin + 1 (loaded an iteration ahead), nn = kin + 2
+    LayV cur = rc.lay(0), nxt = cur;
+    if (nrl > 1) nxt = rc.lay(1);
+    double tz_top = cur.t;
+    double la_c = log_d(cur.pl);
     for (int kin = 0; kin < nrl; ++kin) {
         const int kk = nrl - kin;  // 1-based surface-first layer
+        LayV nn = nxt;
+        if (kin + 2 < nrl) nn = rc.lay(kin + 2);
         double tz_bot, la_n = 0.0;
         if (kin + 1 < nrl) {
-            la_n = log_d(rc.plyr(kin + 1));
-            tz_bot = rc.tlvl(kin + 1, la_n, la_c, log_d(rc.plvl(kin + 1)));
+            la_n = log_d(nxt.pl);
+            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
         } else {
             tz_bot = rc.tskn();
         }
         LwLay<R> L;
-        lw_layer<R>(T, rc, par, kin, la_c, L);
+        lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
         lw_optics<R, B>(T, L, tau, fr);
         const R blay = planck<R>(T, B, L.tavel);
@@ -248,23 +253,29 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
         acc.lw_dn[(size_t)(kk - 1) * n + t] = (double)s;
         tz_top = tz_bot;
         la_c = la_n;
+        cur = nxt, nxt = nn;
     }
     // ---- up sweep, surface to top ----
     const R reflct = (R)1 - semis;
     R radtotu[NG], totradu[NG], clrradu[NG];
     double tz_bot = rc.tskn();
-    la_c = log_d(rc.plyr(nrl - 1));
+    // layer inputs: cur = layer kin, nxt = kin - 1 (loaded an iteration ahead), nn = kin - 2
+    cur = rc.lay(nrl - 1), nxt = cur;
+    if (nrl > 1) nxt = rc.lay(nrl - 2);
+    la_c = log_d(cur.pl);
     for (int kin = nrl - 1; kin >= 0; --kin) {
         const int kk = nrl - kin;
+        LayV nn = nxt;
+        if (kin >= 2) nn = rc.lay(kin - 2);
         double la_m = 0.0;
         if (kin > 0) {
-            la_m = log_d(rc.plyr(kin - 1));
-            tz_top = rc.tlvl(kin, la_c, la_m, log_d(rc.plvl(kin)));
+            la_m = log_d(nxt.pl);
+            tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_m);
         } else {
-            tz_top = rc.tlyr(0);
+            tz_top = cur.t;
         }
         LwLay<R> L;
-        lw_layer<R>(T, rc, par, kin, la_c, L);
+        lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
         lw_optics<R, B>(T, L, tau, fr);
         const R blay = planck<R>(T, B, L.tavel);
@@ -318,6 +329,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
         acc.lw_up[(size_t)kk * n + t] = (double)s;
         tz_bot = tz_top;
         la_c = la_m;
+        cur = nxt, nxt = nn;
     }
 }
 
