@@ -59,6 +59,9 @@ typedef struct {
     const uint64_t *col_id;  /* [ncol] global column index that keys the McICA stream; NULL = i  */
     const uint64_t *seed_dev;/* optional device pointer to the McICA seed; when set it replaces
                                 wr_params.seed (lets a captured HIP graph be replayed per request) */
+    const int32_t *ncol_dev; /* optional device pointer to the number of valid columns (<= ncol);
+                                columns from it on are skipped and their outputs left untouched
+                                (lets one graph captured for ncol columns serve shorter tiles)    */
 } wr_columns;
 
 /* Call parameters (NAM values by default; see wr_default_params). */

@@ -160,3 +160,22 @@ def test_nam_route_matches_oracle(rad, seed):
         a = out[k].cpu().numpy()
         assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), \
             f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
+
+
+@pytest.mark.parametrize("mcica", [1, 0])
+def test_valid_column_count_skips_the_rest(rad, mcica):
+    """ncol_dev = m < ncol (a replayed graph of a longer tile): the first m columns equal a call on
+    those m columns alone, bit for bit, and the outputs of the others are left as they were."""
+    print(f"case ncol_dev mcica={mcica}", flush=True)
+    cols = to_dev(gen.columns(700, seed=71), rad.device)
+    m = 413
+    params = Radiation.default_params(mcica=mcica)
+    out = {k: torch.full(s, -7.0, dtype=torch.float64, device=rad.device)
+           for k, s in abi.output_shapes(700, 60).items()}
+    part = dict(cols, ncol_dev=torch.tensor([m], dtype=torch.int32, device=rad.device))
+    rad.run(part, params=params, out=out)
+    alone, _ = rad.run({k: v[..., :m].contiguous() for k, v in cols.items()}, params=params)
+    torch.cuda.synchronize()
+    for k in abi.OUTPUT_FIELDS:
+        assert torch.equal(out[k][..., :m], alone[k]), k
+        assert bool((out[k][..., m:] == -7.0).all()), k

@@ -19,7 +19,8 @@ class WrColumns(C.Structure):
     _fields_ = [("ncol", C.c_int32), ("nlay", C.c_int32),
                 ("p_lev", _dp), ("t_lay", _dp), ("q_lay", _dp), ("o3_lay", _dp), ("cldf", _dp),
                 ("qc", _dp), ("qi", _dp), ("t_sfc", _dp), ("emis", _dp), ("alb", _dp), ("cosz", _dp),
-                ("col_id", C.POINTER(C.c_uint64)), ("seed_dev", C.POINTER(C.c_uint64))]
+                ("col_id", C.POINTER(C.c_uint64)), ("seed_dev", C.POINTER(C.c_uint64)),
+                ("ncol_dev", C.POINTER(C.c_int32))]
 
 
 class WrParams(C.Structure):

@@ -102,6 +102,7 @@ struct ColIn {
     const WR_G double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
     const WR_G uint64_t *col_id;
     const WR_G uint64_t *seed_dev;  // optional device copy of the McICA seed
+    const WR_G int32_t *ncol_dev;   // optional device count of valid columns (a shorter replayed tile)
     int ncol;  // stride of the [k][column] fields
     int nlay;  // model layers
 };
@@ -115,6 +116,14 @@ struct RunPar {
 
 // Radiation-grid accessors of one column (grrad_setup phys_rrtm_grrad_setup.cu:102,
 // grrad_gases .cu:56): layer kin top-first, kin = 0 is the layer added above the model top.
+// columns of the tile [col0, col0 + n) that hold data: all of them, or up to *ncol_dev
+__device__ __forceinline__ int valid_cols(const ColIn &in, int col0, int n)
+{
+    if (!in.ncol_dev) return n;
+    const int v = *in.ncol_dev - col0;
+    return v < 0 ? 0 : (v < n ? v : n);
+}
+
 // The inputs of one radiation layer l as the layer physics uses them: interface pressures plvl(l),
 // plvl(l+1), layer pressure, temperature, water vapour, ozone and progcld2's cloud triple.  The band
 // kernels load the next layer's values one iteration ahead, so that their HBM latency overlaps the

@@ -305,7 +305,7 @@ __global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T,
                                              ColScal cs, wr_diag dg)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= valid_cols(in, col0, n)) return;
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
@@ -435,7 +435,7 @@ ColIn to_colin(const wr_columns *in)
     ci.p_lev = gptr(in->p_lev), ci.t_lay = gptr(in->t_lay), ci.q_lay = gptr(in->q_lay), ci.o3_lay = gptr(in->o3_lay);
     ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
     ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
-    ci.seed_dev = gptr(in->seed_dev);
+    ci.seed_dev = gptr(in->seed_dev), ci.ncol_dev = gptr(in->ncol_dev);
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     return ci;
 }

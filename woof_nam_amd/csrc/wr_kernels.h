@@ -66,6 +66,10 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
+    if (t >= valid_cols(in, col0, n)) {  // padding column of a shorter tile: not in the day list
+        cs.day[t] = 0;
+        return;
+    }
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
@@ -328,7 +332,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
     const int t = gt / GS;
     const int g0 = (gt % GS) * NGL;
     const bool lead = (gt % GS) == 0;  // writes the column's level fluxes
-    if (t >= n) return;  // both lanes of a column leave together
+    if (t >= valid_cols(in, col0, n)) return;  // both lanes of a column leave together
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
@@ -664,7 +668,7 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= valid_cols(in, col0, n)) return;
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl, nm = in.nlay;

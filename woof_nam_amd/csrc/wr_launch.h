@@ -109,20 +109,25 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
     }
 }
 
-// True when the band kernels may leave stream s: auxiliary streams exist, s is not being captured
-// into a graph (a captured tile keeps one stream; graph replays overlap across slots) and every
-// auxiliary stream has its cache pool (allocated here on first use, outside any capture).
+// True when the band kernels may leave stream s: auxiliary streams exist and every one has its
+// cache pool.  Pools are allocated here on first use outside graph capture; under capture the fork
+// and join events bring the auxiliary streams into the graph (a captured tile then holds the same
+// band concurrency as an eager one), or the tile stays on s when a pool is missing.
 inline bool spread_bands(wr_ctx *c, hipStream_t s)
 {
     if (c->naux <= 0) return false;
     hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &st) != hipSuccess || st != hipStreamCaptureStatusNone) return false;
-    for (int q = 1; q < c->naux; ++q)
-        if (!c->pool[q] && hipMalloc((void **)&c->pool[q], c->pool_bytes) != hipSuccess) {
+    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
+    const bool capturing = st != hipStreamCaptureStatusNone;
+    for (int q = 1; q < c->naux; ++q) {
+        if (c->pool[q]) continue;
+        if (capturing) return false;
+        if (hipMalloc((void **)&c->pool[q], c->pool_bytes) != hipSuccess) {
             c->pool[q] = nullptr;
             (void)hipGetLastError();
             return false;
         }
+    }
     return true;
 }
 

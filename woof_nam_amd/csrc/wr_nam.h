@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(64) k_rtr_fac(ColIn in, int col0, int n, Rtr<R
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= valid_cols(in, col0, n)) return;
     const RadCol rc{in, col0 + t, in.nlay + 1};
     const int nl = rc.nrl;
     const double eps = 1.0e-6;
@@ -178,7 +178,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
 {
     constexpr int NG = LW_NG[B];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= valid_cols(in, col0, n)) return;
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;

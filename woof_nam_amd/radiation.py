@@ -40,16 +40,20 @@ def input_shapes(ncol: int, nlay: int) -> dict:
 
 
 def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
-    """WrColumns over device tensors; optional col_id [ncol] and seed_dev [1] are int64 tensors."""
-    ids, sd = cols.get("col_id"), cols.get("seed_dev")
+    """WrColumns over device tensors; optional col_id [ncol] and seed_dev [1] are int64 tensors,
+    optional ncol_dev [1] (valid columns, for a replayed graph of a longer tile) an int32 tensor."""
+    ids, sd, nd = cols.get("col_id"), cols.get("seed_dev"), cols.get("ncol_dev")
     if ids is not None:
         _check_field("col_id", ids, (ncol,), torch.int64, device)
     if sd is not None:
         _check_field("seed_dev", sd, (1,), torch.int64, device)
+    if nd is not None:
+        _check_field("ncol_dev", nd, (1,), torch.int32, device)
     u64 = C.POINTER(C.c_uint64)
     return abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
                          C.cast(C.c_void_p(ids.data_ptr()), u64) if ids is not None else None,
-                         C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None)
+                         C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None,
+                         C.cast(C.c_void_p(nd.data_ptr()), C.POINTER(C.c_int32)) if nd is not None else None)
 
 
 class Radiation:

@@ -8,11 +8,12 @@ seeds) arrive at any time.  The server
   on its own HIP stream with its own work space, so tiles of different requests overlap on the
   device; a tile is handed to the next free slot as soon as one frees up (requests that arrive
   while others run join the tile queue at once: continuous batching at tile granularity);
-* runs each tile as one replay of a HIP graph captured once per slot (the 35 kernels of a tile:
-  accumulator clear, column scalars, day list, 16 LW and 14 SW band kernels, post) over the
-  slot's staging buffers; a tile is gathered into the staging buffers, the graph replayed and
-  the outputs scattered back, all on the slot's stream, with the request's seed read by the
-  kernels from device memory;
+* runs each tile as one replay of a HIP graph captured once per slot (the kernels of a tile:
+  column scalars, day list, 16 LW and 14 SW band kernels spread over the context's band streams,
+  post) over the slot's staging buffers; a tile is gathered into the staging buffers, the graph
+  replayed and the outputs scattered back, all on the slot's stream, with the request's seed and
+  the tile's column count read by the kernels from device memory (a short last tile costs only
+  its own columns);
 * records per-request latency (arrival to last tile done) for p50/p99.
 
 Every column's result depends only on its own inputs, its column id and its request's seed, so a
@@ -73,6 +74,7 @@ class _Slot:
             self.inp[k].copy_(filler[k])
         self.inp["col_id"] = torch.arange(tile, dtype=torch.int64, device=device)
         self.inp["seed_dev"] = torch.zeros(1, dtype=torch.int64, device=device)
+        self.inp["ncol_dev"] = torch.full((1,), tile, dtype=torch.int32, device=device)   # valid columns
         self.out = {k: torch.empty(s, dtype=torch.float64, device=device)
                     for k, s in abi.output_shapes(tile, nlay).items()}
         self.event = torch.cuda.Event()
@@ -99,6 +101,7 @@ class _Slot:
             else:
                 self.inp["col_id"][:n].copy_(torch.arange(c0, c0 + n, dtype=torch.int64, device=self.inp["col_id"].device))
             self.inp["seed_dev"].fill_(int(np.int64(np.uint64(req.seed))))
+            self.inp["ncol_dev"].fill_(n)                  # the graph skips the tile's unused columns
             if self.graph is not None:
                 self.graph.replay()
             else:
