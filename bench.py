@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11696, "k_lw_b02": 11502, "k_lw_b03": 8067, "k_lw_b04": 9063, "k_lw_b05": 9232, "k_lw_b06": 9938,
-    "k_lw_b07": 8900, "k_lw_b08": 8564, "k_lw_b09": 7829, "k_lw_b10": 9130, "k_lw_b11": 9877, "k_lw_b12": 7601,
-    "k_lw_b13": 8091, "k_lw_b14": 7654, "k_lw_b15": 8459, "k_lw_b16": 7467,
-    "k_sw_b16": 20020, "k_sw_b17": 41438, "k_sw_b18": 24399, "k_sw_b19": 24383, "k_sw_b20": 28747,
-    "k_sw_b21": 28760, "k_sw_b22": 11269, "k_sw_b23": 28757, "k_sw_b24": 25337, "k_sw_b25": 21012,
-    "k_sw_b26": 20036, "k_sw_b27": 25326, "k_sw_b28": 20984, "k_sw_b29": 35641,
+    "k_lw_b01": 8794, "k_lw_b02": 8035, "k_lw_b03": 8457, "k_lw_b04": 9275, "k_lw_b05": 9466, "k_lw_b06": 7621,
+    "k_lw_b07": 9026, "k_lw_b08": 8598, "k_lw_b09": 8064, "k_lw_b10": 9169, "k_lw_b11": 7611, "k_lw_b12": 7636,
+    "k_lw_b13": 8146, "k_lw_b14": 7656, "k_lw_b15": 8475, "k_lw_b16": 7486, "k_sw_b16": 20023, "k_sw_b17": 37554,
+    "k_sw_b18": 24406, "k_sw_b19": 24387, "k_sw_b20": 28746, "k_sw_b21": 28767, "k_sw_b22": 11277,
+    "k_sw_b23": 28760, "k_sw_b24": 25346, "k_sw_b25": 21008, "k_sw_b26": 20036, "k_sw_b27": 25325,
+    "k_sw_b28": 20982, "k_sw_b29": 33104,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 11623, "k_lw_b02": 11439, "k_lw_b03": 20854, "k_lw_b04": 13159, "k_lw_b05": 18798,
-    "k_lw_b06": 9900, "k_lw_b07": 13138, "k_lw_b08": 10878, "k_lw_b09": 11874, "k_lw_b10": 9564, "k_lw_b11": 9872,
-    "k_lw_b12": 9915, "k_lw_b13": 9330, "k_lw_b14": 7647, "k_lw_b15": 8613, "k_lw_b16": 7648, "k_sw_b16": 14055,
-    "k_sw_b17": 21639, "k_sw_b18": 16425, "k_sw_b19": 16416, "k_sw_b20": 19103, "k_sw_b21": 19103,
-    "k_sw_b22": 9247, "k_sw_b23": 19081, "k_sw_b24": 17360, "k_sw_b25": 15011, "k_sw_b26": 14032,
-    "k_sw_b27": 17397, "k_sw_b28": 14982, "k_sw_b29": 21601,
+    "k_lw_b01": 11721, "k_lw_b02": 11466, "k_lw_b03": 8394, "k_lw_b04": 9162, "k_lw_b05": 9385, "k_lw_b06": 9949,
+    "k_lw_b07": 8985, "k_lw_b08": 10968, "k_lw_b09": 8015, "k_lw_b10": 9139, "k_lw_b11": 9937, "k_lw_b12": 7579,
+    "k_lw_b13": 9339, "k_lw_b14": 7643, "k_lw_b15": 8432, "k_lw_b16": 7459, "k_sw_b16": 14031, "k_sw_b17": 21535,
+    "k_sw_b18": 16398, "k_sw_b19": 16408, "k_sw_b20": 19096, "k_sw_b21": 18993, "k_sw_b22": 9232,
+    "k_sw_b23": 19052, "k_sw_b24": 17359, "k_sw_b25": 14984, "k_sw_b26": 14022, "k_sw_b27": 17354,
+    "k_sw_b28": 14976, "k_sw_b29": 21502,
 }
 
 
