@@ -93,7 +93,7 @@ def served_block(d, dev, ncol, seconds):
     kinds = {"parent": (d, pid), "nest3km": dev_grid(333, 333, 11, 10_000_000),
              "nest6km": dev_grid(200, 200, 12, 20_000_000), "member": (d, pid)}
     t0 = time.perf_counter()
-    tune = autotune(0, tiles=(65536, 131072, 262144), slots=(2, 4), ncol=262144, precision="mixed")
+    tune = autotune(0, tiles=(65536, 131072, 262144), slots=(2, 4), ncol=786432, precision="mixed", nreq=12)
     best = tune["best"]
     print(f"# autotune {time.perf_counter() - t0:.1f} s: tile {best['tile']} slots {best['slots']}",
           file=sys.stderr, flush=True)

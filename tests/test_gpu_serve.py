@@ -99,3 +99,12 @@ def test_poisson_stream_reports_latency(gpu_device):
     assert res["requests"] > 10
     assert res["p99_ms"] >= res["p50_ms"] > 0
     assert res["columns_per_s"] > 0
+
+
+def test_autotune_picks_a_measured_shape(gpu_device):
+    print("case autotune", flush=True)
+    from woof_nam_amd.serve import autotune
+    r = autotune(0, tiles=(4096, 8192), slots=(1, 2), ncol=20000, precision="mixed", nreq=5)
+    assert len(r["all"]) == 4
+    assert all(x["columns_per_s"] > 0 for x in r["all"])
+    assert r["best"] == max(r["all"], key=lambda x: x["columns_per_s"])

@@ -256,15 +256,16 @@ def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed:
 
 
 def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), ncol: int = 262144,
-             precision: str = "mixed", cache: str | None = None) -> dict:
+             precision: str = "mixed", cache: str | None = None, nreq: int = 4) -> dict:
     """Pick the tile size and slot count with the highest columns/s on a fixed batch of requests.
 
-    The batch is `ncol` generated columns submitted as 4 requests of uneven size.  Results are
-    cached in `cache` (JSON) keyed by device name and precision when a path is given.
+    The batch is `ncol` generated columns submitted at once as `nreq` requests of uneven size
+    (a queue deep enough to keep every slot busy, as under load).  Results are cached in `cache`
+    (JSON) keyed by device name, precision and batch when a path is given.
     """
     from . import gen
     name = torch.cuda.get_device_name(device)
-    key = f"{name}|{precision}|{ncol}"
+    key = f"{name}|{precision}|{ncol}|{nreq}"
     if cache and os.path.exists(cache):
         with open(cache) as f:
             db = json.load(f)
@@ -272,7 +273,8 @@ def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), nco
             return db[key]
     cols = gen.columns(ncol, seed=7)
     dev = {k: torch.from_numpy(np.ascontiguousarray(v)).to(f"cuda:{device}") for k, v in cols.items()}
-    cuts = [0, ncol // 8, ncol // 3, (2 * ncol) // 3, ncol]
+    w = np.array([1 + (3 * j) % 5 for j in range(nreq)], dtype=np.float64)   # uneven request sizes
+    cuts = [0] + [int(x) for x in np.round(np.cumsum(w) / w.sum() * ncol)]
     results = []
     for tile in tiles:
         for ns in slots:
@@ -282,7 +284,7 @@ def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), nco
             best = None
             for rep in range(2):
                 reqs = [Request(rid=j, kind="tune", cols={k: v[..., cuts[j]:cuts[j + 1]].contiguous() for k, v in dev.items()},
-                                seed=j) for j in range(4)]
+                                seed=j) for j in range(nreq)]
                 torch.cuda.synchronize(device)
                 t0 = time.perf_counter()
                 for r in reqs:
