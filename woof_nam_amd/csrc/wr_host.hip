@@ -9,6 +9,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -538,7 +539,8 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     if (rc == WR_OK) {
         // SW level-cache pool: room for every 64-column block of a tile in the largest layout
         // (2 values x 16 g x 65 levels, FP64), at most WR_POOL_GB (default 6) GB per stream
-        const size_t lane_max = cache_slot_bytes(SW_NV, 16, c->nl_max + 1, (int)sizeof(double));
+        const size_t lane_max = std::max(cache_slot_bytes(SW_NV, 16, c->nl_max + 1, (int)sizeof(double)),
+                                         cache_slot_bytes(sw_nv_mc<float>(), 16, c->nl_max + 1, (int)sizeof(float)));
         const size_t need = (m + 63) / 64 * lane_max;
         const char *pe = getenv("WR_POOL_GB");
         const double gb = pe ? atof(pe) : 6.0;

@@ -85,7 +85,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg);
         else
-            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3(band_grid(j, SW_NV, SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, j.sg, pool);
         tend(j.c, s);
     }
@@ -103,7 +103,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             hipLaunchKernelGGL((k_lwr<R, B>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
                                j.cs, acc, j.rt);
         else
-            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3(band_grid(j, SW_NV, SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
         tend(j.c, s);
     }

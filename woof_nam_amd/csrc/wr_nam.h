@@ -375,6 +375,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
                                            int n, const ColScal &cs, const R *__restrict__ sflux, const Acc &acc, int t,
                                            const LaneCache<R> &cc)
 {
+    constexpr bool CACHE_RT = sw_nv_mc<R>() > SW_NV;  // FP32: layer R/T kept for the top-down sweep
     constexpr int NG = SW_NG[B];
     constexpr int NS = SW_NS[B];
     const int i = col0 + t;
@@ -413,6 +414,13 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
                 pb[g] = nb, pd[g] = nd;
                 cc.at(0, NG, g, k + 1) = nb;
                 cc.at(1, NG, g, k + 1) = nd;
+                if constexpr (CACHE_RT) {
+                    cc.at(2, NG, g, k) = o.rb[g];
+                    cc.at(3, NG, g, k) = o.rd[g];
+                    cc.at(4, NG, g, k) = o.tb[g];
+                    cc.at(5, NG, g, k) = o.td[g];
+                    cc.at(6, NG, g, k) = o.ld[g];
+                }
             }
         }
         // top-down and level fluxes (layer R/T recomputed); pass 0 leaves the clear-sky fluxes in the
@@ -431,7 +439,15 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
         for (int s = nrl; s >= 0; --s) {
             if (s < nrl) {
                 SwLayerRT<R, B> o;
-                swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o);
+                if constexpr (CACHE_RT) {
+#pragma unroll
+                    for (int g = 0; g < NG; ++g) {
+                        o.rb[g] = cc.at(2, NG, g, s), o.rd[g] = cc.at(3, NG, g, s), o.tb[g] = cc.at(4, NG, g, s);
+                        o.td[g] = cc.at(5, NG, g, s), o.ld[g] = cc.at(6, NG, g, s);
+                    }
+                } else {
+                    swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o);
+                }
                 su = (R)0, sd = (R)0;
 #pragma unroll
                 for (int g = 0; g < NG; ++g) {
@@ -472,7 +488,7 @@ template <typename R, int B>
 __global__ void __launch_bounds__(64, WR_SW_WAVES) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                             ColScal cs, const R *__restrict__ sflux, Acc acc, char *__restrict__ pool)
 {
-    const LaneCache<R> cc = lane_cache<R>(pool, SW_NV, SW_NG[B], in.nlay + 2);
+    const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
     const int nday = *cs.nday;
     for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
         const int tt = blk * 64 + threadIdx.x;
