@@ -482,6 +482,131 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
     }
 }
 
+// FP32 LW band kernel with a cached upward sweep.  The down sweep also forms, per layer and g-point,
+// the upward transmittance a = trng * efclrfr and source b (gas plus cloud term) and keeps them in the
+// lane's pool slot (8 bytes per layer and g-point); the up sweep is then rad = rad * a + b, with no
+// optics.  The FP64 kernels recompute the upward optics instead: their sums follow the reference's
+// operation order, and twice the bytes would not pay.
+constexpr int LW_NV_F32 = 2;
+template <int B, bool SURR>
+__device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T, const ColIn &in, const RunPar &par,
+                                               int col0, int n, const ColScal &cs, const Acc &acc, const Surr &sg,
+                                               int t, const LaneCache<float> &cc)
+{
+    using R = float;
+    constexpr int NG = LW_NG[B];
+    constexpr int NS = LW_NS[B];
+    const int i = col0 + t;
+    const RadCol rc{in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const int lcf1 = cs.lcf1[t];
+    const int table = SURR ? sg.flag[t] : 1;
+    const double pwv = cs.pwv[t];
+    const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
+    const double sfe = in.emis[i];
+    const R semis = (R)((sfe > EPS && sfe <= 1.0) ? sfe : 1.0);
+    const R reflct = (R)1 - semis;
+    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
+    uint64_t mb[NG];
+    for (int g = 0; g < NG; ++g)
+        mb[g] = mcica_g(col, seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+
+    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
+    R rad[NG];
+#pragma unroll
+    for (int g = 0; g < NG; ++g) rad[g] = (R)0;
+    LayV cur = rc.lay(0);  // layer kin; the next one is loaded an iteration ahead (its p and T are
+                           // needed at once for the level temperature below the layer)
+    double tz_top = cur.t;
+    double la_c = log_d(cur.pl);
+    for (int kin = 0; kin < nrl; ++kin) {
+        const int k = nrl - 1 - kin;
+        const LayV nxt = (kin + 1 < nrl) ? rc.lay(kin + 1) : cur;
+        double tz_bot, la_n = 0.0;
+        if (kin + 1 < nrl) {
+            la_n = log_d(nxt.pl);
+            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
+        } else {
+            tz_bot = rc.tskn();
+        }
+        LwLay<R> L;
+        R tau[NG], fr[NG];
+        if (SURR && !table) {
+            lw_layer_cloud<R>(cur, L);
+            surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
+        } else {
+            lw_layer<R>(T, cur, par, la_c, L);
+            lw_optics<R, B>(T, L, tau, fr);
+        }
+        const R blay = planck<R>(T, B, L.tavel);
+        const R dplnkd = planck<R>(T, B, tz_bot) - blay;
+        const R dplnku = planck<R>(T, B, tz_top) - blay;
+        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const R plfrac = fr[g];
+            const R gassrcd = plfrac * (blay + dplnkd * q.gasfac) * q.atrgas;
+            const R gassrcu = plfrac * (blay + dplnku * q.gasfac) * q.atrgas;
+            const R clfm = ((mb[g] >> k) & 1) ? (R)1 : (R)0;
+            R a = q.trng, b = gassrcu;
+            if (clfm >= (R)EPS) {
+                const R odcld = sdif * taucld;
+                const R efclrfr = (R)1 - ((R)1 - M<R>::exp(-odcld)) * clfm;
+                R totfac, atrtot;
+                lw_tot<R>(T, q.odepth + odcld, totfac, atrtot);
+                const R totsrcd = plfrac * (blay + dplnkd * totfac) * atrtot;
+                const R totsrcu = plfrac * (blay + dplnku * totfac) * atrtot;
+                rad[g] = rad[g] * q.trng * efclrfr + gassrcd + clfm * (totsrcd - gassrcd);
+                a = q.trng * efclrfr;
+                b = gassrcu + clfm * (totsrcu - gassrcu);
+            } else {
+                rad[g] = rad[g] * q.trng + gassrcd;
+            }
+            s = s + rad[g];
+            cc.at(0, NG, g, k) = a;
+            cc.at(1, NG, g, k) = b;
+        }
+        acc.lw_dn[(size_t)k * n + t] = (double)s;
+        if (k == 0) {  // surface emission and reflection start the upward sweep
+            const R pk0 = planck<R>(T, B, rc.tskn());
+            R s0 = (R)0;
+#pragma unroll
+            for (int g = 0; g < NG; ++g) {
+                rad[g] = semis * fr[g] * pk0 + reflct * rad[g];
+                s0 = s0 + rad[g];
+            }
+            acc.lw_up[t] = (double)s0;
+        }
+        tz_top = tz_bot;
+        la_c = la_n;
+        cur = nxt;
+    }
+    for (int k = 0; k < nrl; ++k) {  // up sweep over the cached layers
+        R s = (R)0;
+#pragma unroll
+        for (int g = 0; g < NG; ++g) {
+            rad[g] = rad[g] * cc.at(0, NG, g, k) + cc.at(1, NG, g, k);
+            s = s + rad[g];
+        }
+        acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
+    }
+}
+
+template <int B, bool SURR = false>
+__global__ void __launch_bounds__(64, WR_LW_WAVES) k_lwc(const Tabs<float> *__restrict__ T, ColIn in, RunPar par,
+                                                         int col0, int n, ColScal cs, Acc acc, Surr sg,
+                                                         char *__restrict__ pool)
+{
+    const LaneCache<float> cc = lane_cache<float>(pool, LW_NV_F32, LW_NG[B], in.nlay + 2);
+    for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
+        const int t = blk * 64 + threadIdx.x;
+        if (t < valid_cols(in, col0, n)) lw_column_f32c<B, SURR>(T, in, par, col0, n, cs, acc, sg, t, cc);
+    }
+}
+
 // ---- SW band kernel (McICA two-stream) -------------------------------------------------------------
 // single-scattering albedo above which the mixed path evaluates the SW layer R/T in FP64
 constexpr double W0_FP64 = 0.999999;

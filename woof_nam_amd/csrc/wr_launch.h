@@ -81,7 +81,10 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
         if (id != B) return launch_band<R, SURR, B + 1>(j, id, s, pool);
         const Acc acc = band_acc(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
-        if constexpr (B < 16)
+        if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1)
+            hipLaunchKernelGGL((k_lwc<B, SURR>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
+                               j.par, j.col0, j.n, j.cs, acc, j.sg, pool);
+        else if constexpr (B < 16)
             hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg);
         else
