@@ -117,7 +117,7 @@ def main(argv=None):
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
-    ap.add_argument("--max-cols", type=int, default=262144)
+    ap.add_argument("--max-cols", type=int, default=0, help="tile columns (0 = the whole grid in one tile)")
     ap.add_argument("--nx", type=int, default=954)
     ap.add_argument("--ny", type=int, default=835)
     ap.add_argument("--cpu-cols", type=int, default=0, help="CPU baseline sample (0 = sized from cores)")
@@ -154,6 +154,7 @@ def main(argv=None):
           file=sys.stderr, flush=True)
     dev = torch.device("cuda", 0)
     d = {k: torch.from_numpy(v).to(dev) for k, v in cols.items()}
+    a.max_cols = a.max_cols or ncol
     rad = Radiation(0, max_cols=a.max_cols)
     out, _ = rad.run(d, precision=a.precision)
     torch.cuda.synchronize()

@@ -56,7 +56,7 @@ def _context(ctx, device, ncol):
     tables = ctx.tables
     rad = tables.get(KEY_CTX)
     if rad is None or rad.device != device:
-        rad = Radiation(device.index or 0, max_cols=min(max(ncol, 1), int(_cfg(ctx, "rrtm_tile", 262144))))
+        rad = Radiation(device.index or 0, max_cols=min(max(ncol, 1), int(_cfg(ctx, "rrtm_tile", 1048576))))
         tables[KEY_CTX] = rad
     return rad
 
