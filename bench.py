@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 8794, "k_lw_b02": 8035, "k_lw_b03": 8457, "k_lw_b04": 9275, "k_lw_b05": 9466, "k_lw_b06": 7621,
-    "k_lw_b07": 9026, "k_lw_b08": 8598, "k_lw_b09": 8064, "k_lw_b10": 9169, "k_lw_b11": 7611, "k_lw_b12": 7636,
-    "k_lw_b13": 8146, "k_lw_b14": 7656, "k_lw_b15": 8475, "k_lw_b16": 7486, "k_sw_b16": 20023, "k_sw_b17": 37554,
-    "k_sw_b18": 24406, "k_sw_b19": 24387, "k_sw_b20": 28746, "k_sw_b21": 28767, "k_sw_b22": 11277,
-    "k_sw_b23": 28760, "k_sw_b24": 25346, "k_sw_b25": 21008, "k_sw_b26": 20036, "k_sw_b27": 25325,
-    "k_sw_b28": 20982, "k_sw_b29": 33104,
+    "k_lw_b01": 8781, "k_lw_b02": 8053, "k_lw_b03": 8482, "k_lw_b04": 9271, "k_lw_b05": 9470, "k_lw_b06": 7619,
+    "k_lw_b07": 9027, "k_lw_b08": 8589, "k_lw_b09": 8062, "k_lw_b10": 9174, "k_lw_b11": 7606, "k_lw_b12": 7636,
+    "k_lw_b13": 8148, "k_lw_b14": 7657, "k_lw_b15": 8473, "k_lw_b16": 7488, "k_sw_b16": 20027, "k_sw_b17": 37555,
+    "k_sw_b18": 24405, "k_sw_b19": 24383, "k_sw_b20": 28744, "k_sw_b21": 28768, "k_sw_b22": 11276,
+    "k_sw_b23": 28755, "k_sw_b24": 25348, "k_sw_b25": 21009, "k_sw_b26": 20043, "k_sw_b27": 25330,
+    "k_sw_b28": 20983, "k_sw_b29": 33101,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 11721, "k_lw_b02": 11466, "k_lw_b03": 8394, "k_lw_b04": 9162, "k_lw_b05": 9385, "k_lw_b06": 9949,
-    "k_lw_b07": 8985, "k_lw_b08": 10968, "k_lw_b09": 8015, "k_lw_b10": 9139, "k_lw_b11": 9937, "k_lw_b12": 7579,
-    "k_lw_b13": 9339, "k_lw_b14": 7643, "k_lw_b15": 8432, "k_lw_b16": 7459, "k_sw_b16": 14031, "k_sw_b17": 21535,
-    "k_sw_b18": 16398, "k_sw_b19": 16408, "k_sw_b20": 19096, "k_sw_b21": 18993, "k_sw_b22": 9232,
-    "k_sw_b23": 19052, "k_sw_b24": 17359, "k_sw_b25": 14984, "k_sw_b26": 14022, "k_sw_b27": 17354,
-    "k_sw_b28": 14976, "k_sw_b29": 21502,
+    "k_lw_b01": 18400, "k_lw_b02": 20676, "k_lw_b03": 21675, "k_lw_b04": 19948, "k_lw_b05": 22171,
+    "k_lw_b06": 12971, "k_lw_b07": 17770, "k_lw_b08": 13446, "k_lw_b09": 17296, "k_lw_b10": 12606,
+    "k_lw_b11": 15114, "k_lw_b12": 12927, "k_lw_b13": 10065, "k_lw_b14": 6773, "k_lw_b15": 7237, "k_lw_b16": 6754,
+    "k_sw_b16": 26155, "k_sw_b17": 48985, "k_sw_b18": 33602, "k_sw_b19": 33595, "k_sw_b20": 41456,
+    "k_sw_b21": 41334, "k_sw_b22": 11385, "k_sw_b23": 41417, "k_sw_b24": 34051, "k_sw_b25": 26400,
+    "k_sw_b26": 25809, "k_sw_b27": 34157, "k_sw_b28": 26692, "k_sw_b29": 48932,
 }
 
 
