@@ -322,8 +322,74 @@ template <int GS, typename R> __device__ __forceinline__ R gsum(R s)
     return s;
 }
 
+// Optics chunks and wave bounds of the LW band kernels, per band.  These kernels wait on table gathers
+// most of the time, so a second resident wave per SIMD pays wherever the registers allow it:
+// * chunk: g-points per optics chunk (0 = the whole band at once).  A scheduling barrier between
+//   chunks keeps the gathers of one chunk from being hoisted above the arithmetic of the previous
+//   one, which bounds the registers the gathered values hold.  The arithmetic of every g-point is
+//   unchanged, so the FP64 results stay bit-identical;
+// * waves: waves per SIMD the compiler must fit.
+// FP32 cached kernel (k_lwc): the two-key-species bands 3, 4, 5, 7, 9 spill 18-80 registers at two
+// waves and still run 25-30 % faster.  The other bands reach two waves by their register count
+// (bands 6, 8, 12 through the chunks); for them the bound itself made the schedule 15-20 % slower,
+// and chunking bands 1, 2, 10, 11 only lowered the loads in flight.
+// FP64 recompute kernel (k_lw<double>): two-g-point chunks bring bands 1, 2, 6, 11 to two waves
+// without spills (26-33 % faster); bands 8 and 12 gain 6-10 % from a two-wave bound with spills.
+// The other bands stay at one wave unchunked: chunked at one wave they lost half their speed (fewer
+// gathers in flight, no second wave to cover the waits).
+// WR_LWC_WAVES / WR_LW64_WAVES > 0 or WR_LW_CHUNK / WR_LW64_CHUNK >= 0 override the tables for every
+// band (tuning builds).
+constexpr int LWC_WAVES[16] = {1, 1, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 1, 1, 1, 1};
+constexpr int LWC_CHUNK[16] = {0, 0, 4, 4, 4, 4, 4, 4, 4, 0, 0, 4, 0, 0, 0, 0};
+constexpr int LW64_WAVES[16] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 2, 1, 1, 1, 1};
+constexpr int LW64_CHUNK[16] = {2, 2, 0, 0, 0, 2, 0, 4, 0, 0, 2, 2, 0, 0, 0, 0};
+#ifndef WR_LWC_WAVES
+#define WR_LWC_WAVES 0
+#endif
+#ifndef WR_LW_CHUNK
+#define WR_LW_CHUNK -1
+#endif
+#ifndef WR_LW64_WAVES
+#define WR_LW64_WAVES 0
+#endif
+#ifndef WR_LW64_CHUNK
+#define WR_LW64_CHUNK -1
+#endif
+template <int B> constexpr int lwc_waves() { return WR_LWC_WAVES > 0 ? WR_LWC_WAVES : LWC_WAVES[B]; }
+template <typename R, int B> constexpr int lw_waves()
+{
+    return sizeof(R) == 8 ? (WR_LW64_WAVES > 0 ? WR_LW64_WAVES : LW64_WAVES[B]) : WR_LW_WAVES;
+}
+// the smallest divisor of ngl that is >= want (ngl when want <= 0)
+constexpr int chunk_of(int ngl, int want)
+{
+    if (want <= 0) return ngl;
+    for (int c = want; c < ngl; ++c)
+        if (ngl % c == 0) return c;
+    return ngl;
+}
+template <int B> constexpr int lw_chunk() { return chunk_of(LW_NG[B], WR_LW_CHUNK >= 0 ? WR_LW_CHUNK : LWC_CHUNK[B]); }
+template <typename R, int B, int NGL> constexpr int lw_chunk_r()
+{
+    return sizeof(R) == 8 ? chunk_of(NGL, WR_LW64_CHUNK >= 0 ? WR_LW64_CHUNK : LW64_CHUNK[B]) : NGL;
+}
+// lw_optics of g-points g0 .. g0 + NGL - 1 in chunks of CH
+template <typename R, int B, int NGL, int CH>
+__device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0)
+{
+    if constexpr (CH >= NGL) {
+        lw_optics<R, B, NGL>(T, L, tau, fr, g0);
+    } else {
+#pragma unroll
+        for (int c = 0; c < NGL; c += CH) {
+            lw_optics<R, B, CH>(T, L, tau + c, fr + c, g0 + c);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+    }
+}
+
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, Acc acc, Surr sg)
 {
     constexpr int GS = lw_gsplit<R>();
@@ -379,7 +445,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            lw_optics<R, B, NGL>(T, L, tau, fr, g0);
+            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
@@ -436,7 +502,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            lw_optics<R, B, NGL>(T, L, tau, fr, g0);
+            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
@@ -488,35 +554,7 @@ __global__ void __launch_bounds__(64, WR_LW_WAVES) k_lw(const Tabs<R> *__restric
 // optics.  The FP64 kernels recompute the upward optics instead: their sums follow the reference's
 // operation order, and twice the bytes would not pay.
 constexpr int LW_NV_F32 = 2;
-// Per-band build of the FP32 cached LW kernel.  These kernels wait on table gathers most of the time,
-// so a second resident wave per SIMD pays wherever the registers allow it:
-// * LWC_CHUNK: g-points per optics chunk (0 = the whole band at once).  A scheduling barrier between
-//   chunks keeps the gathers of one chunk from being hoisted above the arithmetic of the previous
-//   one, which bounds the registers the gathered values hold (bands 6, 8, 12 then fit two waves);
-// * LWC_WAVES: waves per SIMD the compiler must fit.  The two-key-species bands 3, 4, 5, 7, 9 spill
-//   18-80 registers at two waves and still run 25-30 % faster.  The other bands reach two waves by
-//   their register count (bands 6, 8, 12 through the chunks); for them the bound itself made the
-//   schedule 15-20 % slower, and chunking bands 1, 2, 10, 11 only lowered the loads in flight.
-// WR_LWC_WAVES > 0 or WR_LW_CHUNK >= 0 override the tables for every band (tuning builds).
-constexpr int LWC_WAVES[16] = {1, 1, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 1, 1, 1, 1};
-constexpr int LWC_CHUNK[16] = {0, 0, 4, 4, 4, 4, 4, 4, 4, 0, 0, 4, 0, 0, 0, 0};
-#ifndef WR_LWC_WAVES
-#define WR_LWC_WAVES 0
-#endif
-#ifndef WR_LW_CHUNK
-#define WR_LW_CHUNK -1
-#endif
-template <int B> constexpr int lwc_waves() { return WR_LWC_WAVES > 0 ? WR_LWC_WAVES : LWC_WAVES[B]; }
 // the smallest divisor of the band's g-point count that is >= the chunk asked for
-template <int B> constexpr int lw_chunk()
-{
-    const int ng = LW_NG[B];
-    const int want = WR_LW_CHUNK >= 0 ? WR_LW_CHUNK : LWC_CHUNK[B];
-    if (want <= 0) return ng;
-    for (int c = want; c < ng; ++c)
-        if (ng % c == 0) return c;
-    return ng;
-}
 template <int B, bool SURR>
 __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T, const ColIn &in, const RunPar &par,
                                                int col0, int n, const ColScal &cs, const Acc &acc, const Surr &sg,
@@ -566,16 +604,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
             surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            constexpr int CH = lw_chunk<B>();
-            if constexpr (CH == NG) {
-                lw_optics<R, B>(T, L, tau, fr);
-            } else {
-#pragma unroll
-                for (int c = 0; c < NG; c += CH) {
-                    lw_optics<R, B, CH>(T, L, tau + c, fr + c, c);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-            }
+            lw_optics_ch<R, B, NG, lw_chunk<B>()>(T, L, tau, fr, 0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
