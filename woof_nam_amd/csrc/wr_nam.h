@@ -172,8 +172,33 @@ __device__ __forceinline__ void lw_tot3(const Tabs<R> *__restrict__ T, R odtot, 
 }
 
 // ---- LW band kernel of the NAM route (rtrnmr) ---------------------------------------------------
+// Per-band optics chunks (g-points, 0 = whole band) and wave bounds, by precision, as for the McICA
+// route's LW kernels: a second resident wave covers the table-gather waits where the registers allow.
+// Chosen per band from serialised timings on the 954x835x60 grid (chunks of 2 or 4, bound 1 or 2):
+// FP32 bands 1, 2, 4, 7, 8, 9, 12 run 14-29 % faster at two waves; the FP32 bands 3 and 5 and most
+// FP64 bands were fastest as they were (FP64 bands 6 and 11 gain 9 and 17 %).
+// WR_LWR_WAVES > 0 or WR_LWR_CHUNK >= 0 override the tables for every band (tuning builds).
+constexpr int LWR64_WAVES[16] = {1, 1, 1, 1, 1, 2, 1, 1, 1, 1, 2, 1, 1, 1, 1, 1};
+constexpr int LWR64_CHUNK[16] = {0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 4, 0, 0, 0, 0, 0};
+constexpr int LWR32_WAVES[16] = {2, 2, 1, 2, 1, 1, 2, 2, 2, 1, 1, 2, 1, 1, 1, 1};
+constexpr int LWR32_CHUNK[16] = {4, 4, 0, 2, 0, 0, 2, 4, 2, 0, 0, 4, 0, 0, 0, 0};
+#ifndef WR_LWR_WAVES
+#define WR_LWR_WAVES 0
+#endif
+#ifndef WR_LWR_CHUNK
+#define WR_LWR_CHUNK -1
+#endif
+template <typename R, int B> constexpr int lwr_waves()
+{
+    return WR_LWR_WAVES > 0 ? WR_LWR_WAVES : (sizeof(R) == 8 ? LWR64_WAVES[B] : LWR32_WAVES[B]);
+}
+template <typename R, int B> constexpr int lwr_chunk()
+{
+    return chunk_of(LW_NG[B], WR_LWR_CHUNK >= 0 ? WR_LWR_CHUNK : (sizeof(R) == 8 ? LWR64_CHUNK[B] : LWR32_CHUNK[B]));
+}
+
 template <typename R, int B>
-__global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                             ColScal cs, Acc acc, Rtr<R> rt)
 {
     constexpr int NG = LW_NG[B];
@@ -214,7 +239,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics<R, B>(T, L, tau, fr);
+        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, L, tau, fr, 0);
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
@@ -277,7 +302,7 @@ __global__ void __launch_bounds__(64) k_lwr(const Tabs<R> *__restrict__ T, ColIn
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics<R, B>(T, L, tau, fr);
+        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, L, tau, fr, 0);
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
         if (kk == 1) {  // surface emission and reflection
