@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 8781, "k_lw_b02": 8053, "k_lw_b03": 8482, "k_lw_b04": 9271, "k_lw_b05": 9470, "k_lw_b06": 7619,
-    "k_lw_b07": 9027, "k_lw_b08": 8589, "k_lw_b09": 8062, "k_lw_b10": 9174, "k_lw_b11": 7606, "k_lw_b12": 7636,
-    "k_lw_b13": 8148, "k_lw_b14": 7657, "k_lw_b15": 8473, "k_lw_b16": 7488, "k_sw_b16": 20027, "k_sw_b17": 37555,
-    "k_sw_b18": 24405, "k_sw_b19": 24383, "k_sw_b20": 28744, "k_sw_b21": 28768, "k_sw_b22": 11276,
-    "k_sw_b23": 28755, "k_sw_b24": 25348, "k_sw_b25": 21009, "k_sw_b26": 20043, "k_sw_b27": 25330,
-    "k_sw_b28": 20983, "k_sw_b29": 33101,
+    "k_lw_b01": 11639, "k_lw_b02": 11452, "k_lw_b03": 8490, "k_lw_b04": 9263, "k_lw_b05": 9474, "k_lw_b06": 9891,
+    "k_lw_b07": 9024, "k_lw_b08": 11958, "k_lw_b09": 8064, "k_lw_b10": 9168, "k_lw_b11": 9892, "k_lw_b12": 9899,
+    "k_lw_b13": 8149, "k_lw_b14": 7657, "k_lw_b15": 8474, "k_lw_b16": 7487, "k_sw_b16": 19969, "k_sw_b17": 33251,
+    "k_sw_b18": 24320, "k_sw_b19": 24301, "k_sw_b20": 28638, "k_sw_b21": 28659, "k_sw_b22": 11274,
+    "k_sw_b23": 28641, "k_sw_b24": 25263, "k_sw_b25": 20949, "k_sw_b26": 19980, "k_sw_b27": 25249,
+    "k_sw_b28": 20928, "k_sw_b29": 32968,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 18400, "k_lw_b02": 20676, "k_lw_b03": 21675, "k_lw_b04": 19948, "k_lw_b05": 22171,
-    "k_lw_b06": 12971, "k_lw_b07": 17770, "k_lw_b08": 13446, "k_lw_b09": 17296, "k_lw_b10": 12606,
-    "k_lw_b11": 15114, "k_lw_b12": 12927, "k_lw_b13": 10065, "k_lw_b14": 6773, "k_lw_b15": 7237, "k_lw_b16": 6754,
-    "k_sw_b16": 26155, "k_sw_b17": 48985, "k_sw_b18": 33602, "k_sw_b19": 33595, "k_sw_b20": 41456,
-    "k_sw_b21": 41334, "k_sw_b22": 11385, "k_sw_b23": 41417, "k_sw_b24": 34051, "k_sw_b25": 26400,
-    "k_sw_b26": 25809, "k_sw_b27": 34157, "k_sw_b28": 26692, "k_sw_b29": 48932,
+    "k_lw_b01": 18402, "k_lw_b02": 20678, "k_lw_b03": 30379, "k_lw_b04": 24734, "k_lw_b05": 30412,
+    "k_lw_b06": 15070, "k_lw_b07": 22457, "k_lw_b08": 15591, "k_lw_b09": 21752, "k_lw_b10": 12606,
+    "k_lw_b11": 15114, "k_lw_b12": 15155, "k_lw_b13": 10064, "k_lw_b14": 6773, "k_lw_b15": 7237, "k_lw_b16": 6754,
+    "k_sw_b16": 26085, "k_sw_b17": 48809, "k_sw_b18": 33501, "k_sw_b19": 33512, "k_sw_b20": 41337,
+    "k_sw_b21": 41188, "k_sw_b22": 11378, "k_sw_b23": 41282, "k_sw_b24": 33954, "k_sw_b25": 26349,
+    "k_sw_b26": 25763, "k_sw_b27": 34055, "k_sw_b28": 26620, "k_sw_b29": 48756,
 }
 
 

@@ -769,9 +769,7 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
     const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     uint64_t mb[NG];
-    R zsolar[NG];
     for (int g = 0; g < NG; ++g) {
-        zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
         mb[g] = mcica_g(col, seed, 1, NS + g, nrl, par.iovr, [&](int k) {
             const double ct = rc.cfrac(nrl - 1 - k);
             return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
@@ -807,10 +805,11 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
         }
     }
     // top-down: direct beam, diffuse transmission and reflection from above, level fluxes
-    R tdbt[NG], tdn[NG], rdnd[NG];
+    R tdbt[NG], tdn[NG], rdnd[NG], zsolar[NG];
     R su = (R)0, sd = (R)0;
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
+        zsolar[g] = (R)ssolar * sflux[(size_t)(NS + g) * n + t];
         tdbt[g] = (R)1, tdn[g] = (R)1, rdnd[g] = (R)0;
         const R zden1 = (R)1 / ((R)1 - rdnd[g] * pd[g]);
         const R zfu = (tdbt[g] * pb[g] + (tdn[g] - tdbt[g]) * pd[g]) * zden1;
