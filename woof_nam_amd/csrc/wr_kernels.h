@@ -276,19 +276,23 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const LayV 
     L.delp = v.pb - v.pt;
 }
 
-// ---- per-wave level cache of the SW kernels ----------------------------------------------------------
+// ---- per-wave level cache of the band kernels -------------------------------------------------------
 // The SW adding method keeps two reflectances per g-point and level from the bottom-up sweep for the
-// top-down sweep.  They live in a slot of the context's pool, not in per-lane private arrays (which
-// would need kilobytes of scratch per lane, backed by the runtime on every HIP queue): one slot per
-// resident 64-lane workgroup (the kernels loop over column blocks, so the slot count bounds the
-// memory), laid out [value][g][level][lane] so that every access is one coalesced 64-lane row.
+// top-down sweep (the FP32 kernels also keep each layer's R/T, and the FP32 LW kernel its upward
+// transmittance and source).  They live in a slot of the context's pool, not in per-lane private
+// arrays (which would need kilobytes of scratch per lane, backed by the runtime on every HIP queue):
+// one slot per resident 64-lane workgroup (the kernels loop over column blocks, so the slot count
+// bounds the memory), laid out [level][value][g][lane] so that every access is one coalesced 64-lane
+// row.  Level-major order makes the offset of every (value, g) row within a level a compile-time
+// constant; with the level innermost instead, the compiler kept one 64-bit address per row live
+// across the level loops, and the SW kernels spilled up to 150 registers more (8 % slower).
 // Entries per value and g: nk = nrl + 1 (layers 0..nrl-1, or levels 0..nrl).
 template <typename R> struct LaneCache {
     WR_G R *p;  // this lane's element 0
-    int nk;
+    int nk, nv;
     __device__ __forceinline__ R &at(int v, int ng, int g, int k) const
     {
-        return p[(((size_t)v * ng + g) * nk + k) * 64];
+        return p[(((size_t)k * nv + v) * ng + g) * 64];
     }
 };
 __host__ __device__ constexpr size_t cache_slot_bytes(int nv, int ng, int nk, int rbytes)
@@ -301,6 +305,7 @@ __device__ __forceinline__ LaneCache<R> lane_cache(char *pool, int nv, int ng, i
     LaneCache<R> c;
     c.p = (WR_G R *)(pool + (size_t)blockIdx.x * cache_slot_bytes(nv, ng, nk, (int)sizeof(R))) + threadIdx.x;
     c.nk = nk;
+    c.nv = nv;
     return c;
 }
 constexpr int SW_NV = 2;  // per level and g: zrupb, zrupd (reflectances of the medium below)
