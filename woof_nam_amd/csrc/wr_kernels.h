@@ -862,9 +862,20 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     }
 }
 
+// Waves per SIMD of the McICA-route SW kernels: WR_SW_WAVES for the FP32 units; per band in FP64,
+// where bands 16, 25, 26 and 28 fit two waves with few spills.  WR_SW64_WAVES > 0 overrides the table.
+constexpr int SW64_WAVES[14] = {2, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 1, 2, 1};
+#ifndef WR_SW64_WAVES
+#define WR_SW64_WAVES 0
+#endif
+template <typename R, int B> constexpr int sw_waves()
+{
+    return sizeof(R) == 8 ? (WR_SW64_WAVES > 0 ? WR_SW64_WAVES : SW64_WAVES[B]) : WR_SW_WAVES;
+}
+
 // SW band kernel: blocks of 64 day-list entries, a grid no larger than the pool's slot count.
 template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, WR_SW_WAVES) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                            ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg, char *__restrict__ pool)
 {
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
