@@ -18,7 +18,7 @@ struct wr_ctx {
     wr::Tabs<float> host_f;
     // tile work space
     double *acc;                     // band flux planes: 60 x [NL + 1][max_cols] + 14 x [max_cols]
-    int *iws;                        // 6 x [max_cols]
+    int *iws;                        // 7 x [max_cols]: 6 column words, then day-list segment counts
     double *dws;                     // 3 x [max_cols]
     void *rtr;                       // maximum-random overlap factors of the NAM LW route
     double *sflux_d;                 // [112][max_cols]

@@ -3,13 +3,16 @@
 // Per column tile (n columns starting at col0):
 //   k_colprep   column scalars: LW laytrop, precipitable water, cloud flag; SW day flag, column cloud
 //               cover (max-random), SW laytrop and the solar source of the 112 g-points
-//   k_daylist   ordered compaction of day columns (one workgroup, deterministic order)
+//   k_day*      ordered compaction of the day columns (count, scan, scatter; deterministic order)
 //   k_lw<B>     one thread per column, all g-points of band B in registers: McICA masks, then the
-//               rtrnmc down sweep and up sweep (optics recomputed per sweep, so nothing per layer is
-//               stored); band fluxes added level by level into the FP64 accumulators
+//               rtrnmc down sweep and up sweep (optics recomputed per sweep); the band's level fluxes
+//               go to its own FP64 planes (FP64, and FP32 with two lanes per column)
+//   k_lwc<B>    FP32: as k_lw, with each layer's upward transmittance and source kept in the pool slot
+//               by the down sweep, so that the up sweep needs no optics
 //   k_sw<B>     one thread per day column: for each g-point of band B the two-stream adding method
-//               as one bottom-up sweep (reflectance of the atmosphere below each level) and one
-//               top-down sweep (direct/diffuse transmission and level fluxes)
+//               as one bottom-up sweep (reflectance of the atmosphere below each level, to the pool
+//               slot; FP32 also the layer R/T) and one top-down sweep (direct/diffuse transmission
+//               and level fluxes)
 //   k_post      FP64 level fluxes -> heating rates, top-first outputs
 #pragma once
 #include "wr_lw.h"
@@ -167,28 +170,89 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
     }
 }
 
-// ordered compaction of the day columns of a tile (single workgroup of 1024 threads)
-static __global__ void __launch_bounds__(1024) k_daylist(const int *__restrict__ day, int n, int *__restrict__ dlist,
-                                                  int *__restrict__ nday)
+// ---- ordered compaction of the day columns of a tile --------------------------------------------
+// Three small kernels, all reads coalesced: per-segment day counts (DAY_SEG columns per 256-thread
+// block), one workgroup's exclusive scan of the counts, then each segment writes its day columns in
+// column order.  The result is the same ordered list whatever the launch shape.
+constexpr int DAY_T = 256, DAY_IT = 8, DAY_SEG = DAY_T * DAY_IT;
+
+// exclusive prefix of v over the 256 threads of the block (4 waves); total = the block sum
+__device__ __forceinline__ int block_excl_scan256(int v, int *lds, int &total)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    int x = v;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const int y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) lds[w] = x;
+    __syncthreads();
+    int before = 0;
+    total = 0;
+#pragma unroll
+    for (int q = 0; q < DAY_T / 64; ++q) {
+        const int c = lds[q];
+        before += (q < w) ? c : 0;
+        total += c;
+    }
+    __syncthreads();  // lds is reused by the next call
+    return before + x - v;
+}
+
+static __global__ void __launch_bounds__(DAY_T) k_daycount(const int *__restrict__ day, int n, int *__restrict__ seg)
+{
+    __shared__ int lds[DAY_T / 64];
+    const int j0 = blockIdx.x * DAY_SEG + threadIdx.x;
+    int c = 0;
+#pragma unroll
+    for (int it = 0; it < DAY_IT; ++it) {
+        const int j = j0 + it * DAY_T;
+        c += (j < n && day[j] != 0) ? 1 : 0;
+    }
+    int tot;
+    (void)block_excl_scan256(c, lds, tot);
+    if (threadIdx.x == 0) seg[blockIdx.x] = tot;
+}
+
+// in place: segment counts -> exclusive segment offsets; *nday = the total
+static __global__ void __launch_bounds__(1024) k_dayscan(int *__restrict__ seg, int nseg, int *__restrict__ nday)
 {
     __shared__ int part[1024];
     const int tid = threadIdx.x;
-    const int chunk = (n + 1023) / 1024;
-    const int b = tid * chunk, e = min(n, b + chunk);
-    int cnt = 0;
-    for (int j = b; j < e; ++j) cnt += day[j];
-    part[tid] = cnt;
-    __syncthreads();
-    for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
-        const int v = (tid >= off) ? part[tid - off] : 0;
+    int carry = 0;
+    for (int base = 0; base < nseg; base += 1024) {
+        const int v = (base + tid < nseg) ? seg[base + tid] : 0;
+        part[tid] = v;
         __syncthreads();
-        part[tid] += v;
+        for (int off = 1; off < 1024; off <<= 1) {  // Hillis-Steele inclusive scan
+            const int u = (tid >= off) ? part[tid - off] : 0;
+            __syncthreads();
+            part[tid] += u;
+            __syncthreads();
+        }
+        if (base + tid < nseg) seg[base + tid] = carry + part[tid] - v;
+        const int tot = part[1023];
         __syncthreads();
+        carry += tot;
     }
-    int pos = part[tid] - cnt;
-    for (int j = b; j < e; ++j)
-        if (day[j]) dlist[pos++] = j;
-    if (tid == 1023) *nday = part[1023];
+    if (tid == 0) *nday = carry;
+}
+
+static __global__ void __launch_bounds__(DAY_T) k_dayscatter(const int *__restrict__ day, int n, const int *__restrict__ seg,
+                                                         int *__restrict__ dlist)
+{
+    __shared__ int lds[DAY_T / 64];
+    const int j0 = blockIdx.x * DAY_SEG + threadIdx.x;
+    int run = seg[blockIdx.x];
+    for (int it = 0; it < DAY_IT; ++it) {
+        const int j = j0 + it * DAY_T;
+        const int f = (j < n && day[j] != 0) ? 1 : 0;
+        int tot;
+        const int r = block_excl_scan256(f, lds, tot);
+        if (f) dlist[run + r] = j;
+        run += tot;
+    }
 }
 
 // ---- LW band kernel (rtrnmc, McICA) -------------------------------------------------------------
@@ -862,8 +926,9 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     }
 }
 
-// Waves per SIMD of the McICA-route SW kernels: WR_SW_WAVES for the FP32 units; per band in FP64,
-// where bands 16, 25, 26 and 28 fit two waves with few spills.  WR_SW64_WAVES > 0 overrides the table.
+// Waves per SIMD of the SW kernels (both routes): WR_SW_WAVES for the FP32 units; per band in FP64,
+// where bands 16, 25, 26 and 28 fit two waves with few spills (McICA route 16-27 % faster each).
+// WR_SW64_WAVES > 0 overrides the table.
 constexpr int SW64_WAVES[14] = {2, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 1, 2, 1};
 #ifndef WR_SW64_WAVES
 #define WR_SW64_WAVES 0

@@ -36,6 +36,19 @@ constexpr bool band_plan_complete()
 }
 static_assert(band_plan_complete(), "every band kernel appears exactly once in BAND_PLAN");
 
+// Day list of a tile: segment counts into the words after nday (cs.nday + 1 .. + nseg; the integer
+// work space holds 2 max_cols - 1 words there, nseg <= max_cols), then their scan, then the ordered
+// scatter.
+inline void launch_daylist(const ColScal &cs, int n, hipStream_t s)
+{
+    const int nseg = (n + DAY_SEG - 1) / DAY_SEG;
+    int *seg = (int *)cs.nday + 1;
+    hipLaunchKernelGGL(k_daycount, dim3(nseg), dim3(DAY_T), 0, s, (const int *)cs.day, n, seg);
+    hipLaunchKernelGGL(k_dayscan, dim3(1), dim3(1024), 0, s, seg, nseg, (int *)cs.nday);
+    hipLaunchKernelGGL(k_dayscatter, dim3(nseg), dim3(DAY_T), 0, s, (const int *)cs.day, n, (const int *)seg,
+                       (int *)cs.dlist);
+}
+
 inline BandAcc tile_planes(wr_ctx *c, int n, int nrl, int band)
 {
     BandAcc ba;
@@ -175,7 +188,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
     hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
     tbeg(c, KID_DAYLIST, s);
-    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)j.cs.day, n, j.cs.dlist, j.cs.nday);
+    launch_daylist(j.cs, n, s);
     tend(c, s);
     int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
@@ -202,7 +215,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
     tbeg(c, KID_DAYLIST, s);
-    hipLaunchKernelGGL(k_daylist, dim3(1), dim3(1024), 0, s, (const int *)j.cs.day, n, j.cs.dlist, j.cs.nday);
+    launch_daylist(j.cs, n, s);
     tend(c, s);
     tbeg(c, KID_RTRFAC, s);
     hipLaunchKernelGGL((k_rtr_fac<R>), dim3((n + 63) / 64), dim3(64), 0, s, in, col0, n, rt);
