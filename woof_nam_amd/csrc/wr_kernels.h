@@ -30,30 +30,35 @@
 
 namespace wr {
 
-// FP64 level fluxes of ONE band for one tile ([nrl+1][n] planes, written once per level; sw_clr0 [n])
-struct Acc {
-    WR_G double *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
+// Level fluxes of ONE band for one tile ([nrl+1][n] planes, written once per level; sw_clr0 [n]), in
+// the band kernel's precision P: FP64 planes for the FP64 kernels; FP32 planes for the FP32 kernels,
+// whose band sums are FP32 values, so that nothing is lost and k_post reads half the bytes.
+template <typename P> struct AccT {
+    WR_G P *lw_up, *lw_dn, *sw_up, *sw_dn, *sw_clr0;
 };
 // All band planes of a tile: LW band b at planes 2b, 2b+1; SW band b at planes 32+2b, 33+2b; the
-// clear-sky surface SW of SW band b at base + 60 P + b n.  k_post sums the bands in band order, which
-// reproduces the sequential accumulation of the reference bit for bit.
+// clear-sky surface SW of SW band b at base + 60 P + b n (in elements of the plane type; the space is
+// sized for FP64).  k_post sums the bands in band order in FP64, which reproduces the sequential
+// accumulation of the reference bit for bit.
 struct BandAcc {
     WR_G double *base;
     long long P;  // (nrl + 1) * n
     int n;
     unsigned lw_mask, sw_mask;  // bands present (bit b)
 };
-__host__ __device__ inline Acc band_acc(const BandAcc &ba, int id)  // id 0..15 LW, 16..29 SW
+template <typename P>
+__host__ __device__ inline AccT<P> band_acc(const BandAcc &ba, int id)  // id 0..15 LW, 16..29 SW
 {
-    Acc a = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    AccT<P> a = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    WR_G P *base = (WR_G P *)ba.base;
     if (id < 16) {
-        a.lw_up = ba.base + (size_t)(2 * id) * ba.P;
-        a.lw_dn = ba.base + (size_t)(2 * id + 1) * ba.P;
+        a.lw_up = base + (size_t)(2 * id) * ba.P;
+        a.lw_dn = base + (size_t)(2 * id + 1) * ba.P;
     } else {
         const int b = id - 16;
-        a.sw_up = ba.base + (size_t)(32 + 2 * b) * ba.P;
-        a.sw_dn = ba.base + (size_t)(33 + 2 * b) * ba.P;
-        a.sw_clr0 = ba.base + (size_t)60 * ba.P + (size_t)b * ba.n;
+        a.sw_up = base + (size_t)(32 + 2 * b) * ba.P;
+        a.sw_dn = base + (size_t)(33 + 2 * b) * ba.P;
+        a.sw_clr0 = base + (size_t)60 * ba.P + (size_t)b * ba.n;
     }
     return a;
 }
@@ -459,7 +464,7 @@ __device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, cons
 
 template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, Acc acc, Surr sg)
+                                           ColScal cs, AccT<R> acc, Surr sg)
 {
     constexpr int GS = lw_gsplit<R>();
     constexpr int NG = LW_NG[B];
@@ -486,7 +491,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
     for (int jj = 0; jj < NGL; ++jj)
         mb[jj] = mcica_g(col, seed, 0, NS + g0 + jj, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
 
-    if (lead) acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
+    if (lead) acc.lw_dn[(size_t)nrl * n + t] = (R)0;  // no downward LW at the top
     // ---- down sweep, top to surface ----
     R rad[NGL];
 #pragma unroll
@@ -541,7 +546,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             s = s + rad[jj];
         }
         s = gsum<GS>(s);
-        if (lead) acc.lw_dn[(size_t)k * n + t] = (double)s;
+        if (lead) acc.lw_dn[(size_t)k * n + t] = s;
         tz_top = tz_bot;
         la_c = la_n;
         cur = nxt, nxt = nn;
@@ -585,7 +590,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
                 s0 = s0 + rad[jj];
             }
             s0 = gsum<GS>(s0);
-            if (lead) acc.lw_up[t] = (double)s0;
+            if (lead) acc.lw_up[t] = s0;
         }
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         R s = (R)0;
@@ -610,7 +615,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             s = s + rad[jj];
         }
         s = gsum<GS>(s);
-        if (lead) acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
+        if (lead) acc.lw_up[(size_t)(k + 1) * n + t] = s;
         tz_bot = tz_top;
         la_c = la_m;
         cur = nxt, nxt = nn;
@@ -626,7 +631,7 @@ constexpr int LW_NV_F32 = 2;
 // the smallest divisor of the band's g-point count that is >= the chunk asked for
 template <int B, bool SURR>
 __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T, const ColIn &in, const RunPar &par,
-                                               int col0, int n, const ColScal &cs, const Acc &acc, const Surr &sg,
+                                               int col0, int n, const ColScal &cs, const AccT<float> &acc, const Surr &sg,
                                                int t, const LaneCache<float> &cc)
 {
     using R = float;
@@ -648,7 +653,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
     for (int g = 0; g < NG; ++g)
         mb[g] = mcica_g(col, seed, 0, NS + g, nrl, par.iovr, [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
 
-    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
+    acc.lw_dn[(size_t)nrl * n + t] = (R)0;  // no downward LW at the top
     R rad[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) rad[g] = (R)0;
@@ -705,7 +710,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
             cc.at(0, NG, g, k) = a;
             cc.at(1, NG, g, k) = b;
         }
-        acc.lw_dn[(size_t)k * n + t] = (double)s;
+        acc.lw_dn[(size_t)k * n + t] = s;
         if (k == 0) {  // surface emission and reflection start the upward sweep
             const R pk0 = planck<R>(T, B, rc.tskn());
             R s0 = (R)0;
@@ -714,7 +719,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
                 rad[g] = semis * fr[g] * pk0 + reflct * rad[g];
                 s0 = s0 + rad[g];
             }
-            acc.lw_up[t] = (double)s0;
+            acc.lw_up[t] = s0;
         }
         tz_top = tz_bot;
         la_c = la_n;
@@ -727,13 +732,13 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
             rad[g] = rad[g] * cc.at(0, NG, g, k) + cc.at(1, NG, g, k);
             s = s + rad[g];
         }
-        acc.lw_up[(size_t)(k + 1) * n + t] = (double)s;
+        acc.lw_up[(size_t)(k + 1) * n + t] = s;
     }
 }
 
 template <int B, bool SURR = false>
 __global__ void __launch_bounds__(64, lwc_waves<B>()) k_lwc(const Tabs<float> *__restrict__ T, ColIn in, RunPar par,
-                                                         int col0, int n, ColScal cs, Acc acc, Surr sg,
+                                                         int col0, int n, ColScal cs, AccT<float> acc, Surr sg,
                                                          char *__restrict__ pool)
 {
     const LaneCache<float> cc = lane_cache<float>(pool, LW_NV_F32, LW_NG[B], in.nlay + 2);
@@ -818,7 +823,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
 // to the lane cache), then the top-down pass with the layer R/T recomputed and the level fluxes.
 template <typename R, int B, bool SURR>
 __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
-                                          int n, const ColScal &cs, const R *__restrict__ sflux, const Acc &acc,
+                                          int n, const ColScal &cs, const R *__restrict__ sflux, const AccT<R> &acc,
                                           const Surr &sg, int t, const LaneCache<R> &cc)
 {
     constexpr bool CACHE_RT = sw_nv_mc<R>() > SW_NV;
@@ -886,9 +891,9 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
         su = su + zsolar[g] * zfu;
         sd = sd + zsolar[g] * zfd;
     }
-    acc.sw_up[(size_t)nrl * n + t] = (double)su;
-    acc.sw_dn[(size_t)nrl * n + t] = (double)sd;
-    acc.sw_clr0[t] = 0.0;  // clear-sky surface flux: NAM route only
+    acc.sw_up[(size_t)nrl * n + t] = su;
+    acc.sw_dn[(size_t)nrl * n + t] = sd;
+    acc.sw_clr0[t] = (R)0;  // clear-sky surface flux: NAM route only
     for (int s = nrl; s >= 1; --s) {
         SwLayerRT<R, B> o;
         if constexpr (CACHE_RT) {
@@ -921,8 +926,8 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
             su = su + zsolar[g] * zfu;
             sd = sd + zsolar[g] * zfd;
         }
-        acc.sw_up[(size_t)(s - 1) * n + t] = (double)su;
-        acc.sw_dn[(size_t)(s - 1) * n + t] = (double)sd;
+        acc.sw_up[(size_t)(s - 1) * n + t] = su;
+        acc.sw_dn[(size_t)(s - 1) * n + t] = sd;
     }
 }
 
@@ -941,7 +946,7 @@ template <typename R, int B> constexpr int sw_waves()
 // SW band kernel: blocks of 64 day-list entries, a grid no larger than the pool's slot count.
 template <typename R, int B, bool SURR = false>
 __global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, const R *__restrict__ sflux, Acc acc, Surr sg, char *__restrict__ pool)
+                                           ColScal cs, const R *__restrict__ sflux, AccT<R> acc, Surr sg, char *__restrict__ pool)
 {
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
     const int nday = *cs.nday;
@@ -955,7 +960,8 @@ __global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__
 struct OutPtrs {
     WR_G double *hr_lw, *hr_sw, *flw_up, *flw_dn, *fsw_up, *fsw_dn, *fsw_dn_sfc_clr;
 };
-static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, BandAcc ba, OutPtrs out)
+template <typename P>
+__global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal cs, BandAcc ba, OutPtrs out)
 {
 #pragma clang fp contract(off)
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -965,17 +971,18 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
     const int nrl = rc.nrl, nm = in.nlay;
     const size_t nc = (size_t)in.ncol;
     const double flxfac = 0.5 * FLUXFAC;
+    const WR_G P *pl = (const WR_G P *)ba.base;
     // band sums in band order (the order the reference accumulates them)
     auto lwsum = [&](int plane0, int k) {
         double v = 0.0;
         for (int b = 0; b < 16; ++b)
-            if ((ba.lw_mask >> b) & 1) v = v + ba.base[(size_t)(2 * b + plane0) * ba.P + (size_t)k * n + t];
+            if ((ba.lw_mask >> b) & 1) v = v + (double)pl[(size_t)(2 * b + plane0) * ba.P + (size_t)k * n + t];
         return v;
     };
     auto swsum = [&](int plane0, int k) {
         double v = 0.0;
         for (int b = 0; b < 14; ++b)
-            if ((ba.sw_mask >> b) & 1) v = v + ba.base[(size_t)(32 + 2 * b + plane0) * ba.P + (size_t)k * n + t];
+            if ((ba.sw_mask >> b) & 1) v = v + (double)pl[(size_t)(32 + 2 * b + plane0) * ba.P + (size_t)k * n + t];
         return v;
     };
     const double up0 = lwsum(0, 0), dn0 = lwsum(1, 0);
@@ -1010,7 +1017,7 @@ static __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, 
         if (out.fsw_dn_sfc_clr) {
             double v = 0.0;
             for (int b = 0; b < 14; ++b)
-                if ((ba.sw_mask >> b) & 1) v = v + ba.base[(size_t)60 * ba.P + (size_t)b * n + t];
+                if ((ba.sw_mask >> b) & 1) v = v + (double)pl[(size_t)60 * ba.P + (size_t)b * n + t];
             out.fsw_dn_sfc_clr[i] = v;
         }
     } else {

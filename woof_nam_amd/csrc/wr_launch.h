@@ -92,7 +92,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
         if (id != B) return launch_band<R, SURR, B + 1>(j, id, s, pool);
-        const Acc acc = band_acc(j.ba, B);
+        const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1)
             hipLaunchKernelGGL((k_lwc<B, SURR>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
@@ -113,7 +113,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
         if (id != B) return launch_band_nam<R, B + 1>(j, id, s, pool);
-        const Acc acc = band_acc(j.ba, B);
+        const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16)
             hipLaunchKernelGGL((k_lwr<R, B>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
@@ -193,7 +193,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
     int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
-    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
+    hipLaunchKernelGGL(k_post<R>, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
     tend(c, s);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }
@@ -223,7 +223,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band_nam<R>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
-    hipLaunchKernelGGL(k_post, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
+    hipLaunchKernelGGL(k_post<R>, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
     tend(c, s);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }

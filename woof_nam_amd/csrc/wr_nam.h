@@ -199,7 +199,7 @@ template <typename R, int B> constexpr int lwr_chunk()
 
 template <typename R, int B>
 __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, Acc acc, Rtr<R> rt)
+                                            ColScal cs, AccT<R> acc, Rtr<R> rt)
 {
     constexpr int NG = LW_NG[B];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -215,7 +215,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
     const uint64_t lstu = rt.lstu[t], lstd = rt.lstd[t];
     const R eps = (R)1.0e-6;
 
-    acc.lw_dn[(size_t)nrl * n + t] = 0.0;  // no downward LW at the top
+    acc.lw_dn[(size_t)nrl * n + t] = (R)0;  // no downward LW at the top
     // ---- down sweep, top to surface ----
     R radtotd[NG], totradd[NG], clrradd[NG], rad[NG];
 #pragma unroll
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
                 s = s + radtotd[g];
             }
         }
-        acc.lw_dn[(size_t)(kk - 1) * n + t] = (double)s;
+        acc.lw_dn[(size_t)(kk - 1) * n + t] = s;
         tz_top = tz_bot;
         la_c = la_n;
         cur = nxt, nxt = nn;
@@ -316,7 +316,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
                 rad[g] = (R)0;
                 s0 = s0 + radtotu[g];
             }
-            acc.lw_up[t] = (double)s0;
+            acc.lw_up[t] = s0;
         }
         const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
         const R clfr = (R)lw_cldfrc(rc, kk);
@@ -351,7 +351,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
                 s = s + radtotu[g];
             }
         }
-        acc.lw_up[(size_t)kk * n + t] = (double)s;
+        acc.lw_up[(size_t)kk * n + t] = s;
         tz_bot = tz_top;
         la_c = la_m;
         cur = nxt, nxt = nn;
@@ -397,7 +397,7 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
 // the top-down sweep with the layer R/T recomputed.
 template <typename R, int B>
 __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
-                                           int n, const ColScal &cs, const R *__restrict__ sflux, const Acc &acc, int t,
+                                           int n, const ColScal &cs, const R *__restrict__ sflux, const AccT<R> &acc, int t,
                                            const LaneCache<R> &cc)
 {
     constexpr bool CACHE_RT = sw_nv_mc<R>() > SW_NV;  // FP32: layer R/T kept for the top-down sweep
@@ -496,13 +496,13 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
                 }
             }
             const size_t o = (size_t)s * n + t;  // level s (surface-first)
-            if (pass == 0 && s == 0) acc.sw_clr0[t] = (double)sd;
+            if (pass == 0 && s == 0) acc.sw_clr0[t] = sd;
             if (pass == 0) {
-                acc.sw_up[o] = (double)su;
-                acc.sw_dn[o] = (double)sd;
+                acc.sw_up[o] = su;
+                acc.sw_dn[o] = sd;
             } else {
-                acc.sw_up[o] = cf1 * (double)su + cf0 * acc.sw_up[o];
-                acc.sw_dn[o] = cf1 * (double)sd + cf0 * acc.sw_dn[o];
+                acc.sw_up[o] = (R)(cf1 * (double)su + cf0 * (double)acc.sw_up[o]);
+                acc.sw_dn[o] = (R)(cf1 * (double)sd + cf0 * (double)acc.sw_dn[o]);
             }
         }
     }
@@ -511,7 +511,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
 // SW band kernel of the NAM route: blocks of 64 day-list entries looped over a pool-sized grid.
 template <typename R, int B>
 __global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, const R *__restrict__ sflux, Acc acc, char *__restrict__ pool)
+                                            ColScal cs, const R *__restrict__ sflux, AccT<R> acc, char *__restrict__ pool)
 {
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
     const int nday = *cs.nday;
