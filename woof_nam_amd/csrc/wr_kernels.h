@@ -972,17 +972,25 @@ __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal
     const size_t nc = (size_t)in.ncol;
     const double flxfac = 0.5 * FLUXFAC;
     const WR_G P *pl = (const WR_G P *)ba.base;
-    // band sums in band order (the order the reference accumulates them)
+    // band sums in band order (the order the reference accumulates them).  Every plane is loaded and
+    // the sum is selected per band, not branched on, so that the loads of a level are all in flight
+    // at once; a band that is absent (single-band runs) keeps the sum unchanged, as skipping it would.
     auto lwsum = [&](int plane0, int k) {
         double v = 0.0;
-        for (int b = 0; b < 16; ++b)
-            if ((ba.lw_mask >> b) & 1) v = v + (double)pl[(size_t)(2 * b + plane0) * ba.P + (size_t)k * n + t];
+#pragma unroll
+        for (int b = 0; b < 16; ++b) {
+            const double x = (double)pl[(size_t)(2 * b + plane0) * ba.P + (size_t)k * n + t];
+            v = ((ba.lw_mask >> b) & 1) ? v + x : v;
+        }
         return v;
     };
     auto swsum = [&](int plane0, int k) {
         double v = 0.0;
-        for (int b = 0; b < 14; ++b)
-            if ((ba.sw_mask >> b) & 1) v = v + (double)pl[(size_t)(32 + 2 * b + plane0) * ba.P + (size_t)k * n + t];
+#pragma unroll
+        for (int b = 0; b < 14; ++b) {
+            const double x = (double)pl[(size_t)(32 + 2 * b + plane0) * ba.P + (size_t)k * n + t];
+            v = ((ba.sw_mask >> b) & 1) ? v + x : v;
+        }
         return v;
     };
     const double up0 = lwsum(0, 0), dn0 = lwsum(1, 0);
@@ -1016,8 +1024,11 @@ __global__ void __launch_bounds__(256) k_post(ColIn in, int col0, int n, ColScal
         }
         if (out.fsw_dn_sfc_clr) {
             double v = 0.0;
-            for (int b = 0; b < 14; ++b)
-                if ((ba.sw_mask >> b) & 1) v = v + (double)pl[(size_t)60 * ba.P + (size_t)b * n + t];
+#pragma unroll
+            for (int b = 0; b < 14; ++b) {
+                const double x = (double)pl[(size_t)60 * ba.P + (size_t)b * n + t];
+                v = ((ba.sw_mask >> b) & 1) ? v + x : v;
+            }
             out.fsw_dn_sfc_clr[i] = v;
         }
     } else {
