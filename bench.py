@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11639, "k_lw_b02": 11452, "k_lw_b03": 8490, "k_lw_b04": 9263, "k_lw_b05": 9474, "k_lw_b06": 9891,
-    "k_lw_b07": 9024, "k_lw_b08": 11958, "k_lw_b09": 8064, "k_lw_b10": 9168, "k_lw_b11": 9892, "k_lw_b12": 9899,
-    "k_lw_b13": 8149, "k_lw_b14": 7657, "k_lw_b15": 8474, "k_lw_b16": 7487, "k_sw_b16": 19969, "k_sw_b17": 33251,
-    "k_sw_b18": 24320, "k_sw_b19": 24301, "k_sw_b20": 28638, "k_sw_b21": 28659, "k_sw_b22": 11274,
-    "k_sw_b23": 28641, "k_sw_b24": 25263, "k_sw_b25": 20949, "k_sw_b26": 19980, "k_sw_b27": 25249,
-    "k_sw_b28": 20928, "k_sw_b29": 32968,
+    "k_lw_b01": 11639, "k_lw_b02": 11446, "k_lw_b03": 8494, "k_lw_b04": 9283, "k_lw_b05": 9461, "k_lw_b06": 9893,
+    "k_lw_b07": 9016, "k_lw_b08": 11902, "k_lw_b09": 8053, "k_lw_b10": 9169, "k_lw_b11": 9893, "k_lw_b12": 9888,
+    "k_lw_b13": 8149, "k_lw_b14": 7657, "k_lw_b15": 8475, "k_lw_b16": 7486, "k_sw_b16": 20333, "k_sw_b17": 32952,
+    "k_sw_b18": 24322, "k_sw_b19": 24301, "k_sw_b20": 28632, "k_sw_b21": 28651, "k_sw_b22": 11271,
+    "k_sw_b23": 28638, "k_sw_b24": 25261, "k_sw_b25": 21049, "k_sw_b26": 19991, "k_sw_b27": 25248,
+    "k_sw_b28": 21240, "k_sw_b29": 32948,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 18402, "k_lw_b02": 20678, "k_lw_b03": 30379, "k_lw_b04": 24734, "k_lw_b05": 30412,
-    "k_lw_b06": 15070, "k_lw_b07": 22457, "k_lw_b08": 15591, "k_lw_b09": 21752, "k_lw_b10": 12606,
-    "k_lw_b11": 15114, "k_lw_b12": 15155, "k_lw_b13": 10064, "k_lw_b14": 6773, "k_lw_b15": 7237, "k_lw_b16": 6754,
-    "k_sw_b16": 26085, "k_sw_b17": 48809, "k_sw_b18": 33501, "k_sw_b19": 33512, "k_sw_b20": 41337,
-    "k_sw_b21": 41188, "k_sw_b22": 11378, "k_sw_b23": 41282, "k_sw_b24": 33954, "k_sw_b25": 26349,
-    "k_sw_b26": 25763, "k_sw_b27": 34055, "k_sw_b28": 26620, "k_sw_b29": 48756,
+    "k_lw_b01": 17891, "k_lw_b02": 20151, "k_lw_b03": 29033, "k_lw_b04": 23872, "k_lw_b05": 29033,
+    "k_lw_b06": 14563, "k_lw_b07": 21917, "k_lw_b08": 15076, "k_lw_b09": 21062, "k_lw_b10": 12105,
+    "k_lw_b11": 14622, "k_lw_b12": 14654, "k_lw_b13": 9584, "k_lw_b14": 6273, "k_lw_b15": 6736, "k_lw_b16": 6253,
+    "k_sw_b16": 25293, "k_sw_b17": 47719, "k_sw_b18": 32517, "k_sw_b19": 32511, "k_sw_b20": 40253,
+    "k_sw_b21": 40126, "k_sw_b22": 10872, "k_sw_b23": 40227, "k_sw_b24": 32944, "k_sw_b25": 25746,
+    "k_sw_b26": 25257, "k_sw_b27": 33021, "k_sw_b28": 25770, "k_sw_b29": 47660,
 }
 
 
