@@ -539,8 +539,8 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
             rc = WR_ERR_NOMEM;
     }
     if (rc == WR_OK) {
-        // SW level-cache pool: room for every 64-column block of a tile in the largest layout
-        // (2 values x 16 g x 65 levels, FP64), at most WR_POOL_GB (default 6) GB per stream
+        // level pool: room for every 64-column block of a tile in the largest slot layout (FP64 SW:
+        // 2 values, FP32 SW: 7 values, x 16 g x levels), at most WR_POOL_GB (default 6) GB per stream
         const size_t lane_max = std::max(cache_slot_bytes(SW_NV, 16, c->nl_max + 1, (int)sizeof(double)),
                                          cache_slot_bytes(sw_nv_mc<float>(), 16, c->nl_max + 1, (int)sizeof(float)));
         const size_t need = (m + 63) / 64 * lane_max;
