@@ -38,13 +38,11 @@ def hipcc() -> str:
 
 
 # The mixed-precision and surrogate units contract FP32 multiply-adds and use the hardware FP32
-# reciprocal/sqrt; the FP64 index arithmetic inside them keeps contraction off by pragma.  Their SW
-# band kernels are compiled for two waves per SIMD (a few hundred bytes of register spills per lane
-# at most; 3-12 % faster than one wave on the 954x835x60 grid); LW and the FP64 unit for one (no
-# spills: two LW waves spill up to 500 bytes per lane and ran 13 % slower).  WR_LW_GSPLIT=2 (two
-# lanes per column in LW, half of the g-points each) halves the LW registers but measured 9 %
-# slower: the per-layer column work then runs twice.
-_FP32_UNIT = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt", "-DWR_SW_WAVES=2"]
+# reciprocal/sqrt; the FP64 index arithmetic inside them keeps contraction off by pragma.  The waves
+# per SIMD of every band kernel come from per-band tables in the sources (wr_kernels.h, wr_nam.h).
+# WR_LW_GSPLIT=2 (two lanes per column in LW, half of the g-points each) halves the LW registers but
+# measured 9 % slower: the per-layer column work then runs twice.
+_FP32_UNIT = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 UNIT_FLAGS = {
     "wr_mixed.hip": _FP32_UNIT,
     "wr_surr.hip": _FP32_UNIT,

@@ -18,15 +18,12 @@
 #include "wr_lw.h"
 #include "wr_sw.h"
 
-// Default minimum waves per SIMD of the LW and SW band kernels (register budget 512 / waves).  One
+// Default minimum waves per SIMD of the LW band kernels (register budget 512 / waves).  One
 // keeps everything in registers (AGPRs included), with no scratch (private segment) that the runtime
 // must back on every HIP queue running the kernel.  The per-band tables below raise it to two where
 // that measured faster; the spills this costs stay at or below about 1.1 KB of scratch per lane.
 #ifndef WR_LW_WAVES
 #define WR_LW_WAVES 1
-#endif
-#ifndef WR_SW_WAVES
-#define WR_SW_WAVES 1
 #endif
 
 namespace wr {
@@ -932,16 +929,26 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     }
 }
 
-// Waves per SIMD of the SW kernels (both routes): WR_SW_WAVES for the FP32 units; per band in FP64,
-// where bands 16, 25, 26 and 28 fit two waves with few spills (McICA route 16-27 % faster each).
-// WR_SW64_WAVES > 0 overrides the table.
+// Waves per SIMD the SW kernels are compiled for, per band, measured serialised on the 954x835x60
+// grid:
+// * FP64 (both routes): bands 16, 25, 26, 28 fit two waves with few spills (16-27 % faster each);
+// * FP32 McICA route: two waves for the bands that spill at one; bands 16, 18, 19, 22, 24, 25, 26 fit
+//   two waves by their register count, and the bound only slowed their schedule (8-14 %);
+// * FP32 NAM route: two waves for every band but 22 (one wave 25-45 % slower).
+// WR_SW64_WAVES / WR_SW32_WAVES > 0 override the tables for every band (tuning builds).
 constexpr int SW64_WAVES[14] = {2, 1, 1, 1, 1, 1, 1, 1, 1, 2, 2, 1, 2, 1};
+constexpr int SW32_WAVES[14] = {1, 2, 1, 1, 2, 2, 1, 2, 1, 1, 1, 2, 2, 2};
+constexpr int SWR32_WAVES[14] = {2, 2, 2, 2, 2, 2, 1, 2, 2, 2, 2, 2, 2, 2};
 #ifndef WR_SW64_WAVES
 #define WR_SW64_WAVES 0
 #endif
-template <typename R, int B> constexpr int sw_waves()
+#ifndef WR_SW32_WAVES
+#define WR_SW32_WAVES 0
+#endif
+template <typename R, int B, bool NAM = false> constexpr int sw_waves()
 {
-    return sizeof(R) == 8 ? (WR_SW64_WAVES > 0 ? WR_SW64_WAVES : SW64_WAVES[B]) : WR_SW_WAVES;
+    if (sizeof(R) == 8) return WR_SW64_WAVES > 0 ? WR_SW64_WAVES : SW64_WAVES[B];
+    return WR_SW32_WAVES > 0 ? WR_SW32_WAVES : (NAM ? SWR32_WAVES[B] : SW32_WAVES[B]);
 }
 
 // SW band kernel: blocks of 64 day-list entries, a grid no larger than the pool's slot count.

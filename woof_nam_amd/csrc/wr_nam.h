@@ -510,7 +510,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
 
 // SW band kernel of the NAM route: blocks of 64 day-list entries looped over a pool-sized grid.
 template <typename R, int B>
-__global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+__global__ void __launch_bounds__(64, (sw_waves<R, B, true>())) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
                                             ColScal cs, const R *__restrict__ sflux, AccT<R> acc, char *__restrict__ pool)
 {
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
