@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11639, "k_lw_b02": 11446, "k_lw_b03": 8494, "k_lw_b04": 9283, "k_lw_b05": 9461, "k_lw_b06": 9893,
-    "k_lw_b07": 9016, "k_lw_b08": 11902, "k_lw_b09": 8053, "k_lw_b10": 9169, "k_lw_b11": 9893, "k_lw_b12": 9888,
-    "k_lw_b13": 8149, "k_lw_b14": 7657, "k_lw_b15": 8475, "k_lw_b16": 7486, "k_sw_b16": 20333, "k_sw_b17": 32952,
-    "k_sw_b18": 24322, "k_sw_b19": 24301, "k_sw_b20": 28632, "k_sw_b21": 28651, "k_sw_b22": 11271,
-    "k_sw_b23": 28638, "k_sw_b24": 25261, "k_sw_b25": 21049, "k_sw_b26": 19991, "k_sw_b27": 25248,
-    "k_sw_b28": 21240, "k_sw_b29": 32948,
+    "k_lw_b01": 11652, "k_lw_b02": 11455, "k_lw_b03": 8493, "k_lw_b04": 9276, "k_lw_b05": 9468, "k_lw_b06": 9890,
+    "k_lw_b07": 9026, "k_lw_b08": 11909, "k_lw_b09": 8061, "k_lw_b10": 9168, "k_lw_b11": 9890, "k_lw_b12": 9884,
+    "k_lw_b13": 8147, "k_lw_b14": 7657, "k_lw_b15": 8474, "k_lw_b16": 7484, "k_sw_b16": 20331, "k_sw_b17": 32950,
+    "k_sw_b18": 24319, "k_sw_b19": 24301, "k_sw_b20": 28633, "k_sw_b21": 28650, "k_sw_b22": 11272,
+    "k_sw_b23": 28638, "k_sw_b24": 25259, "k_sw_b25": 21051, "k_sw_b26": 19993, "k_sw_b27": 25247,
+    "k_sw_b28": 21241, "k_sw_b29": 32947,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 17891, "k_lw_b02": 20151, "k_lw_b03": 29033, "k_lw_b04": 23872, "k_lw_b05": 29033,
-    "k_lw_b06": 14563, "k_lw_b07": 21917, "k_lw_b08": 15076, "k_lw_b09": 21062, "k_lw_b10": 12105,
-    "k_lw_b11": 14622, "k_lw_b12": 14654, "k_lw_b13": 9584, "k_lw_b14": 6273, "k_lw_b15": 6736, "k_lw_b16": 6253,
-    "k_sw_b16": 25293, "k_sw_b17": 47719, "k_sw_b18": 32517, "k_sw_b19": 32511, "k_sw_b20": 40253,
-    "k_sw_b21": 40126, "k_sw_b22": 10872, "k_sw_b23": 40227, "k_sw_b24": 32944, "k_sw_b25": 25746,
-    "k_sw_b26": 25257, "k_sw_b27": 33021, "k_sw_b28": 25770, "k_sw_b29": 47660,
+    "k_lw_b01": 17877, "k_lw_b02": 20139, "k_lw_b03": 29015, "k_lw_b04": 23863, "k_lw_b05": 28926,
+    "k_lw_b06": 14550, "k_lw_b07": 21902, "k_lw_b08": 15062, "k_lw_b09": 21050, "k_lw_b10": 12105,
+    "k_lw_b11": 14606, "k_lw_b12": 14646, "k_lw_b13": 9583, "k_lw_b14": 6273, "k_lw_b15": 6735, "k_lw_b16": 6253,
+    "k_sw_b16": 25304, "k_sw_b17": 47716, "k_sw_b18": 32512, "k_sw_b19": 32505, "k_sw_b20": 40252,
+    "k_sw_b21": 40123, "k_sw_b22": 10874, "k_sw_b23": 40234, "k_sw_b24": 32953, "k_sw_b25": 25786,
+    "k_sw_b26": 25287, "k_sw_b27": 33023, "k_sw_b28": 25769, "k_sw_b29": 47657,
 }
 
 
@@ -93,7 +93,10 @@ def served_block(d, dev, ncol, seconds):
     kinds = {"parent": (d, pid), "nest3km": dev_grid(333, 333, 11, 10_000_000),
              "nest6km": dev_grid(200, 200, 12, 20_000_000), "member": (d, pid)}
     t0 = time.perf_counter()
-    tune = autotune(0, tiles=(65536, 131072, 262144), slots=(2, 4), ncol=786432, precision="mixed", nreq=12)
+    # 524,288-column tiles only with 2 slots: 4 such slots beside this run's whole-grid context and the
+    # requests' outputs could pass the 288 GB of HBM
+    tune = autotune(0, ncol=786432, precision="mixed", nreq=12,
+                    shapes=((65536, 4), (131072, 2), (131072, 4), (262144, 2), (262144, 4), (524288, 2)))
     best = tune["best"]
     print(f"# autotune {time.perf_counter() - t0:.1f} s: tile {best['tile']} slots {best['slots']}",
           file=sys.stderr, flush=True)

@@ -256,16 +256,19 @@ def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed:
 
 
 def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), ncol: int = 262144,
-             precision: str = "mixed", cache: str | None = None, nreq: int = 4) -> dict:
+             precision: str = "mixed", cache: str | None = None, nreq: int = 4, shapes=None) -> dict:
     """Pick the tile size and slot count with the highest columns/s on a fixed batch of requests.
 
-    The batch is `ncol` generated columns submitted at once as `nreq` requests of uneven size
-    (a queue deep enough to keep every slot busy, as under load).  Results are cached in `cache`
-    (JSON) keyed by device name, precision and batch when a path is given.
+    The candidates are every (tile, slots) of `tiles` x `slots`, or the explicit list `shapes` of
+    (tile, slots) pairs when given (to leave out shapes whose work space would not fit beside the
+    caller's).  The batch is `ncol` generated columns submitted at once as `nreq` requests of uneven
+    size (a queue deep enough to keep every slot busy, as under load).  Results are cached in `cache`
+    (JSON) keyed by device name, precision, batch and candidates when a path is given.
     """
     from . import gen
     name = torch.cuda.get_device_name(device)
-    key = f"{name}|{precision}|{ncol}|{nreq}"
+    cand = [(int(t), int(ns)) for t, ns in shapes] if shapes is not None else [(t, ns) for t in tiles for ns in slots]
+    key = f"{name}|{precision}|{ncol}|{nreq}|{cand}"
     if cache and os.path.exists(cache):
         with open(cache) as f:
             db = json.load(f)
@@ -276,25 +279,24 @@ def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), nco
     w = np.array([1 + (3 * j) % 5 for j in range(nreq)], dtype=np.float64)   # uneven request sizes
     cuts = [0] + [int(x) for x in np.round(np.cumsum(w) / w.sum() * ncol)]
     results = []
-    for tile in tiles:
-        for ns in slots:
-            srv = Server(device, tile=tile, slots=ns, precision=precision,
-                         filler={k: v[..., :tile] if v.shape[-1] >= tile else v[..., :1].expand(*v.shape[:-1], tile)
-                                 for k, v in dev.items()})
-            best = None
-            for rep in range(2):
-                reqs = [Request(rid=j, kind="tune", cols={k: v[..., cuts[j]:cuts[j + 1]].contiguous() for k, v in dev.items()},
-                                seed=j) for j in range(nreq)]
-                torch.cuda.synchronize(device)
-                t0 = time.perf_counter()
-                for r in reqs:
-                    srv.submit(r)
-                srv.drain()
-                torch.cuda.synchronize(device)
-                dt = time.perf_counter() - t0
-                best = dt if best is None else min(best, dt)
-            srv.close()
-            results.append({"tile": tile, "slots": ns, "columns_per_s": ncol / best})
+    for tile, ns in cand:
+        srv = Server(device, tile=tile, slots=ns, precision=precision,
+                     filler={k: v[..., :tile] if v.shape[-1] >= tile else v[..., :1].expand(*v.shape[:-1], tile)
+                             for k, v in dev.items()})
+        best = None
+        for rep in range(2):
+            reqs = [Request(rid=j, kind="tune", cols={k: v[..., cuts[j]:cuts[j + 1]].contiguous() for k, v in dev.items()},
+                            seed=j) for j in range(nreq)]
+            torch.cuda.synchronize(device)
+            t0 = time.perf_counter()
+            for r in reqs:
+                srv.submit(r)
+            srv.drain()
+            torch.cuda.synchronize(device)
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        srv.close()
+        results.append({"tile": tile, "slots": ns, "columns_per_s": ncol / best})
     top = max(results, key=lambda r: r["columns_per_s"])
     out = {"device": name, "precision": precision, "best": top, "all": results}
     if cache:
