@@ -353,10 +353,10 @@ template <typename R> __device__ __forceinline__ void lw_layer_cloud(const LayV 
 // row.  Level-major order makes the offset of every (value, g) row within a level a compile-time
 // constant; with the level innermost instead, the compiler kept one 64-bit address per row live
 // across the level loops, and the SW kernels spilled up to 150 registers more (8 % slower).
-// Entries per value and g: nk = nrl + 1 (layers 0..nrl-1, or levels 0..nrl).
+// A slot holds nk = nlay + 2 levels (layers 0..nrl-1, or levels 0..nrl) of nv values per g-point.
 template <typename R> struct LaneCache {
     WR_G R *p;  // this lane's element 0
-    int nk, nv;
+    int nv;
     __device__ __forceinline__ R &at(int v, int ng, int g, int k) const
     {
         return p[(((size_t)k * nv + v) * ng + g) * 64];
@@ -371,7 +371,6 @@ __device__ __forceinline__ LaneCache<R> lane_cache(char *pool, int nv, int ng, i
 {
     LaneCache<R> c;
     c.p = (WR_G R *)(pool + (size_t)blockIdx.x * cache_slot_bytes(nv, ng, nk, (int)sizeof(R))) + threadIdx.x;
-    c.nk = nk;
     c.nv = nv;
     return c;
 }
@@ -380,9 +379,10 @@ constexpr int SW_NV = 2;  // per level and g: zrupb, zrupd (reflectances of the 
 // of recomputing optics and two-stream there (3 % faster); in FP64 the extra HBM traffic outweighs it.
 template <typename R> constexpr int sw_nv_mc() { return sizeof(R) == 4 ? SW_NV + 5 : SW_NV; }
 
-// Lanes per column of the LW band kernels: two in the FP32 instantiations (each lane takes half of
-// the band's g-points, which halves the registers a lane holds and doubles the waves in flight; the
-// two partial g-sums meet through a lane swap), one in FP64 (sums over g in the reference's order).
+// Lanes per column of the LW band kernels: one by default.  WR_LW_GSPLIT=2 gives the FP32
+// instantiations two (each lane takes half of the band's g-points, which halves the registers a lane
+// holds; the two partial g-sums meet through a lane swap), but the per-layer column work then runs
+// twice and it measured 9 % slower.  FP64 always uses one (sums over g in the reference's order).
 #ifndef WR_LW_GSPLIT
 #define WR_LW_GSPLIT 1
 #endif

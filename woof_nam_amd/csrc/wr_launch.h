@@ -2,9 +2,9 @@
 //
 // A tile runs the column kernels on the caller's stream, then the 30 band kernels (16 LW, 14 SW),
 // then the FP64 post.  Every band kernel writes its own flux planes (BandAcc), so the band kernels
-// are independent of one another: outside graph capture they are spread over the context's
-// auxiliary streams (fork and join through events), which fills the CUs that a finishing band
-// kernel leaves idle with the next band's waves.  k_post adds the planes in band order, so the
+// are independent of one another: they are spread over the context's auxiliary streams (fork and
+// join through events, also under graph capture once the streams' pools exist), which fills the CUs
+// that a finishing band kernel leaves idle with the next band's waves.  k_post adds the planes in band order, so the
 // result does not depend on which stream ran which band.
 #pragma once
 #include "wr_internal.h"
@@ -13,8 +13,10 @@
 namespace wr {
 
 // Band kernel id: 0..15 = LW band 1..16, 16..29 = SW band 16..29.
-// Static longest-processing-time plan of the 30 band kernels over WR_NAUX streams (measured
-// mixed-path kernel times on a 262,144-column tile; -1 ends a row).
+// Static longest-processing-time plan of the 30 band kernels over WR_NAUX streams (from mixed-path
+// kernel times measured on 262,144-column tiles; -1 ends a row).  A single band kernel fills the CUs,
+// so the streams only fill the tails of kernels: 2 % (FP64) and 5 % (mixed) against one stream on the
+// whole grid, and the plan's balance matters little.
 constexpr int BAND_PLAN[WR_NAUX][9] = {
     {2, 17, 11, 20, 19, 27, 15, 22, -1},
     {4, 0, 7, 23, 24, 16, 25, -1, -1},
