@@ -404,14 +404,15 @@ template <int GS, typename R> __device__ __forceinline__ R gsum(R s)
 // FP32 cached kernel (k_lwc): the two-key-species bands 3, 4, 5, 7, 9 spill 18-80 registers at two
 // waves and still run 25-30 % faster.  The other bands reach two waves by their register count
 // (bands 6, 8, 12 through the chunks); for them the bound itself made the schedule 15-20 % slower,
-// and chunking bands 1, 2, 10, 11 only lowered the loads in flight.
+// and chunking bands 1, 2, 10, 11 only lowered the loads in flight.  The two-g-point bands 15 and 16
+// run 11-19 % faster at a three-wave bound (6-10 spilled registers).
 // FP64 recompute kernel (k_lw<double>): two-g-point chunks bring bands 1, 2, 6, 11 to two waves
 // without spills (26-33 % faster); bands 8 and 12 gain 6-10 % from a two-wave bound with spills.
 // The other bands stay at one wave unchunked: chunked at one wave they lost half their speed (fewer
 // gathers in flight, no second wave to cover the waits).
 // WR_LWC_WAVES / WR_LW64_WAVES > 0 or WR_LW_CHUNK / WR_LW64_CHUNK >= 0 override the tables for every
 // band (tuning builds).
-constexpr int LWC_WAVES[16] = {1, 1, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 1, 1, 1, 1};
+constexpr int LWC_WAVES[16] = {1, 1, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 1, 1, 3, 3};
 constexpr int LWC_CHUNK[16] = {0, 0, 4, 4, 4, 4, 4, 4, 4, 0, 0, 4, 0, 0, 0, 0};
 constexpr int LW64_WAVES[16] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 2, 1, 1, 1, 1};
 constexpr int LW64_CHUNK[16] = {2, 2, 0, 0, 0, 2, 0, 4, 0, 0, 2, 2, 0, 0, 0, 0};

@@ -175,12 +175,13 @@ __device__ __forceinline__ void lw_tot3(const Tabs<R> *__restrict__ T, R odtot, 
 // Per-band optics chunks (g-points, 0 = whole band) and wave bounds, by precision, as for the McICA
 // route's LW kernels: a second resident wave covers the table-gather waits where the registers allow.
 // Chosen per band from serialised timings on the 954x835x60 grid (chunks of 2 or 4, bound 1 or 2):
-// FP32 bands 1, 2, 4, 7, 8, 9, 12 run 14-29 % faster at two waves; the FP32 bands 3 and 5 and most
-// FP64 bands were fastest as they were (FP64 bands 6 and 11 gain 9 and 17 %).
+// FP32 bands 1, 2, 4, 7, 8, 9, 12 run 14-29 % faster at two waves, band 16 7 % faster at three; the
+// FP32 bands 3 and 5 and most FP64 bands were fastest as they were (FP64 bands 6 and 11 gain 9 and
+// 17 %).
 // WR_LWR_WAVES > 0 or WR_LWR_CHUNK >= 0 override the tables for every band (tuning builds).
 constexpr int LWR64_WAVES[16] = {1, 1, 1, 1, 1, 2, 1, 1, 1, 1, 2, 1, 1, 1, 1, 1};
 constexpr int LWR64_CHUNK[16] = {0, 0, 0, 0, 0, 2, 0, 0, 0, 0, 4, 0, 0, 0, 0, 0};
-constexpr int LWR32_WAVES[16] = {2, 2, 1, 2, 1, 1, 2, 2, 2, 1, 1, 2, 1, 1, 1, 1};
+constexpr int LWR32_WAVES[16] = {2, 2, 1, 2, 1, 1, 2, 2, 2, 1, 1, 2, 1, 1, 1, 3};
 constexpr int LWR32_CHUNK[16] = {4, 4, 0, 2, 0, 0, 2, 4, 2, 0, 0, 4, 0, 0, 0, 0};
 #ifndef WR_LWR_WAVES
 #define WR_LWR_WAVES 0
