@@ -532,7 +532,7 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     if (rc == WR_OK) {
         if (hipMalloc((void **)&c->acc, sizeof(double) * (60 * (NL + 1) + 14) * m) != hipSuccess ||
             hipMalloc((void **)&c->iws, sizeof(int) * 7 * m) != hipSuccess ||  // 6 column words + day-list segments
-            hipMalloc((void **)&c->dws, sizeof(double) * 3 * m) != hipSuccess ||
+            hipMalloc((void **)&c->dws, sizeof(double) * (4 + 2 * (size_t)NL) * m) != hipSuccess ||
             hipMalloc(&c->rtr, (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_d, sizeof(double) * 112 * m) != hipSuccess ||
             hipMalloc((void **)&c->sflux_f, sizeof(float) * 112 * m) != hipSuccess)

@@ -19,7 +19,7 @@ struct wr_ctx {
     // tile work space
     double *acc;                     // band flux planes: 60 x [NL + 1][max_cols] + 14 x [max_cols]
     int *iws;                        // 7 x [max_cols]: 6 column words, then day-list segment counts
-    double *dws;                     // 3 x [max_cols]
+    double *dws;                     // (4 + 2 NL) x [max_cols]: 3 column words, layer logs, level temperatures
     void *rtr;                       // maximum-random overlap factors of the NAM LW route
     double *sflux_d;                 // [112][max_cols]
     float *sflux_f;
@@ -76,6 +76,8 @@ inline ColScal colscal(wr_ctx *c)
     s.pwv = gptr(c->dws + 0 * m);
     s.zcf1 = gptr(c->dws + 1 * m);
     s.zcf0 = gptr(c->dws + 2 * m);
+    s.la = gptr(c->dws + 3 * m);                 // [nrl][n], nrl <= NL
+    s.tz = gptr(c->dws + (3 + (size_t)NL) * m);  // [nrl + 1][n]
     return s;
 }
 }  // namespace wr

@@ -63,6 +63,10 @@ __host__ __device__ inline AccT<P> band_acc(const BandAcc &ba, int id)  // id 0.
 struct ColScal {
     WR_G int *lw_laytrop, *lcf1, *day, *sw_laytrop, *dlist, *nday;
     WR_G double *pwv, *zcf1, *zcf0;
+    // per tile column, written by k_colprep: la[kin * n + t] = log of the layer pressure (layer kin,
+    // top-first); tz[L * n + t] = LW level temperature, L = 0 the top layer's temperature, L = kin + 1
+    // the interface below layer kin, L = nrl the skin temperature
+    WR_G double *la, *tz;
 };
 
 template <typename R>
@@ -95,6 +99,7 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
         tem1 = tem1 + cdry + ca0;
         tem2 = tem2 + ca0;
         const double la = log_d(rc.plyr(kin));
+        cs.la[(size_t)kin * n + t] = la;
         if (la > 4.56) {
             lt += 1;
             swlt = k + 1;
@@ -112,6 +117,12 @@ __global__ void __launch_bounds__(256) k_colprep(const Tabs<R> *__restrict__ T, 
     }
     cs.lw_laytrop[t] = lt;
     cs.lcf1[t] = lcf;
+    // LW level temperatures, the operations the band kernels' sweeps used to repeat per band
+    cs.tz[t] = rc.tlyr(0);
+    for (int kin = 0; kin + 1 < nrl; ++kin)
+        cs.tz[(size_t)(kin + 1) * n + t] = tlvl_v(rc.tlyr(kin + 1), rc.tlyr(kin), log_d(rc.plvl(kin + 1)),
+                                                  cs.la[(size_t)(kin + 1) * n + t], cs.la[(size_t)kin * n + t]);
+    cs.tz[(size_t)nrl * n + t] = rc.tskn();
     const double tem0p = 10.0 * tem2 / (AMDW * tem1 * CON_G);
     cs.pwv[t] = tem0p * rc.plvl(nrl);
     const double cz = in.cosz[i];
@@ -498,18 +509,26 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
     // layer inputs: cur = layer kin, nxt = kin + 1 (loaded an iteration ahead), nn = kin + 2
     LayV cur = rc.lay(0), nxt = cur;
     if (nrl > 1) nxt = rc.lay(1);
-    double tz_top = cur.t;
-    double la_c = log_d(cur.pl);
+    double tz_top = cs.tz[t], la_nx = cs.la[t], tz_nx = 0.0;
+    if constexpr (sizeof(R) != 8) tz_nx = cs.tz[(size_t)n + t];
     for (int kin = 0; kin < nrl; ++kin) {
         const int k = nrl - 1 - kin;
         LayV nn = nxt;
         if (kin + 2 < nrl) nn = rc.lay(kin + 2);
-        double tz_bot, la_n = 0.0;
-        if (kin + 1 < nrl) {
-            la_n = log_d(nxt.pl);
-            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
+        // this layer's log pressure and bottom level temperature: FP32 kernels load k_colprep's values an
+        // iteration ahead; FP64 kernels evaluate them here by the same operations (measured faster there)
+        double la_c, tz_bot;
+        if constexpr (sizeof(R) == 8) {
+            la_c = la_nx;
+            if (kin + 1 < nrl) {
+                la_nx = log_d(nxt.pl);
+                tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_nx, la_c);
+            } else {
+                tz_bot = rc.tskn();
+            }
         } else {
-            tz_bot = rc.tskn();
+            la_c = la_nx, tz_bot = tz_nx;
+            if (kin + 1 < nrl) la_nx = cs.la[(size_t)(kin + 1) * n + t], tz_nx = cs.tz[(size_t)(kin + 2) * n + t];
         }
         LwLay<R> L;
         R tau[NGL], fr[NGL];
@@ -547,26 +566,33 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
         s = gsum<GS>(s);
         if (lead) acc.lw_dn[(size_t)k * n + t] = s;
         tz_top = tz_bot;
-        la_c = la_n;
         cur = nxt, nxt = nn;
     }
     // ---- up sweep, surface to top ----
     const R reflct = (R)1 - semis;
-    double tz_bot = rc.tskn();
+    double tz_bot = cs.tz[(size_t)nrl * n + t];
     // layer inputs: cur = layer kin, nxt = kin - 1 (loaded an iteration ahead), nn = kin - 2
     cur = rc.lay(nrl - 1), nxt = cur;
     if (nrl > 1) nxt = rc.lay(nrl - 2);
-    la_c = log_d(cur.pl);
+    if constexpr (sizeof(R) == 8)
+        la_nx = log_d(cur.pl);
+    else
+        la_nx = cs.la[(size_t)(nrl - 1) * n + t], tz_nx = cs.tz[(size_t)(nrl - 1) * n + t];
     for (int kin = nrl - 1; kin >= 0; --kin) {
         const int k = nrl - 1 - kin;
         LayV nn = nxt;
         if (kin >= 2) nn = rc.lay(kin - 2);
-        double la_m = 0.0;
-        if (kin > 0) {
-            la_m = log_d(nxt.pl);
-            tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_m);
+        const double la_c = la_nx;
+        if constexpr (sizeof(R) == 8) {
+            if (kin > 0) {
+                la_nx = log_d(nxt.pl);
+                tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_nx);
+            } else {
+                tz_top = cur.t;
+            }
         } else {
-            tz_top = cur.t;
+            tz_top = tz_nx;
+            if (kin > 0) la_nx = cs.la[(size_t)(kin - 1) * n + t], tz_nx = cs.tz[(size_t)(kin - 1) * n + t];
         }
         LwLay<R> L;
         R tau[NGL], fr[NGL];
@@ -616,7 +642,6 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
         s = gsum<GS>(s);
         if (lead) acc.lw_up[(size_t)(k + 1) * n + t] = s;
         tz_bot = tz_top;
-        la_c = la_m;
         cur = nxt, nxt = nn;
     }
 }
@@ -656,20 +681,14 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
     R rad[NG];
 #pragma unroll
     for (int g = 0; g < NG; ++g) rad[g] = (R)0;
-    LayV cur = rc.lay(0);  // layer kin; the next one is loaded an iteration ahead (its p and T are
-                           // needed at once for the level temperature below the layer)
-    double tz_top = cur.t;
-    double la_c = log_d(cur.pl);
+    LayV cur = rc.lay(0);  // layer kin; the next one is loaded an iteration ahead
+    double tz_top = cs.tz[t], la_nx = cs.la[t], tz_nx = cs.tz[(size_t)n + t];
     for (int kin = 0; kin < nrl; ++kin) {
         const int k = nrl - 1 - kin;
         const LayV nxt = (kin + 1 < nrl) ? rc.lay(kin + 1) : cur;
-        double tz_bot, la_n = 0.0;
-        if (kin + 1 < nrl) {
-            la_n = log_d(nxt.pl);
-            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
-        } else {
-            tz_bot = rc.tskn();
-        }
+        // this layer's log pressure and bottom level temperature, loaded an iteration ahead
+        const double la_c = la_nx, tz_bot = tz_nx;
+        if (kin + 1 < nrl) la_nx = cs.la[(size_t)(kin + 1) * n + t], tz_nx = cs.tz[(size_t)(kin + 2) * n + t];
         LwLay<R> L;
         R tau[NG], fr[NG];
         if (SURR && !table) {
@@ -721,7 +740,6 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
             acc.lw_up[t] = s0;
         }
         tz_top = tz_bot;
-        la_c = la_n;
         cur = nxt;
     }
     for (int k = 0; k < nrl; ++k) {  // up sweep over the cached layers
@@ -790,7 +808,7 @@ __device__ __forceinline__ void layer_rt_eval(const Tabs<R> *__restrict__ T, R t
 template <typename R, int B, bool SURR = false>
 __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
                                             int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o,
-                                            const Surr &sg, size_t s, int table)
+                                            const Surr &sg, size_t s, int table, double la)
 {
     constexpr int NG = SW_NG[B];
     const int nrl = rc.nrl;
@@ -807,7 +825,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
             tg[g] = tau - tr[g];
         }
     } else {
-        sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+        sw_layer<R>(T, rc, par, kin, la, L);
         sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
     }
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
@@ -859,7 +877,8 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     }
     for (int k = 0; k < nrl; ++k) {
         SwLayerRT<R, B> o;
-        sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table);
+        sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table,
+                                cs.la[(size_t)(nrl - 1 - k) * n + t]);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
             const R zden1 = (R)1 / ((R)1 - pd[g] * o.rd[g]);
@@ -902,7 +921,8 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
                 o.td[g] = cc.at(5, NG, g, s - 1), o.ld[g] = cc.at(6, NG, g, s - 1);
             }
         } else {
-            sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table);
+            sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table,
+                                    cs.la[(size_t)(nrl - s) * n + t]);
         }
         su = (R)0, sd = (R)0;
 #pragma unroll

@@ -224,18 +224,26 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
     // layer inputs: cur = layer kin, nxt = kin + 1 (loaded an iteration ahead), nn = kin + 2
     LayV cur = rc.lay(0), nxt = cur;
     if (nrl > 1) nxt = rc.lay(1);
-    double tz_top = cur.t;
-    double la_c = log_d(cur.pl);
+    double tz_top = cs.tz[t], la_nx = cs.la[t], tz_nx = 0.0;
+    if constexpr (sizeof(R) != 8) tz_nx = cs.tz[(size_t)n + t];
     for (int kin = 0; kin < nrl; ++kin) {
         const int kk = nrl - kin;  // 1-based surface-first layer
         LayV nn = nxt;
         if (kin + 2 < nrl) nn = rc.lay(kin + 2);
-        double tz_bot, la_n = 0.0;
-        if (kin + 1 < nrl) {
-            la_n = log_d(nxt.pl);
-            tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_n, la_c);
+        // this layer's log pressure and bottom level temperature: FP32 kernels load k_colprep's values an
+        // iteration ahead; FP64 kernels evaluate them here by the same operations (measured faster there)
+        double la_c, tz_bot;
+        if constexpr (sizeof(R) == 8) {
+            la_c = la_nx;
+            if (kin + 1 < nrl) {
+                la_nx = log_d(nxt.pl);
+                tz_bot = tlvl_v(nxt.t, cur.t, log_d(cur.pb), la_nx, la_c);
+            } else {
+                tz_bot = rc.tskn();
+            }
         } else {
-            tz_bot = rc.tskn();
+            la_c = la_nx, tz_bot = tz_nx;
+            if (kin + 1 < nrl) la_nx = cs.la[(size_t)(kin + 1) * n + t], tz_nx = cs.tz[(size_t)(kin + 2) * n + t];
         }
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
@@ -278,27 +286,34 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         }
         acc.lw_dn[(size_t)(kk - 1) * n + t] = s;
         tz_top = tz_bot;
-        la_c = la_n;
         cur = nxt, nxt = nn;
     }
     // ---- up sweep, surface to top ----
     const R reflct = (R)1 - semis;
     R radtotu[NG], totradu[NG], clrradu[NG];
-    double tz_bot = rc.tskn();
+    double tz_bot = cs.tz[(size_t)nrl * n + t];
     // layer inputs: cur = layer kin, nxt = kin - 1 (loaded an iteration ahead), nn = kin - 2
     cur = rc.lay(nrl - 1), nxt = cur;
     if (nrl > 1) nxt = rc.lay(nrl - 2);
-    la_c = log_d(cur.pl);
+    if constexpr (sizeof(R) == 8)
+        la_nx = log_d(cur.pl);
+    else
+        la_nx = cs.la[(size_t)(nrl - 1) * n + t], tz_nx = cs.tz[(size_t)(nrl - 1) * n + t];
     for (int kin = nrl - 1; kin >= 0; --kin) {
         const int kk = nrl - kin;
         LayV nn = nxt;
         if (kin >= 2) nn = rc.lay(kin - 2);
-        double la_m = 0.0;
-        if (kin > 0) {
-            la_m = log_d(nxt.pl);
-            tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_m);
+        const double la_c = la_nx;
+        if constexpr (sizeof(R) == 8) {
+            if (kin > 0) {
+                la_nx = log_d(nxt.pl);
+                tz_top = tlvl_v(cur.t, nxt.t, log_d(cur.pt), la_c, la_nx);
+            } else {
+                tz_top = cur.t;
+            }
         } else {
-            tz_top = cur.t;
+            tz_top = tz_nx;
+            if (kin > 0) la_nx = cs.la[(size_t)(kin - 1) * n + t], tz_nx = cs.tz[(size_t)(kin - 1) * n + t];
         }
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
@@ -354,7 +369,6 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         }
         acc.lw_up[(size_t)kk * n + t] = s;
         tz_bot = tz_top;
-        la_c = la_m;
         cur = nxt, nxt = nn;
     }
 }
@@ -364,12 +378,12 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
 // clear ones by the layer cloud fraction zc1 = cfrac / zcf1 (phys_rrtm_sw_spcvrtc.cu, cloudy block)
 template <typename R, int B>
 __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
-                                          int lt, int pass, double cf1, R cz, R snt, SwLayerRT<R, B> &o)
+                                          int lt, int pass, double cf1, R cz, R snt, SwLayerRT<R, B> &o, double la)
 {
     constexpr int NG = SW_NG[B];
     const int kin = rc.nrl - 1 - k;
     SwLay<R> L;
-    sw_layer<R>(T, rc, par, kin, log_d(rc.plyr(kin)), L);
+    sw_layer<R>(T, rc, par, kin, la, L);
     R tg[NG], tr[NG];
     sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
 #pragma unroll
@@ -431,7 +445,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
         }
         for (int k = 0; k < nrl; ++k) {
             SwLayerRT<R, B> o;
-            swr_layer<R, B>(T, rc, par, k, lt, pass, cf1, cz, snt, o);
+            swr_layer<R, B>(T, rc, par, k, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - k) * n + t]);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
                 const R zden1 = (R)1 / ((R)1 - pd[g] * o.rd[g]);
@@ -472,7 +486,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
                         o.td[g] = cc.at(5, NG, g, s), o.ld[g] = cc.at(6, NG, g, s);
                     }
                 } else {
-                    swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o);
+                    swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - s) * n + t]);
                 }
                 su = (R)0, sd = (R)0;
 #pragma unroll
