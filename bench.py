@@ -38,22 +38,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11652, "k_lw_b02": 11455, "k_lw_b03": 8493, "k_lw_b04": 9276, "k_lw_b05": 9468, "k_lw_b06": 9890,
-    "k_lw_b07": 9026, "k_lw_b08": 11909, "k_lw_b09": 8061, "k_lw_b10": 9168, "k_lw_b11": 9890, "k_lw_b12": 9884,
-    "k_lw_b13": 8147, "k_lw_b14": 7657, "k_lw_b15": 8474, "k_lw_b16": 7484, "k_sw_b16": 20331, "k_sw_b17": 32950,
-    "k_sw_b18": 24319, "k_sw_b19": 24301, "k_sw_b20": 28633, "k_sw_b21": 28650, "k_sw_b22": 11272,
-    "k_sw_b23": 28638, "k_sw_b24": 25259, "k_sw_b25": 21051, "k_sw_b26": 19993, "k_sw_b27": 25247,
-    "k_sw_b28": 21241, "k_sw_b29": 32947,
+    "k_lw_b01": 11658, "k_lw_b02": 11461, "k_lw_b03": 8475, "k_lw_b04": 9285, "k_lw_b05": 9480, "k_lw_b06": 9904,
+    "k_lw_b07": 9036, "k_lw_b08": 11586, "k_lw_b09": 8081, "k_lw_b10": 9179, "k_lw_b11": 9895, "k_lw_b12": 9894,
+    "k_lw_b13": 8157, "k_lw_b14": 7666, "k_lw_b15": 8480, "k_lw_b16": 7495, "k_sw_b16": 21330, "k_sw_b17": 33929,
+    "k_sw_b18": 25289, "k_sw_b19": 25269, "k_sw_b20": 29605, "k_sw_b21": 29622, "k_sw_b22": 12240,
+    "k_sw_b23": 29609, "k_sw_b24": 26232, "k_sw_b25": 22005, "k_sw_b26": 19991, "k_sw_b27": 26217,
+    "k_sw_b28": 22213, "k_sw_b29": 33917,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 17877, "k_lw_b02": 20139, "k_lw_b03": 29015, "k_lw_b04": 23863, "k_lw_b05": 28926,
-    "k_lw_b06": 14550, "k_lw_b07": 21902, "k_lw_b08": 15062, "k_lw_b09": 21050, "k_lw_b10": 12105,
-    "k_lw_b11": 14606, "k_lw_b12": 14646, "k_lw_b13": 9583, "k_lw_b14": 6273, "k_lw_b15": 6735, "k_lw_b16": 6253,
-    "k_sw_b16": 25304, "k_sw_b17": 47716, "k_sw_b18": 32512, "k_sw_b19": 32505, "k_sw_b20": 40252,
-    "k_sw_b21": 40123, "k_sw_b22": 10874, "k_sw_b23": 40234, "k_sw_b24": 32953, "k_sw_b25": 25786,
-    "k_sw_b26": 25287, "k_sw_b27": 33023, "k_sw_b28": 25769, "k_sw_b29": 47657,
+    "k_lw_b01": 18798, "k_lw_b02": 21083, "k_lw_b03": 29220, "k_lw_b04": 24534, "k_lw_b05": 27984,
+    "k_lw_b06": 15474, "k_lw_b07": 21969, "k_lw_b08": 15961, "k_lw_b09": 21459, "k_lw_b10": 13109,
+    "k_lw_b11": 15509, "k_lw_b12": 15525, "k_lw_b13": 10489, "k_lw_b14": 7274, "k_lw_b15": 7750, "k_lw_b16": 7268,
+    "k_sw_b16": 25803, "k_sw_b17": 48083, "k_sw_b18": 33010, "k_sw_b19": 33010, "k_sw_b20": 40622,
+    "k_sw_b21": 40492, "k_sw_b22": 11373, "k_sw_b23": 40636, "k_sw_b24": 33460, "k_sw_b25": 26281,
+    "k_sw_b26": 25288, "k_sw_b27": 33465, "k_sw_b28": 26261, "k_sw_b29": 48027,
 }
 
 
