@@ -77,6 +77,22 @@ def test_fp64_bit_identical(rad):
             f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
 
 
+def test_day_list_over_several_segments(gpu_device):
+    """One 7,000-column tile: the day list is compacted from four 2,048-column segments, in order."""
+    cols = gen.columns(7000, seed=13)
+    nday = int(np.count_nonzero(cols["cosz"] > 0.0))
+    assert 0 < nday < 7000
+    r = Radiation(0, max_cols=8192)
+    try:
+        out, _, ref, _ = run_both(r, cols)
+    finally:
+        r.close()
+    for k in abi.OUTPUT_FIELDS:
+        a = out[k].cpu().numpy()
+        assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), \
+            f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
+
+
 @pytest.mark.parametrize("iovr", [0, 1, 2])
 def test_overlap_and_seed(rad, iovr):
     cols = gen.columns(500, seed=20 + iovr)
