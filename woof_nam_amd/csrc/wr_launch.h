@@ -4,8 +4,8 @@
 // then the FP64 post.  Every band kernel writes its own flux planes (BandAcc), so the band kernels
 // are independent of one another: they are spread over the context's auxiliary streams (fork and
 // join through events, also under graph capture once the streams' pools exist), which fills the CUs
-// that a finishing band kernel leaves idle with the next band's waves.  k_post adds the planes in band order, so the
-// result does not depend on which stream ran which band.
+// that a finishing band kernel leaves idle with the next band's waves.  k_post adds the planes in
+// band order, so the result does not depend on which stream ran which band.
 #pragma once
 #include "wr_internal.h"
 #include "wr_nam.h"
