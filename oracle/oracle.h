@@ -131,4 +131,14 @@ void osw_cldprop(const OT *t, SwCol *c, int mcica);
 void osw_taumol(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL]);
 void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL], const uint32_t *mask,
                  int mcica, double *fup, double *fdn, double *fdn0_sfc);
+/* pieces exported for the oracle's own pins (oracle_pins.c) */
+void olw_band(const OT *t, const LwCol *c, int b, int k, double *tau, double *fr);
+void osw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, double *tr);
+double osw_expm(const OT *t, double z, int clamped);
+void osw_layer_opt(const OT *t, double tau0, double ssa0, double asy0, double cz, double *rb, double *rd, double *tb,
+                   double *td, double *tau1);
+void osw_gpoint(const double *tbl, int nlay, double cz, double snt, double albb, double albd, const double *tr,
+                const double *tg, int cloudy, const double *zc, const double *tc, const double *sc, const double *ac,
+                double *f0u, double *f0d, double *fcu, double *fcd);
+const OT *oracle_tables(void);
 #endif

@@ -172,7 +172,6 @@ int ot_bind(OT *t, const char *path)
     const double *s16 = opt_d(t, "sw/b16/strrat16"), *s28 = opt_d(t, "sw/b28/strrat28");
     if (s16) t->sw[0].strrat = s16[0];
     if (s28) t->sw[12].strrat = s28[0];
-    t->sw[2].strrat = 3.89589e+1; /* strrat(18): phys_rrtm_sw_taumol18.py:43 */
     osw_table(t->sw_exp_tbl);
     return WR_OK;
 }
@@ -404,6 +403,7 @@ int oracle_run(const wr_columns *in, const wr_params *p, const wr_outputs *out, 
 }
 
 /* pieces for tests */
+const OT *oracle_tables(void) { return g_tab; }
 void oracle_lw_tables(double *e, double *f, double *t) { olw_tables(e, f, t); }
 void oracle_sw_table(double *e) { osw_table(e); }
 double oracle_exp(double x) { return om_exp(x); }

@@ -1127,3 +1127,6 @@ void olw_rtrnmr(const OT *t, const LwCol *c, double (*taug)[O_NL], double (*frac
         fprev = fn;
     }
 }
+
+/* one band of one layer from a hand-built setcoef record (the oracle's own pins, tests) */
+void olw_band(const OT *t, const LwCol *c, int b, int k, double *tau, double *fr) { lw_band(t, c, b, k, tau, fr); }

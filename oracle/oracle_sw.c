@@ -373,7 +373,7 @@ static void sw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, doub
     case 19: {
         for (int g = 0; g < ng; ++g) tr[g] = cm * 2.29e-09;
         if (kk <= L) {
-            const double speccomb = ca[0] + 5.49281e+0 * ca[1];
+            const double speccomb = ca[0] + B->strrat * ca[1];
             const double specmult = 8.0 * fmin(0.999999, ca[0] / speccomb);
             const int js = 1 + (int)specmult;
             fac8(specmult - (double)((int)specmult), f00, f01, f10, f11, fc);
@@ -402,7 +402,7 @@ static void sw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, doub
     }
     case 21: {
         for (int g = 0; g < ng; ++g) tr[g] = cm * 9.41e-09;
-        const double speccomb = ca[0] + 4.53210e-03 * ca[1];
+        const double speccomb = ca[0] + B->strrat * ca[1];
         if (kk < L) {
             const double specmult = 8.0 * fmin(O_ONEMINUS, ca[0] / speccomb);
             const int js = 1 + (int)specmult;
@@ -424,7 +424,7 @@ static void sw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, doub
         for (int g = 0; g < ng; ++g) tr[g] = cm * 1.54e-08;
         const double o2cont = (4.35e-4 / (350.0 * 2.0)) * ca[5];
         if (kk <= L) {
-            const double speccomb = ca[0] + 3.63328e-02 * ca[5];
+            const double speccomb = ca[0] + B->strrat * ca[5];
             const double ratio = (speccomb > 0.0) ? (ca[0] / speccomb) : 0.0;
             const double specmult = 8.0 * fmin(O_ONEMINUS, ratio);
             const int js = 1 + (int)specmult;
@@ -450,7 +450,7 @@ static void sw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, doub
     }
     case 24: {
         if (kk <= L) {
-            const double speccomb = ca[0] + 1.24692e-01 * ca[5];
+            const double speccomb = ca[0] + B->strrat * ca[5];
             const double specmult = 8.0 * fmin(O_ONEMINUS, ca[0] / speccomb);
             const int jsi = (int)specmult;
             const int js = 1 + jsi;
@@ -666,6 +666,78 @@ static const int SW_IDXSFC[14] = {1, 1, 1, 1, 1, 1, 1, 1, 0, 2, 2, 2, 2, 1}; /* 
 /* Total-sky and clear-sky band fluxes summed to level fluxes fup/fdn [nlay+1] (surface-first).
  * mcica = 0: the reference's spcvrtc (cldfrc mixing, cf1 weighting); mcica = 1: the same two-stream
  * with the per-g 0/1 McICA mask in place of cldfrc and no cf1 weighting. */
+/* One g-point of spcvrtc for one column, both passes (the body of the reference's per-g launch,
+ * phys_rrtm_sw_spcvrtc.cu:217-484).  Per-layer inputs are surface-first (index k-1 for Fortran layer
+ * k): tr/tg the Rayleigh and gas optical depths; zc the cloud weight of the total-sky pass (cldfrc
+ * on the NAM route, the McICA 0/1 mask bit otherwise) and tc/sc/ac the band's cloud optics.  Writes
+ * the clear-sky level fluxes f0u/f0d and, when cloudy != 0, the total-sky ones fcu/fcd, per unit
+ * incident flux, levels 1..nlay+1 (index 0 unused). */
+void osw_gpoint(const double *tbl, int nlay, double cz, double snt, double albb, double albd, const double *tr,
+                const double *tg, int cloudy, const double *zc, const double *tc, const double *sc, const double *ac,
+                double *f0u, double *f0d, double *fcu, double *fcd)
+{
+    const int nlp1 = nlay + 1;
+    double zrefb[O_NL + 3], zrefd[O_NL + 3], ztrab[O_NL + 3], ztrad[O_NL + 3], ztdbt[O_NL + 3], zldbt[O_NL + 3];
+    double ztaus[O_NL + 3], zssas[O_NL + 3], zasys[O_NL + 3], zldbt0[O_NL + 3];
+    double ztdbt0 = 1.0;
+    ztdbt[nlp1] = 1.0;
+    zldbt[1] = 0.0;
+    zrefb[1] = albb;
+    zrefd[1] = albd;
+    ztrab[1] = 0.0;
+    ztrad[1] = 0.0;
+    for (int k = nlay; k >= 1; --k) {
+        const int kp = k + 1;
+        const double ztau0 = fmax(O_FTINY, tr[k - 1] + tg[k - 1] + 0.0);
+        const double zssa0 = tr[k - 1] + 0.0;
+        const double zasy0 = 0.0;
+        ztaus[k] = ztau0;
+        zssas[k] = zssa0;
+        zasys[k] = zasy0;
+        double ztau1;
+        layer_opt(ztau0, zssa0, zasy0, cz, snt, tbl, &zrefb[kp], &zrefd[kp], &ztrab[kp], &ztrad[kp], &ztau1);
+        const double zexp3 = expm(ztau1 * snt, tbl, 1);
+        ztdbt[k] = zexp3 * ztdbt[kp];
+        zldbt[kp] = zexp3;
+        const double zexp4 = expm(ztau0 * snt, tbl, 1);
+        zldbt0[k] = zexp4;
+        ztdbt0 = zexp4 * ztdbt0;
+    }
+    swflux(zrefb, zrefd, ztrab, ztrad, zldbt, ztdbt, nlay, f0u, f0d);
+    if (!cloudy) return;
+    ztdbt0 = 1.0;
+    zldbt[1] = 0.0;
+    for (int k = nlay; k >= 1; --k) {
+        const int kp = k + 1;
+        const double zc1 = zc[k - 1];
+        const double zc0 = 1.0 - zc1;
+        if (zc1 > O_FTINY) {
+            const double ztau0 = ztaus[k] + tc[k - 1];
+            const double zssa0 = zssas[k] + sc[k - 1];
+            const double zasy0 = zasys[k] + ac[k - 1];
+            const double zrefb1 = zrefb[kp], zrefd1 = zrefd[kp], ztrab1 = ztrab[kp], ztrad1 = ztrad[kp];
+            double ztau1;
+            layer_opt(ztau0, zssa0, zasy0, cz, snt, tbl, &zrefb[kp], &zrefd[kp], &ztrab[kp], &ztrad[kp], &ztau1);
+            zrefb[kp] = zc0 * zrefb1 + zc1 * zrefb[kp];
+            zrefd[kp] = zc0 * zrefd1 + zc1 * zrefd[kp];
+            ztrab[kp] = zc0 * ztrab1 + zc1 * ztrab[kp];
+            ztrad[kp] = zc0 * ztrad1 + zc1 * ztrad[kp];
+            const double zexp3 = expm(ztau1 * snt, tbl, 1);
+            zldbt[kp] = zc0 * zldbt[kp] + zc1 * zexp3;
+            ztdbt[k] = zldbt[kp] * ztdbt[kp];
+            const double zexp4 = expm(ztau0 * snt, tbl, 1);
+            ztdbt0 = (zc0 * zldbt0[k] + zc1 * zexp4) * ztdbt0;
+        } else {
+            ztdbt[k] = zldbt[kp] * ztdbt[kp];
+            ztdbt0 = zldbt0[k] * ztdbt0;
+        }
+    }
+    swflux(zrefb, zrefd, ztrab, ztrad, zldbt, ztdbt, nlay, fcu, fcd);
+}
+
+/* Total-sky and clear-sky band fluxes summed to level fluxes fup/fdn [nlay+1] (surface-first).
+ * mcica = 0: the reference's spcvrtc (cldfrc mixing, cf1 weighting); mcica = 1: the same two-stream
+ * with the per-g 0/1 McICA mask in place of cldfrc and no cf1 weighting. */
 void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*taur)[O_NL], const uint32_t *mask,
                  int mcica, double *fup, double *fdn, double *fdn0_sfc)
 {
@@ -675,85 +747,33 @@ void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*tau
     double fxupc[14][O_NL + 1], fxdnc[14][O_NL + 1], fxup0[14][O_NL + 1], fxdn0[14][O_NL + 1];
     for (int b = 0; b < 14; ++b)
         for (int k = 0; k < nlp1; ++k) fxupc[b][k] = fxdnc[b][k] = fxup0[b][k] = fxdn0[b][k] = 0.0;
-    double zrefb[O_NL + 3], zrefd[O_NL + 3], ztrab[O_NL + 3], ztrad[O_NL + 3], ztdbt[O_NL + 3], zldbt[O_NL + 3];
-    double zfu[O_NL + 3], zfd[O_NL + 3], ztaus[O_NL + 3], zssas[O_NL + 3], zasys[O_NL + 3], zldbt0[O_NL + 3];
+    double f0u[O_NL + 3], f0d[O_NL + 3], fcu[O_NL + 3], fcd[O_NL + 3], zc[O_NL];
+    const int cloudy = mcica || cf1v > O_EPS;
     for (int bi = 0; bi < 14; ++bi) {
         const int ibd = SW_IDXSFC[bi];
+        double albb, albd;
+        if (ibd != 0) {
+            albb = c->albbm[ibd - 1];
+            albd = c->albdf[ibd - 1];
+        } else {
+            albb = 0.5 * (c->albbm[0] + c->albbm[1]);
+            albd = 0.5 * (c->albdf[0] + c->albdf[1]);
+        }
         for (int jg = SW_NSA[bi]; jg < SW_NSA[bi] + SW_NGA[bi]; ++jg) {
             const double zsolar = c->ssolar * c->sfluxzen[jg];
-            double ztdbt0 = 1.0;
-            ztdbt[nlp1] = 1.0;
-            zldbt[1] = 0.0;
-            if (ibd != 0) {
-                zrefb[1] = c->albbm[ibd - 1];
-                zrefd[1] = c->albdf[ibd - 1];
-            } else {
-                zrefb[1] = 0.5 * (c->albbm[0] + c->albbm[1]);
-                zrefd[1] = 0.5 * (c->albdf[0] + c->albdf[1]);
-            }
-            ztrab[1] = 0.0;
-            ztrad[1] = 0.0;
-            for (int k = nlay; k >= 1; --k) {
-                const int kp = k + 1;
-                const double tr = taur[jg][k - 1], tgv = taug[jg][k - 1];
-                const double ztau0 = fmax(O_FTINY, tr + tgv + 0.0);
-                const double zssa0 = tr + 0.0;
-                const double zasy0 = 0.0;
-                ztaus[k] = ztau0;
-                zssas[k] = zssa0;
-                zasys[k] = zasy0;
-                double ztau1;
-                layer_opt(ztau0, zssa0, zasy0, cz, snt, tbl, &zrefb[kp], &zrefd[kp], &ztrab[kp], &ztrad[kp], &ztau1);
-                const double zexp3 = expm(ztau1 * snt, tbl, 1);
-                ztdbt[k] = zexp3 * ztdbt[kp];
-                zldbt[kp] = zexp3;
-                const double zexp4 = expm(ztau0 * snt, tbl, 1);
-                zldbt0[k] = zexp4;
-                ztdbt0 = zexp4 * ztdbt0;
-            }
-            swflux(zrefb, zrefd, ztrab, ztrad, zldbt, ztdbt, nlay, zfu, zfd);
+            for (int k = 0; k < nlay; ++k)
+                zc[k] = mcica ? (((mask[k * 4 + (jg >> 5)] >> (jg & 31)) & 1u) ? 1.0 : 0.0) : c->cldfrc[k];
+            osw_gpoint(tbl, nlay, cz, snt, albb, albd, taur[jg], taug[jg], cloudy, zc, c->taucw[bi], c->ssacw[bi],
+                       c->asycw[bi], f0u, f0d, fcu, fcd);
             for (int k = 1; k <= nlp1; ++k) {
-                fxup0[bi][k - 1] = fxup0[bi][k - 1] + zsolar * zfu[k];
-                fxdn0[bi][k - 1] = fxdn0[bi][k - 1] + zsolar * zfd[k];
+                fxup0[bi][k - 1] = fxup0[bi][k - 1] + zsolar * f0u[k];
+                fxdn0[bi][k - 1] = fxdn0[bi][k - 1] + zsolar * f0d[k];
             }
-            if (mcica || cf1v > O_EPS) {
-                ztdbt0 = 1.0;
-                zldbt[1] = 0.0;
-                for (int k = nlay; k >= 1; --k) {
-                    const int kp = k + 1;
-                    double zc1;
-                    if (mcica)
-                        zc1 = ((mask[(k - 1) * 4 + (jg >> 5)] >> (jg & 31)) & 1u) ? 1.0 : 0.0;
-                    else
-                        zc1 = c->cldfrc[k - 1];
-                    const double zc0 = 1.0 - zc1;
-                    if (zc1 > O_FTINY) {
-                        const double ztau0 = ztaus[k] + c->taucw[bi][k - 1];
-                        const double zssa0 = zssas[k] + c->ssacw[bi][k - 1];
-                        const double zasy0 = zasys[k] + c->asycw[bi][k - 1];
-                        const double zrefb1 = zrefb[kp], zrefd1 = zrefd[kp], ztrab1 = ztrab[kp], ztrad1 = ztrad[kp];
-                        double ztau1;
-                        layer_opt(ztau0, zssa0, zasy0, cz, snt, tbl, &zrefb[kp], &zrefd[kp], &ztrab[kp], &ztrad[kp], &ztau1);
-                        zrefb[kp] = zc0 * zrefb1 + zc1 * zrefb[kp];
-                        zrefd[kp] = zc0 * zrefd1 + zc1 * zrefd[kp];
-                        ztrab[kp] = zc0 * ztrab1 + zc1 * ztrab[kp];
-                        ztrad[kp] = zc0 * ztrad1 + zc1 * ztrad[kp];
-                        const double zexp3 = expm(ztau1 * snt, tbl, 1);
-                        zldbt[kp] = zc0 * zldbt[kp] + zc1 * zexp3;
-                        ztdbt[k] = zldbt[kp] * ztdbt[kp];
-                        const double zexp4 = expm(ztau0 * snt, tbl, 1);
-                        ztdbt0 = (zc0 * zldbt0[k] + zc1 * zexp4) * ztdbt0;
-                    } else {
-                        ztdbt[k] = zldbt[kp] * ztdbt[kp];
-                        ztdbt0 = zldbt0[k] * ztdbt0;
-                    }
-                }
-                swflux(zrefb, zrefd, ztrab, ztrad, zldbt, ztdbt, nlay, zfu, zfd);
+            if (cloudy)
                 for (int k = 1; k <= nlp1; ++k) {
-                    fxupc[bi][k - 1] = fxupc[bi][k - 1] + zsolar * zfu[k];
-                    fxdnc[bi][k - 1] = fxdnc[bi][k - 1] + zsolar * zfd[k];
+                    fxupc[bi][k - 1] = fxupc[bi][k - 1] + zsolar * fcu[k];
+                    fxdnc[bi][k - 1] = fxdnc[bi][k - 1] + zsolar * fcd[k];
                 }
-            }
         }
     }
     if (!mcica) {
@@ -785,4 +805,13 @@ void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*tau
             for (int b = 0; b < 14; ++b) s0 = s0 + fxdn0[b][0];
         *fdn0_sfc = s0;
     }
+}
+
+/* ---- pieces for the oracle's own pins (tests) ------------------------------------------------- */
+void osw_band(const OT *t, const SwCol *c, int bi, int k, double *tg, double *tr) { sw_band(t, c, bi, k, tg, tr); }
+double osw_expm(const OT *t, double z, int clamped) { return expm(z, t->sw_exp_tbl, clamped); }
+void osw_layer_opt(const OT *t, double tau0, double ssa0, double asy0, double cz, double *rb, double *rd, double *tb,
+                   double *td, double *tau1)
+{
+    layer_opt(tau0, ssa0, asy0, cz, 1.0 / cz, t->sw_exp_tbl, rb, rd, tb, td, tau1);
 }

@@ -6,6 +6,7 @@ as the checker.  The product (woof_nam_amd) never imports it.
 from __future__ import annotations
 
 import ctypes as C
+import json
 import os
 import subprocess
 from pathlib import Path
@@ -29,25 +30,66 @@ def build(force: bool = False) -> Path:
     return LIB
 
 
+def _bind(path):
+    """Load one build of the oracle (the default one, or a copy built by a test) and load its tables."""
+    L = C.CDLL(str(path))
+    L.oracle_init.argtypes = [C.c_char_p]
+    L.oracle_run.argtypes = [C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
+                             C.POINTER(abi.WrDiag), C.c_int]
+    for f in ("oracle_exp", "oracle_log"):
+        getattr(L, f).argtypes = [C.c_double]
+        getattr(L, f).restype = C.c_double
+    L.oracle_pow.argtypes = [C.c_double, C.c_double]
+    L.oracle_pow.restype = C.c_double
+    dp = C.POINTER(C.c_double)
+    L.oracle_lw_setcoef_layer.argtypes = [C.c_double, C.c_double, dp, C.c_double, C.c_double, dp]
+    L.oracle_lw_band_layer.argtypes = [dp, C.c_int, dp, dp]
+    L.oracle_sw_setcoef_layer.argtypes = [C.c_double, C.c_double, C.c_double, dp, dp]
+    L.oracle_sw_band_layer.argtypes = [dp, C.c_int, dp, dp]
+    L.oracle_lw_cldprop_layer.argtypes = [C.c_double] * 6 + [dp]
+    L.oracle_sw_cldprop_layer.argtypes = [C.c_double] * 6 + [dp, dp, dp]
+    L.oracle_sw_expm.argtypes = [C.c_double, C.c_int]
+    L.oracle_sw_expm.restype = C.c_double
+    L.oracle_sw_layer_opt.argtypes = [C.c_double] * 4 + [dp]
+    L.oracle_sw_gpoint.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int, dp, dp, dp, dp, dp]
+    rc = L.oracle_init(str(TABLES).encode())
+    if rc != 0:
+        raise RuntimeError(f"oracle: table load failed ({rc})")
+    return L
+
+
 def lib():
     global _lib
     if _lib is None:
         if not LIB.exists():
             build()
-        L = C.CDLL(str(LIB))
-        L.oracle_init.argtypes = [C.c_char_p]
-        L.oracle_run.argtypes = [C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
-                                 C.POINTER(abi.WrDiag), C.c_int]
-        for f in ("oracle_exp", "oracle_log"):
-            getattr(L, f).argtypes = [C.c_double]
-            getattr(L, f).restype = C.c_double
-        L.oracle_pow.argtypes = [C.c_double, C.c_double]
-        L.oracle_pow.restype = C.c_double
-        rc = L.oracle_init(str(TABLES).encode())
-        if rc != 0:
-            raise RuntimeError(f"oracle: table load failed ({rc})")
-        _lib = L
+        _lib = _bind(LIB)
     return _lib
+
+
+def load(path):
+    """A separately built copy of the oracle (for tests that plant a mistake in it)."""
+    return _bind(path)
+
+
+_tables = None
+
+
+def tables() -> dict:
+    """The table file as numpy arrays keyed by name (float32 entries widened to float64, as the
+    oracle reads them)."""
+    global _tables
+    if _tables is None:
+        meta = json.loads(TABLES.with_suffix(".json").read_text())
+        buf = TABLES.read_bytes()
+        out = {}
+        for t in meta["tables"]:
+            dt = np.float32 if t["dtype"] == "float32" else np.int32
+            n = int(np.prod(t["shape"])) if t["shape"] else 1
+            a = np.frombuffer(buf, dtype=dt, count=n, offset=t["offset"]).reshape(t["shape"])
+            out[t["name"]] = a.astype(np.float64) if dt == np.float32 else a.astype(np.int64)
+        _tables = out
+    return _tables
 
 
 def _ptr(a, ctype=C.c_double):

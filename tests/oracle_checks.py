@@ -1,0 +1,640 @@
+"""Pins of the CPU oracle's gas optics, cloud optics, setcoef weights and SW two-stream.
+
+Every check runs one piece of the oracle (oracle/oracle_pins.c entry points, which call the same
+code oracle_run uses) at a point where mathematics or the table data fix the answer, and compares
+with numpy arithmetic on the table file:
+
+* setcoef: bilinear interpolation weights reproduce any linear function of (ln p, T) exactly, so
+  sum(fac) = 1, sum(fac * ln p_node) = ln p and sum(fac * T_node) = T; the minor, self and foreign
+  temperature indices reproduce T through their own node spacing;
+* gas optics (taugb01..16, taumol16..29): with the interpolation weight on one corner, the continua
+  and minor gases off and the binary-species parameter on an eta node, tau is the key amount times
+  ONE table entry (times the band's closed-form correction where the reference applies one: band 1
+  and 2 `corradj`, the g-point factors of bands 4 and 7, band 23's 1.029); continua, minor gases,
+  cross sections and Rayleigh are checked the same way, one term at a time;
+* cloud optics: at the effective-radius nodes the interpolation is the table entry;
+* two-stream: the exponential against exp, R -> 0 and T -> 1 as tau -> 0, energy conservation of a
+  non-absorbing layer and column (which fixes the PIFM gamma1 = gamma2 symmetry at w0 = 1), Beer-Lambert
+  for a purely absorbing column.
+
+The per-band facts used here (key species, eta partner and rate, table row counts, corrections) are
+those of the reference kernels namgpu/kernels/phys_rrtm_lw_taugbNN.cu and phys_rrtm_sw_taumolNN.cu.
+Functions take the ctypes handle of an oracle build, so a test can run them on a copy of the oracle
+with a planted mistake and require that one fails.  TEST INFRASTRUCTURE ONLY.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+
+import numpy as np
+
+from oracle import pyoracle
+
+# record layouts of oracle/oracle_pins.c
+LR = dict(LOWER=0, JP=1, JT=2, JT1=3, INDS=4, INDF=5, INDM=6, F00=7, F01=8, F10=9, F11=10, SF=11, SFR=12, FF=13,
+          FFR=14, MF=15, SCM=16, SCMN2=17, CA=18, WX=25, COLDRY=29, COLBRD=30, PAVEL=31, RF=32, N=44)
+SR = dict(LAYTROP=0, JP=1, JT=2, JT1=3, INDS=4, INDF=5, F00=6, F01=7, F10=8, F11=9, SF=10, SFR=11, FF=12, FFR=13,
+          CA=14, COLMOL=20, N=21)
+LW_NG = [10, 12, 16, 14, 16, 8, 12, 8, 12, 6, 8, 8, 4, 2, 2, 2]
+SW_NG = [6, 12, 8, 8, 10, 10, 2, 10, 8, 6, 6, 8, 6, 12]
+H2O, CO2, O3, N2O, CH4, O2, CO = range(7)
+CORNERS = ("F00", "F10", "F01", "F11")
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _close(a, b, rtol=1e-12, atol=0.0, what=""):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    if not np.allclose(a, b, rtol=rtol, atol=atol):
+        i = int(np.argmax(np.abs(a - b) - rtol * np.abs(b)))
+        raise AssertionError(f"{what}: {a.ravel()[i]!r} != {b.ravel()[i]!r} (rtol {rtol}, atol {atol})")
+
+
+# ---------------------------------------------------------------------------------------------
+# LW setcoef
+# ---------------------------------------------------------------------------------------------
+def lw_setcoef(L, p, t, ca, coldry, colbrd):
+    rec = np.zeros(LR["N"])
+    ca = np.ascontiguousarray(ca, dtype=np.float64)
+    rc = L.oracle_lw_setcoef_layer(p, t, _dp(ca), coldry, colbrd, _dp(rec))
+    assert rc == 0
+    return rec
+
+
+def sw_setcoef(L, p, t, h2ovmr, ca):
+    rec = np.zeros(SR["N"])
+    ca = np.ascontiguousarray(ca, dtype=np.float64)
+    rc = L.oracle_sw_setcoef_layer(p, t, h2ovmr, _dp(ca), _dp(rec))
+    assert rc == 0
+    return rec
+
+
+def _weights_reproduce_linear(rec, R, plog, t, preflog, tref, what):
+    jp, jt, jt1 = int(rec[R["JP"]]), int(rec[R["JT"]]), int(rec[R["JT1"]])
+    f = {c: rec[R[c]] for c in CORNERS}
+    _close(sum(f.values()), 1.0, 0, 1e-14, f"{what} weights sum")
+    # the reference grid is 5 nodes per unit ln p (fp = 5 (preflog(jp) - ln p)): node jp+1 is 0.2 below
+    lnp = {"F00": preflog[jp - 1], "F10": preflog[jp - 1], "F01": preflog[jp - 1] - 0.2, "F11": preflog[jp - 1] - 0.2}
+    tn = {"F00": tref[jp - 1] + 15.0 * (jt - 3), "F10": tref[jp - 1] + 15.0 * (jt - 2),
+          "F01": tref[jp] + 15.0 * (jt1 - 3), "F11": tref[jp] + 15.0 * (jt1 - 2)}
+    _close(sum(f[c] * lnp[c] for c in CORNERS), plog, 0, 1e-9, f"{what} ln p reproduced")
+    _close(sum(f[c] * tn[c] for c in CORNERS), t, 0, 1e-8, f"{what} T reproduced")
+
+
+def check_lw_setcoef(L, n=400, seed=1):
+    """phys_rrtm_lw_setcoef.cu:155-195 weights and temperature indices reproduce (ln p, T)."""
+    T = pyoracle.tables()
+    pref, tref = T["lw/preflog"], T["lw/tref"]
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        jp0 = int(rng.integers(1, 58))                    # interior pressure interval jp0 .. jp0+1
+        plog = pref[jp0 - 1] - 0.2 * rng.uniform(0.02, 0.98)
+        p = math.exp(plog)
+        plog = math.log(p)
+        t = float(tref[jp0 - 1] + rng.uniform(-22.0, 22.0))
+        ca = np.array([rng.uniform(0.1, 5.0), 3.0, 0.01, 0.002, 0.01, 40.0, 1e-4])
+        coldry = 200.0
+        rec = lw_setcoef(L, p, t, ca, coldry, 150.0)
+        assert int(rec[LR["LOWER"]]) == int(plog > 4.56), "lower atmosphere is ln p > 4.56"
+        assert 1 <= rec[LR["JP"]] <= 58
+        _weights_reproduce_linear(rec, LR, plog, t, pref, tref, "LW setcoef")
+        indm, mf = rec[LR["INDM"]], rec[LR["MF"]]
+        if 188.0 < t < 310.0:
+            _close(180.8 + 7.2 * (indm + mf), t, 0, 1e-9, "minor temperature index")
+        if plog > 4.56:
+            if 245.7 < t < 310.3:
+                _close(188.0 + 7.2 * (rec[LR["INDS"]] + 7 + rec[LR["SFR"]]), t, 0, 1e-9, "self temperature index")
+            if 224.1 < t <= 296.0:
+                _close(332.0 - 36.0 * (rec[LR["INDF"]] + rec[LR["FFR"]]), t, 0, 1e-9, "foreign temperature index")
+        else:
+            assert rec[LR["INDF"]] == 3 and rec[LR["INDS"]] == 0
+            _close(188.0 + 36.0 * (rec[LR["FFR"]] + 1.0), t, 0, 1e-9, "upper foreign temperature index")
+        h2o = ca[0] / coldry
+        _close(rec[LR["SF"]], rec[LR["FF"]] * h2o, 1e-14, 0, "self factor = foreign factor * h2o vmr")
+        # foreign-broadening density scales as p / (T (1 + h2o)) (ideal gas)
+        rec2 = lw_setcoef(L, 2.0 * p, 2.0 * t, ca, coldry, 150.0)
+        _close(rec2[LR["FF"]], rec[LR["FF"]], 1e-14, 0, "foreign factor ~ p / T")
+
+
+def check_sw_setcoef(L, n=300, seed=2):
+    """phys_rrtm_sw_setcoef.cu:60-130: the same weights, unclamped."""
+    T = pyoracle.tables()
+    pref, tref = T["sw/preflog"], T["sw/tref"]
+    rng = np.random.default_rng(seed)
+    for _ in range(n):
+        jp0 = int(rng.integers(1, 58))
+        plog = pref[jp0 - 1] - 0.2 * rng.uniform(0.02, 0.98)
+        p = math.exp(plog)
+        plog = math.log(p)
+        t = float(tref[jp0 - 1] + rng.uniform(-22.0, 22.0))
+        h2o = rng.uniform(1e-6, 2e-2)
+        rec = sw_setcoef(L, p, t, h2o, np.array([1.0, 3.0, 0.01, 0.002, 0.01, 40.0]))
+        _weights_reproduce_linear(rec, SR, plog, t, pref, tref, "SW setcoef")
+        if plog > 4.56:
+            _close(rec[SR["SF"]], rec[SR["FF"]] * h2o, 1e-14, 0, "SW self factor")
+        else:
+            assert rec[SR["SF"]] == 0.0 and rec[SR["INDF"]] == 3
+
+
+# ---------------------------------------------------------------------------------------------
+# LW gas optics: band specifications (phys_rrtm_lw_taugbNN.cu)
+# ---------------------------------------------------------------------------------------------
+def _corr1_lower(p):
+    return 1.0 - 0.15 * (250.0 - p) / 154.4 if p < 250.0 else 1.0
+
+
+# per band (1-based): lower, upper; ("one", key) or ("eta", a, b, rate m, mult) or ("zero",)
+LW_SPEC = {
+    1: (("one", H2O), ("one", H2O)), 2: (("one", H2O), ("one", H2O)),
+    3: (("eta", H2O, CO2, 0, 8), ("eta", H2O, CO2, 0, 4)), 4: (("eta", H2O, CO2, 0, 8), ("eta", O3, CO2, 5, 4)),
+    5: (("eta", H2O, CO2, 0, 8), ("eta", O3, CO2, 5, 4)), 6: (("one", H2O), ("zero",)),
+    7: (("eta", H2O, O3, 1, 8), ("one", O3)), 8: (("one", H2O), ("one", O3)),
+    9: (("eta", H2O, CH4, 3, 8), ("one", CH4)), 10: (("one", H2O), ("one", H2O)),
+    11: (("one", H2O), ("one", H2O)), 12: (("eta", H2O, CO2, 0, 8), ("zero",)),
+    13: (("eta", H2O, N2O, 2, 8), ("zero",)), 14: (("one", CO2), ("one", CO2)),
+    15: (("eta", N2O, CO2, 4, 8), ("zero",)), 16: (("eta", H2O, CH4, 3, 8), ("one16", CH4)),
+}
+LW_UPPER_FOR = {1, 2, 3, 10, 11}                         # bands whose upper atmosphere has a foreign continuum
+LW_GFAC = {4: {7: 0.92, 8: 0.88, 9: 1.07, 10: 1.1, 11: 0.99, 12: 0.88, 13: 0.943},
+           7: {5: 0.92, 6: 0.88, 7: 1.07, 8: 1.1, 9: 0.99, 10: 0.855}}
+
+
+def lw_corr(band, lower, p):
+    if band == 1:
+        return _corr1_lower(p) if lower else 1.0 - 0.15 * (p / 95.6)
+    if band == 2 and lower:
+        return 1.0 - 0.05 * (p - 100.0) / 900.0
+    return 1.0
+
+
+def lw_band(L, rec, b):
+    tau = np.zeros(16)
+    fr = np.zeros(16)
+    assert L.oracle_lw_band_layer(_dp(rec), b - 1, _dp(tau), _dp(fr)) == 0
+    ng = LW_NG[b - 1]
+    return tau[:ng], fr[:ng]
+
+
+def _lw_rec(lower, jp, jt, jt1, p, rng):
+    r = np.zeros(LR["N"])
+    r[LR["LOWER"]] = 1.0 if lower else 0.0
+    r[LR["JP"]], r[LR["JT"]], r[LR["JT1"]] = jp, jt, jt1
+    r[LR["INDS"]], r[LR["INDF"]], r[LR["INDM"]] = (int(rng.integers(1, 10)), int(rng.integers(1, 3)), int(rng.integers(1, 19))) \
+        if lower else (0, 3, int(rng.integers(1, 19)))
+    r[LR["COLDRY"]] = 1.0e30                             # keeps the minor-gas ratios (ca / (coldry chi)) tiny
+    r[LR["PAVEL"]] = p
+    for m in range(6):                                   # distinct rates, different on the two pressure sides
+        r[LR["RF"] + 2 * m] = 0.7 + 0.11 * m
+        r[LR["RF"] + 2 * m + 1] = 1.9 + 0.13 * m
+    return r
+
+
+def _lw_node_rows(spec, lower, jp, jt, jt1, corner):
+    """0-based table row the corner weight selects at an eta node js (returned without js)."""
+    l0 = (jp - 1) * 5 + (jt - 1) if lower else (jp - 13) * 5 + (jt - 1)
+    l1 = jp * 5 + (jt1 - 1) if lower else (jp - 12) * 5 + (jt1 - 1)
+    return {"F00": l0, "F10": l0 + 1, "F01": l1, "F11": l1 + 1}[corner]
+
+
+def check_lw_major_nodes(L, seed=3):
+    """tau = amount x one absa/absb entry at every corner, single-key and eta-node (js = 1, 5) cases."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    for b, sides in LW_SPEC.items():
+        for lower in (1, 0):
+            spec = sides[0 if lower else 1]
+            for corner in CORNERS:
+                for trial in range(3):
+                    jp = int(rng.integers(1, 13)) if lower else int(rng.integers(13, 59))
+                    jt, jt1 = int(rng.integers(1, 5)), int(rng.integers(1, 5))
+                    p = rng.uniform(100.0, 1000.0) if lower else rng.uniform(1.0, 95.0)
+                    rec = _lw_rec(lower, jp, jt, jt1, p, rng)
+                    for c in CORNERS:
+                        rec[LR[c]] = 1.0 if c == corner else 0.0
+                    side = 0 if corner in ("F00", "F10") else 1
+                    ng = LW_NG[b - 1]
+                    tabname = f"lw/b{b:02d}/" + ("absa" if lower else "absb")
+                    corr = lw_corr(b, lower, p)
+                    kind = spec[0]
+                    if kind == "zero":
+                        rec[LR["CA"]:LR["CA"] + 7] = 1.0
+                        rec[LR["CA"] + O3] = 0.0                 # band 13's upper O3 minor off
+                        tau, fr = lw_band(L, rec, b)
+                        _close(tau, 0.0, 0, 0, f"LW band {b} upper is zero")
+                        continue
+                    row = _lw_node_rows(spec, lower, jp, jt, jt1, corner)
+                    if kind in ("one", "one16"):
+                        amt = rng.uniform(0.5, 50.0)
+                        rec[LR["CA"] + spec[1]] = amt
+                        if kind == "one16":                      # nspb(16) = 0: rows 1 and 2 (taugb16.cu:272-283)
+                            row = 0 if corner in ("F00", "F01") else 1
+                        expect = corr * amt * T[tabname][row]
+                    else:
+                        _, a, bb, m, mult = spec
+                        nsp = 9 if lower else 5
+                        rate = rec[LR["RF"] + 2 * m + side]
+                        amt_b = rng.uniform(0.5, 50.0)
+                        if trial < 2:                             # eta node js = 1 (species a absent)
+                            rec[LR["CA"] + bb] = amt_b
+                            js, speccomb = 1, rate * amt_b
+                        else:                                     # eta node at specparm = 1/2
+                            rec[LR["CA"] + bb] = amt_b
+                            rec[LR["CA"] + a] = rate * amt_b
+                            js, speccomb = mult // 2 + 1, 2.0 * rate * amt_b
+                        expect = speccomb * T[tabname][row * nsp + js - 1]
+                    if b in LW_GFAC and not lower:
+                        expect = expect * np.array([LW_GFAC[b].get(g, 1.0) for g in range(ng)])
+                    tau, fr = lw_band(L, rec, b)
+                    _close(tau, expect, 1e-12, 0, f"LW band {b} {'lower' if lower else 'upper'} {corner} node")
+                    # Planck fractions: the first eta row at js = 1, or the band's single row
+                    if kind != "eta" or trial < 2:
+                        if lower or b == 6:
+                            fa = T[f"lw/b{b:02d}/fracrefa"]
+                        else:
+                            fa = T[f"lw/b{b:02d}/fracrefb"]
+                        _close(fr, fa.reshape(-1, ng)[0], 1e-14, 0, f"LW band {b} Planck fraction node")
+
+
+def check_lw_continua(L, seed=4):
+    """self and foreign continua: amounts x the selfref/forref node (lower), forref only (upper)."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    for b, sides in LW_SPEC.items():
+        for lower in (1, 0):
+            spec = sides[0 if lower else 1]
+            if spec[0] == "zero":
+                continue
+            jp = int(rng.integers(1, 13)) if lower else int(rng.integers(13, 59))
+            p = rng.uniform(100.0, 1000.0) if lower else rng.uniform(1.0, 95.0)
+            rec = _lw_rec(lower, jp, 2, 3, p, rng)
+            rec[LR["F00"]] = 1.0
+            key = spec[1] if spec[0] != "eta" else spec[2]
+            rec[LR["CA"] + key] = 1e-40                           # major term negligible
+            sf, ff = rng.uniform(1e3, 1e4), rng.uniform(1e3, 1e4)
+            rec[LR["SF"]], rec[LR["FF"]] = sf, ff
+            inds, indf = int(rec[LR["INDS"]]), int(rec[LR["INDF"]])
+            tau, _ = lw_band(L, rec, b)
+            corr = lw_corr(b, lower, p)
+            fref = T[f"lw/b{b:02d}/forref"][indf - 1]
+            if lower:
+                expect = corr * (sf * T[f"lw/b{b:02d}/selfref"][inds - 1] + ff * fref)
+            else:
+                expect = corr * ff * fref if b in LW_UPPER_FOR else np.zeros_like(tau)
+            if b in LW_GFAC and not lower:
+                expect = expect * np.array([LW_GFAC[b].get(g, 1.0) for g in range(LW_NG[b - 1])])
+            _close(tau, expect, 1e-10, 1e-25, f"LW band {b} {'lower' if lower else 'upper'} continua")
+            # interpolation in temperature: selffrac / forfrac = 1 moves to the next row
+            rec[LR["SFR"]], rec[LR["FFR"]] = 1.0, 1.0
+            tau1, _ = lw_band(L, rec, b)
+            fref1 = T[f"lw/b{b:02d}/forref"][indf]
+            if lower:
+                expect1 = corr * (sf * T[f"lw/b{b:02d}/selfref"][inds] + ff * fref1)
+            else:
+                expect1 = corr * ff * fref1 if b in LW_UPPER_FOR else np.zeros_like(tau)
+            if b in LW_GFAC and not lower:
+                expect1 = expect1 * np.array([LW_GFAC[b].get(g, 1.0) for g in range(LW_NG[b - 1])])
+            _close(tau1, expect1, 1e-10, 1e-25, f"LW band {b} continua at the next temperature node")
+
+
+# minor gases and cross sections: (band, lower, record fields set, table, how the amount is formed)
+def check_lw_minor_gases(L, seed=5):
+    """single-table minor gases (N2 of band 1, CO2/O3/N2O of 6 and 8, O2 of 11, O3 of 13) and the
+    CFC/CCl4 cross sections at the minor-temperature node."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    cases = [
+        (1, 1, "n2", "ka_mn2"), (1, 0, "n2", "kb_mn2"),
+        (6, 1, CO2, "ka_mco2"), (8, 1, CO2, "ka_mco2"), (8, 1, O3, "ka_mo3"), (8, 1, N2O, "ka_mn2o"),
+        (8, 0, CO2, "kb_mco2"), (8, 0, N2O, "kb_mn2o"),
+        (11, 1, "o2", "ka_mo2"), (11, 0, "o2", "kb_mo2"), (13, 0, O3, "kb_mo3"),
+    ]
+    for b, lower, what, tab in cases:
+        for mf in (0.0, 1.0):
+            jp = int(rng.integers(1, 13)) if lower else int(rng.integers(13, 59))
+            p = rng.uniform(300.0, 1000.0) if lower else rng.uniform(1.0, 95.0)
+            rec = _lw_rec(lower, jp, 2, 3, p, rng)
+            rec[LR["F00"]] = 1.0
+            spec = LW_SPEC[b][0 if lower else 1]
+            if spec[0] in ("one", "one16"):
+                rec[LR["CA"] + spec[1]] = 0.0
+            rec[LR["MF"]] = mf
+            indm = int(rec[LR["INDM"]]) + (1 if mf == 1.0 else 0)
+            if what == "n2":
+                rec[LR["COLBRD"]], rec[LR["SCMN2"]] = 30.0, 0.02
+                amt = 30.0 * 0.02
+            elif what == "o2":
+                rec[LR["CA"] + O2], rec[LR["SCM"]] = 40.0, 0.3
+                amt = 40.0 * 0.3
+            else:
+                amt = rng.uniform(0.01, 2.0)                      # adjusted amount = amount (ratio to coldry chi tiny)
+                rec[LR["CA"] + what] = amt
+            if spec[0] in ("one", "one16") and spec[1] == what:
+                continue
+            tau, _ = lw_band(L, rec, b)
+            expect = lw_corr(b, lower, p) * amt * T[f"lw/b{b:02d}/{tab}"][indm - 1]
+            _close(tau, expect, 1e-12, 1e-30, f"LW band {b} minor {tab} (minorfrac {mf})")
+    xs = [(5, 1, 0, "ccl4"), (5, 0, 0, "ccl4"), (6, 1, 1, "cfc11adj"), (6, 1, 2, "cfc12"), (6, 0, 1, "cfc11adj"),
+          (6, 0, 2, "cfc12"), (8, 1, 2, "cfc12"), (8, 1, 3, "cfc22adj"), (8, 0, 2, "cfc12"), (8, 0, 3, "cfc22adj")]
+    for b, lower, w, tab in xs:
+        jp = 5 if lower else 30
+        rec = _lw_rec(lower, jp, 2, 3, 500.0 if lower else 50.0, rng)
+        rec[LR["F00"]] = 1.0
+        spec = LW_SPEC[b][0 if lower else 1]
+        if spec[0] == "eta":
+            rec[LR["CA"] + spec[2]] = 1e-40
+        amt = rng.uniform(1e-3, 1.0)
+        rec[LR["WX"] + w] = amt
+        tau, _ = lw_band(L, rec, b)
+        _close(tau, amt * T[f"lw/b{b:02d}/{tab}"], 1e-10, 1e-28, f"LW band {b} cross section {tab}")
+
+
+# ---------------------------------------------------------------------------------------------
+# SW gas optics (phys_rrtm_sw_taumolNN.cu)
+# ---------------------------------------------------------------------------------------------
+def _strrat(b):
+    """strrat(16:29) of radsw_datatb_nmmb.f:1108-1110, single precision in the reference (the band
+    kernels take the float32 values, e.g. phys_rrtm_sw_taumol19.cu:80 5.49281e+0f)."""
+    return float(pyoracle.tables()["sw/strrat"][b - 16])
+
+
+# lower, upper: ("eta", a, b, ratio, mult, nrow) / ("one", key, nrow, factor) / ("vec", species, table) / ("zero",)
+def sw_spec():
+    return {
+        16: (("eta", H2O, CH4, _strrat(16), 8), ("one", CH4, 1.0)),
+        17: (("eta", H2O, CO2, _strrat(17), 8), ("eta", H2O, CO2, _strrat(17), 4)),
+        18: (("eta", H2O, CH4, _strrat(18), 8), ("one", CH4, 1.0)),
+        19: (("eta", H2O, CO2, _strrat(19), 8), ("one", CO2, 1.0)),
+        20: (("one", H2O, 1.0), ("one", H2O, 1.0)),
+        21: (("eta", H2O, CO2, _strrat(21), 8), ("eta", H2O, CO2, _strrat(21), 4)),
+        22: (("eta", H2O, O2, _strrat(22), 8), ("one", O2, 1.6)),
+        23: (("one", H2O, 1.029), ("zero",)),
+        24: (("eta", H2O, O2, _strrat(24), 8), ("one", O2, 1.0)),
+        25: (("one", H2O, 1.0), ("vec", O3, "abso3b")),
+        26: (("zero",), ("zero",)),
+        27: (("one", O3, 1.0), ("one", O3, 1.0)),
+        28: (("eta", O3, O2, _strrat(28), 8), ("eta", O3, O2, _strrat(28), 4)),
+        29: (("one", H2O, 1.0), ("one", CO2, 1.0)),
+    }
+
+
+# Rayleigh: per band the coefficient the reference kernels use ("t" table scalar, "v" table vector,
+# or the literal of the kernel)
+SW_RAYL = {16: "t", 17: 6.86e-10, 18: "t", 19: 2.29e-09, 20: "t", 21: 9.41e-09, 22: 1.54e-08, 23: "v", 25: "v",
+           26: "v", 27: "v", 28: "t", 29: 9.30e-11}
+SW_LOWER_SELF = {16, 17, 18, 19, 20, 21, 22, 23, 24, 29}
+SW_UPPER_FOR = {17, 20, 21}
+O2CONT = 4.35e-4 / (350.0 * 2.0)                        # taumol22.cu o2 continuum per O2 amount
+
+
+def sw_band(L, rec, b):
+    tg = np.zeros(16)
+    tr = np.zeros(16)
+    assert L.oracle_sw_band_layer(_dp(rec), b - 16, _dp(tg), _dp(tr)) == 0
+    ng = SW_NG[b - 16]
+    return tg[:ng], tr[:ng]
+
+
+def _sw_rec(lower, jp, jt, jt1, rng):
+    r = np.zeros(SR["N"])
+    r[SR["LAYTROP"]] = 2 if lower else 0
+    r[SR["JP"]], r[SR["JT"]], r[SR["JT1"]] = jp, jt, jt1
+    r[SR["INDS"]], r[SR["INDF"]] = (int(rng.integers(1, 10)), int(rng.integers(1, 3))) if lower else (0, 3)
+    return r
+
+
+def check_sw_major_nodes(L, seed=6):
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    for b, sides in sw_spec().items():
+        ng = SW_NG[b - 16]
+        for lower in (1, 0):
+            spec = sides[0 if lower else 1]
+            for corner in CORNERS:
+                for trial in range(3):
+                    jp = int(rng.integers(1, 13)) if lower else int(rng.integers(13, 59))
+                    jt, jt1 = int(rng.integers(1, 4)), int(rng.integers(1, 4))
+                    rec = _sw_rec(lower, jp, jt, jt1, rng)
+                    for c in CORNERS:
+                        rec[SR[c]] = 1.0 if c == corner else 0.0
+                    cm = rng.uniform(100.0, 3000.0)
+                    rec[SR["COLMOL"]] = cm
+                    kind = spec[0]
+                    tab = T.get(f"sw/b{b}/" + ("absa" if lower else "absb"))
+                    row = _lw_node_rows(spec, lower, jp, jt, jt1, corner)
+                    extra = 0.0
+                    if kind == "zero":
+                        rec[SR["CA"]:SR["CA"] + 6] = 1.0
+                        expect = np.zeros(ng)
+                    elif kind == "vec":
+                        amt = rng.uniform(0.5, 50.0)
+                        rec[SR["CA"] + spec[1]] = amt
+                        expect = amt * T[f"sw/b{b}/{spec[2]}"]
+                    elif kind == "one":
+                        amt = rng.uniform(0.5, 50.0)
+                        rec[SR["CA"] + spec[1]] = amt
+                        expect = spec[2] * amt * tab[row]
+                        if b == 22:
+                            extra = O2CONT * amt
+                    else:
+                        _, a, bb, ratio, mult = spec
+                        nsp = 9 if lower else 5
+                        amt_b = rng.uniform(0.5, 50.0)
+                        rec[SR["CA"] + bb] = amt_b
+                        if trial < 2:
+                            js, speccomb = 1, ratio * amt_b
+                        else:
+                            rec[SR["CA"] + a] = ratio * amt_b
+                            js, speccomb = mult // 2 + 1, 2.0 * ratio * amt_b
+                        expect = speccomb * tab[row * nsp + js - 1]
+                        if b == 22:
+                            extra = O2CONT * amt_b
+                    tg, tr = sw_band(L, rec, b)
+                    _close(tg, expect + extra, 1e-12, 0, f"SW band {b} {'lower' if lower else 'upper'} {corner} node")
+                    # Rayleigh: colmol x coefficient (band 24: eta-interpolated rayla, raylb aloft)
+                    if b == 24:
+                        if lower:
+                            if trial < 2:
+                                _close(tr, cm * T["sw/b24/rayla"][0], 1e-14, 0, "SW band 24 Rayleigh node")
+                        else:
+                            _close(tr, cm * T["sw/b24/raylb"], 1e-14, 0, "SW band 24 Rayleigh aloft")
+                    else:
+                        r = SW_RAYL[b]
+                        coef = T[f"sw/b{b}/rayl"] if r in ("t", "v") else r
+                        _close(tr, cm * np.broadcast_to(coef, (ng,)), 1e-14, 0, f"SW band {b} Rayleigh")
+
+
+def check_sw_continua_and_terms(L, seed=7):
+    """SW continua (amount x (selffac selfref + forfac forref)) and the single-vector terms of bands
+    20 (CH4), 24/25 (O3) and 29 (CO2 below, H2O above), each as a difference of two evaluations."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    for b, sides in sw_spec().items():
+        for lower in (1, 0):
+            jp = int(rng.integers(1, 13)) if lower else int(rng.integers(13, 59))
+            rec = _sw_rec(lower, jp, 2, 2, rng)
+            rec[SR["F00"]] = 1.0
+            rec[SR["COLMOL"]] = 500.0
+            rec[SR["CA"]:SR["CA"] + 6] = 1.0
+            w = rng.uniform(0.5, 5.0)
+            rec[SR["CA"] + H2O] = w
+            base, _ = sw_band(L, rec, b)
+            sf, ff = rng.uniform(1e3, 1e4), rng.uniform(1e3, 1e4)
+            rec[SR["SF"]], rec[SR["FF"]] = sf, ff
+            tg, _ = sw_band(L, rec, b)
+            inds, indf = int(rec[SR["INDS"]]), int(rec[SR["INDF"]])
+            if lower and b in SW_LOWER_SELF:
+                expect = w * (sf * T[f"sw/b{b}/selfref"][inds - 1] + ff * T[f"sw/b{b}/forref"][indf - 1])
+            elif not lower and b in SW_UPPER_FOR:
+                expect = w * ff * T[f"sw/b{b}/forref"][indf - 1]
+            else:
+                expect = np.zeros_like(tg)
+            _close(tg - base, expect, 1e-8, 1e-12 * float(np.max(np.abs(tg))), f"SW band {b} continua")
+    terms = [(20, 1, CH4, "absch4"), (20, 0, CH4, "absch4"), (24, 1, O3, "abso3a"), (24, 0, O3, "abso3b"),
+             (25, 1, O3, "abso3a"), (29, 1, CO2, "absco2"), (29, 0, H2O, "absh2o")]
+    for b, lower, sp, tab in terms:
+        rec = _sw_rec(lower, 5 if lower else 30, 2, 2, rng)
+        rec[SR["F00"]] = 1.0
+        rec[SR["COLMOL"]] = 500.0
+        rec[SR["CA"]:SR["CA"] + 6] = 1.0
+        rec[SR["CA"] + sp] = 0.0
+        base, _ = sw_band(L, rec, b)
+        amt = rng.uniform(1.0, 10.0)
+        rec[SR["CA"] + sp] = amt
+        tg, _ = sw_band(L, rec, b)
+        _close(tg - base, amt * T[f"sw/b{b}/{tab}"], 1e-9, 1e-14 * float(np.max(np.abs(tg))), f"SW band {b} {tab}")
+
+
+# ---------------------------------------------------------------------------------------------
+# cloud optics
+# ---------------------------------------------------------------------------------------------
+def check_lw_cloud_nodes(L, seed=8):
+    """phys_rrtm_lw_cldprop.cu (ilwcliq = 1, ilwcice = 1): liquid at the radius nodes re = 1.5 + n,
+    ice a + b / re with re clamped to [13, 130], rain 0.33e-3 per g/m2."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    absliq, absice, ipat = T["lw/absliq1"], T["lw/absice1"], T["lw/ipat"]
+    out = np.zeros(16)
+    for n in range(1, 58):
+        liq = rng.uniform(1.0, 300.0)
+        assert L.oracle_lw_cldprop_layer(0.5, liq, 0.0, 1.5 + n, 50.0, 0.0, _dp(out)) == 0
+        _close(out, liq * absliq[:, n - 1], 1e-14, 0, f"LW liquid node {n}")
+    for re in (5.0, 13.0, 20.0, 57.3, 130.0, 200.0):
+        ice = rng.uniform(1.0, 100.0)
+        assert L.oracle_lw_cldprop_layer(0.5, 0.0, ice, 10.0, re, 0.0, _dp(out)) == 0
+        r = min(130.0, max(13.0, re))
+        _close(out, ice * (absice[ipat - 1, 0] + absice[ipat - 1, 1] / r), 1e-14, 0, f"LW ice re {re}")
+    assert L.oracle_lw_cldprop_layer(0.5, 0.0, 0.0, 10.0, 50.0, 7.0, _dp(out)) == 0
+    _close(out, np.full(16, 0.33e-3 * 7.0), 1e-14, 0, "LW rain")
+    assert L.oracle_lw_cldprop_layer(0.0, 10.0, 10.0, 10.0, 50.0, 7.0, _dp(out)) == 0
+    _close(out, 0.0, 0, 0, "LW clear layer")
+
+
+def check_sw_cloud_nodes(L, seed=9):
+    """phys_rrtm_sw_cldprop.cu (iswcliq = 1, iswcice = 3): tau, tau w0, tau w0 g at the liquid radius
+    nodes and the Fu ice generalized-size nodes d = 2 + 3 n (d = 1.0315 re)."""
+    T = pyoracle.tables()
+    rng = np.random.default_rng(seed)
+    ta, sa, aa = np.zeros(14), np.zeros(14), np.zeros(14)
+    for n in range(1, 58):
+        liq = rng.uniform(1.0, 300.0)
+        assert L.oracle_sw_cldprop_layer(0.5, liq, 1.5 + n, 0.0, 50.0, 0.0, _dp(ta), _dp(sa), _dp(aa)) == 0
+        tau = liq * T["sw/extliq1"][:, n - 1]
+        ssa = tau * T["sw/ssaliq1"][:, n - 1]
+        _close(ta, tau, 1e-14, 0, f"SW liquid tau node {n}")
+        _close(sa, ssa, 1e-14, 0, f"SW liquid ssa node {n}")
+        _close(aa, ssa * T["sw/asyliq1"][:, n - 1], 1e-14, 0, f"SW liquid asy node {n}")
+    for n in range(2, 45):
+        ice = rng.uniform(1.0, 100.0)
+        re = (2.0 + 3.0 * n) / 1.0315
+        assert L.oracle_sw_cldprop_layer(0.5, 0.0, 10.0, ice, re, 0.0, _dp(ta), _dp(sa), _dp(aa)) == 0
+        tau = ice * T["sw/extice3"][:, n - 1]
+        ssa = tau * T["sw/ssaice3"][:, n - 1]
+        _close(ta, tau, 1e-9, 0, f"SW ice tau node {n}")
+        _close(sa, ssa, 1e-9, 0, f"SW ice ssa node {n}")
+        _close(aa, ssa * T["sw/asyice3"][:, n - 1], 1e-9, 0, f"SW ice asy node {n}")
+
+
+# ---------------------------------------------------------------------------------------------
+# SW two-stream
+# ---------------------------------------------------------------------------------------------
+def layer_opt(L, tau, ssa, asy, cz):
+    out = np.zeros(5)
+    assert L.oracle_sw_layer_opt(tau, ssa, asy, cz, _dp(out)) == 0
+    return out
+
+
+def gpoint(L, cz, albb, albd, tr, tg, zc=None, tc=None, sc=None, ac=None):
+    n = len(tr)
+    z = np.zeros(n)
+    arr = [np.ascontiguousarray(x if x is not None else z, dtype=np.float64) for x in (tr, tg, zc, tc, sc, ac)]
+    f = np.zeros(4 * (n + 1))
+    cloudy = 1 if zc is not None else 0
+    assert L.oracle_sw_gpoint(n, cz, albb, albd, *[_dp(a) for a in arr[:2]], cloudy, *[_dp(a) for a in arr[2:]],
+                              _dp(f)) == 0
+    return f.reshape(4, n + 1)            # clear up, clear down, total up, total down; index 0 = surface
+
+
+def check_sw_exponential(L):
+    """spcvrtc's exp: 1 - z + z^2/2 up to 0.06, then the rswinit table at the rounded index."""
+    for z in np.linspace(0.0, 0.06, 25):
+        _close(L.oracle_sw_expm(float(z), 0), 1.0 - z + 0.5 * z * z, 1e-15, 0, "SW exp Taylor branch")
+    for z in np.concatenate([np.linspace(0.0601, 2.0, 200), np.linspace(2.0, 10.0, 100)]):
+        # table spacing 1e-4 in tau / (bpade + tau): relative error <= (bpade + z)^2 / bpade * 0.5e-4
+        bound = (1.0 / 0.278 + z) ** 2 * 0.278 * 0.5e-4 * 1.05
+        for cl in (0, 1):
+            _close(L.oracle_sw_expm(float(z), cl), math.exp(-z), bound, 0, f"SW exp table z {z}")
+
+
+def check_sw_two_stream(L, seed=10):
+    rng = np.random.default_rng(seed)
+    # thin layer: R -> 0, T -> 1
+    for cz in (0.1, 0.5, 1.0):
+        rb, rd, tb, td, t1 = layer_opt(L, 1e-9, 0.5e-9, 0.1e-9, cz)
+        assert abs(rb) < 1e-7 and abs(rd) < 1e-7 and abs(tb - 1.0) < 1e-7 and abs(td - 1.0) < 1e-7, "thin layer"
+    # non-absorbing layer (w0 = 1, clamped to 1 - 1e-6): R + T = 1 for beam and diffuse incidence,
+    # which holds only with the PIFM gamma1 = gamma2 at w0 = 1
+    for _ in range(40):
+        tau, g, cz = rng.uniform(0.05, 5.0), rng.uniform(0.0, 0.9), rng.uniform(0.05, 1.0)
+        rb, rd, tb, td, t1 = layer_opt(L, tau, tau, tau * g, cz)
+        lim = 3e-5 * (1.0 + tau / cz)
+        _close(rb + tb, 1.0, 0, lim, f"beam R + T = 1 (tau {tau:.3f}, g {g:.2f}, mu {cz:.2f})")
+        _close(rd + td, 1.0, 0, lim, f"diffuse R + T = 1 (tau {tau:.3f}, g {g:.2f})")
+        # delta scaling: tau' = (1 - w g^2) tau
+        w = 1.0 - 1e-6
+        _close(t1, (1.0 - w * g * g) * tau, 1e-12, 0, "delta-scaled optical depth")
+    # purely absorbing column over a black surface: Beer-Lambert, nothing goes up
+    n = 12
+    for cz in (0.2, 0.6, 1.0):
+        tg = rng.uniform(0.0, 0.2, n)
+        f = gpoint(L, cz, 0.0, 0.0, np.zeros(n), tg)
+        z = tg.sum() / cz
+        bound = (1.0 / 0.278 + 0.2 / cz) ** 2 * 0.278 * 0.5e-4 * 1.05 * n + 1e-6
+        _close(f[1, 0], math.exp(-z), bound, 0, "direct beam at the surface = exp(-tau / mu)")
+        _close(f[0], 0.0, 0, 1e-12, "no upward flux without scattering or surface reflection")
+        _close(f[1, n], 1.0, 1e-15, 0, "unit incident flux")
+    # non-absorbing column (Rayleigh only, conservative clouds) over a black surface: what enters at
+    # the top leaves at the top or is absorbed by the surface
+    for trial in range(8):
+        cz = rng.uniform(0.1, 1.0)
+        tr = rng.uniform(0.0, 0.3, n)
+        zc = (rng.uniform(0, 1, n) < 0.4) * rng.uniform(0.2, 1.0, n)
+        tc = rng.uniform(0.5, 10.0, n)
+        ac = tc * rng.uniform(0.7, 0.9, n)
+        f = gpoint(L, cz, 0.0, 0.0, tr, np.zeros(n), zc, tc, tc, ac)
+        tot = tr.sum() + tc.sum()
+        lim = 2e-5 * (1.0 + tot / cz)
+        _close(f[1, n] - f[0, n] - f[1, 0], 0.0, 0, lim, "clear column conserves energy")
+        _close(f[3, n] - f[2, n] - f[3, 0], 0.0, 0, lim, "cloudy column conserves energy")
+        # white surface: everything comes back out at the top
+        f = gpoint(L, cz, 1.0, 1.0, tr, np.zeros(n), zc, tc, tc, ac)
+        _close(f[0, n], 1.0, 0, lim * 4, "white surface: clear column reflects all")
+        _close(f[2, n], 1.0, 0, lim * 4, "white surface: cloudy column reflects all")
+
+
+ALL = [check_lw_setcoef, check_sw_setcoef, check_lw_major_nodes, check_lw_continua, check_lw_minor_gases,
+       check_sw_major_nodes, check_sw_continua_and_terms, check_lw_cloud_nodes, check_sw_cloud_nodes,
+       check_sw_exponential, check_sw_two_stream]

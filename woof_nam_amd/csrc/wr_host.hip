@@ -268,13 +268,13 @@ template <typename R> int bind_tables(const TFile &f, Blob &b, Tabs<R> &T)
             B.rayl_v = b.conv<R>(*ry);
         B.strrat = (R)((const float *)srat->data)[k];
     }
-    // strrat(16)/(28) come with their band files (kgb16/kgb28); strrat(18) is a literal
-    // (phys_rrtm_sw_taumol18.py:43)
+    // strrat(16)/(28) come with their band files (kgb16/kgb28).  Every strrat is the single-precision
+    // value of the reference (radsw_datatb_nmmb.f:1108-1110 data; phys_rrtm_sw_taumol18.py:42
+    // np.float32(3.89589e+1), phys_rrtm_sw_taumol19.cu:80 5.49281e+0f, ...), as the table holds it.
     bool ok16 = true, ok28 = true;
     const double s16 = d.scalar("sw/b16/strrat16", &ok16), s28 = d.scalar("sw/b28/strrat28", &ok28);
     if (ok16) T.sw[0].strrat = (R)s16;
     if (ok28) T.sw[12].strrat = (R)s28;
-    T.sw[2].strrat = (R)3.89589e+1;
     return b.bad ? WR_ERR_TABLES : WR_OK;
 }
 

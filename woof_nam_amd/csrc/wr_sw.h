@@ -198,7 +198,7 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
     } else if constexpr (B == 3) { /* band 19 */
         for (int g = 0; g < ng; ++g) tr[g] = cm * (R)2.29e-09;
         if (kk <= Lt) {
-            const R speccomb = ca[0] + (R)5.49281e+0 * ca[1];
+            const R speccomb = ca[0] + Bt.strrat * ca[1];
             const int js = jsfs<R>((R)8 * fmin((R)0.999999, ca[0] / speccomb), fs);
             fac8<R>(fs, f00, f01, f10, f11, fc);
             ind8(id0, id1, js, 585, 9, 10, id);
@@ -221,7 +221,7 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
         }
     } else if constexpr (B == 5) { /* band 21 */
         for (int g = 0; g < ng; ++g) tr[g] = cm * (R)9.41e-09;
-        const R speccomb = ca[0] + (R)4.53210e-03 * ca[1];
+        const R speccomb = ca[0] + Bt.strrat * ca[1];
         if (kk < Lt) {
             const int js = jsfs<R>((R)8 * fmin((R)ONEMINUS, ca[0] / speccomb), fs);
             fac8<R>(fs, f00, f01, f10, f11, fc);
@@ -238,7 +238,7 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
         for (int g = 0; g < ng; ++g) tr[g] = cm * (R)1.54e-08;
         const R o2cont = (R)(4.35e-4 / (350.0 * 2.0)) * ca[5];
         if (kk <= Lt) {
-            const R speccomb = ca[0] + (R)3.63328e-02 * ca[5];
+            const R speccomb = ca[0] + Bt.strrat * ca[5];
             const R ratio = (speccomb > (R)0) ? (ca[0] / speccomb) : (R)0;
             const int js = jsfs<R>((R)8 * fmin((R)ONEMINUS, ratio), fs);
             fac8<R>(fs, f00, f01, f10, f11, fc);
@@ -258,7 +258,7 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
         }
     } else if constexpr (B == 8) { /* band 24 */
         if (kk <= Lt) {
-            const R speccomb = ca[0] + (R)1.24692e-01 * ca[5];
+            const R speccomb = ca[0] + Bt.strrat * ca[5];
             const int js = jsfs<R>((R)8 * fmin((R)ONEMINUS, ca[0] / speccomb), fs);
             fac8<R>(fs, f00, f01, f10, f11, fc);
             ind8(id0, id1, js, 585, 9, 10, id);
