@@ -115,6 +115,12 @@ void wr_default_params(wr_params *p);
  * allocates the work space for tiles of up to `max_cols` columns.  No later call allocates. */
 int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols);
 
+/* Device bytes wr_create allocates for a context of `max_cols` columns (tables, tile work space and
+ * the band streams' level pools, under the same WR_POOL_GB / WR_BAND_STREAMS settings), computed on
+ * the host without allocating: callers size max_cols from free device memory with it, as
+ * namgpu/physics/rrtm/rrtm_call.py:183-203 (batch_count) sizes its batches.  0 for max_cols <= 0. */
+uint64_t wr_context_bytes(int32_t max_cols);
+
 /* Plain FP64 radiation call on device pointers.  Columns beyond max_cols run in tiles. */
 int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
                 const wr_diag *diag, void *stream);

@@ -108,3 +108,61 @@ def test_autotune_picks_a_measured_shape(gpu_device):
     assert len(r["all"]) == 4
     assert all(x["columns_per_s"] > 0 for x in r["all"])
     assert r["best"] == max(r["all"], key=lambda x: x["columns_per_s"])
+
+
+def grid_dev(nx, ny, seed, dx_km, device):
+    c = gen.grid(nx, ny, seed=seed, dx_km=dx_km)
+    return {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in c.items()}, c
+
+
+def test_config3_request_kinds_mix(gpu_device):
+    """The request kinds of the served workload at reduced size: a parent-grid sample (12 km, seed 0),
+    a 3 km and a 6 km nest, and the 8 ensemble members of the parent (McICA seeds 1..8), mixed
+    precision, arriving while earlier ones run.  Every result equals a direct call; the direct FP64
+    call of a member equals the CPU oracle on a column subsample, and the member's mixed result is
+    within the error bars of the FP64 one."""
+    print("case serve config-3 kinds", flush=True)
+    from oracle import pyoracle
+    parent, parent_h = grid_dev(120, 100, 0, 12.0, gpu_device)
+    nest3, _ = grid_dev(70, 60, 11, 3.0, gpu_device)
+    nest6, _ = grid_dev(50, 40, 12, 6.0, gpu_device)
+    pid = torch.arange(parent["t_lay"].shape[1], dtype=torch.int64, device=gpu_device)
+    n3 = torch.arange(10_000_000, 10_000_000 + nest3["t_lay"].shape[1], dtype=torch.int64, device=gpu_device)
+    n6 = torch.arange(20_000_000, 20_000_000 + nest6["t_lay"].shape[1], dtype=torch.int64, device=gpu_device)
+    kinds = [("parent", parent, pid, 0), ("nest3km", nest3, n3, 0), ("nest6km", nest6, n6, 0)]
+    kinds += [("member", parent, pid, m) for m in range(1, 9)]
+    srv = Server(0, tile=4096, slots=3, precision="mixed")
+    reqs = [Request(rid=j, kind=k, cols=c, col_id=ids, seed=sd) for j, (k, c, ids, sd) in enumerate(kinds)]
+    for r in reqs[:4]:
+        srv.submit(r)
+    for _ in range(3):                     # the rest arrive while the first tiles run
+        srv.poll()
+    for r in reqs[4:]:
+        srv.submit(r)
+    srv.drain()
+    torch.cuda.synchronize()
+    rad = Radiation(0, max_cols=16384)
+    for r in reqs:
+        direct, _ = rad.run({**r.cols, "col_id": r.col_id}, params=Radiation.default_params(seed=r.seed),
+                            precision="mixed")
+        torch.cuda.synchronize()
+        assert same(r.out, direct), f"request {r.rid} ({r.kind}, seed {r.seed}) differs from the direct call"
+    # member 3: FP64 direct call against the oracle on a subsample, and the mixed error bars
+    m3 = next(r for r in reqs if r.kind == "member" and r.seed == 3)
+    f64, _ = rad.run({**parent, "col_id": pid}, params=Radiation.default_params(seed=3), precision="fp64")
+    torch.cuda.synchronize()
+    idx = np.arange(0, parent_h["t_lay"].shape[1], 20)
+    sub = {k: np.ascontiguousarray(v[..., idx]) for k, v in parent_h.items()}
+    sub["col_id"] = idx.astype(np.uint64)
+    ref, _ = pyoracle.run(sub, params=Radiation.default_params(seed=3))
+    for k in abi.OUTPUT_FIELDS:
+        a = f64[k].cpu().numpy()[..., idx]
+        assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), f"FP64 {k} differs from the oracle"
+    for k in ("hr_lw", "hr_sw"):
+        rms = float(torch.sqrt(torch.mean((m3.out[k] - f64[k]) ** 2))) * 86400.0
+        assert rms <= 0.05, f"{k} RMS {rms:.3g} K/day"
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        rms = float(torch.sqrt(torch.mean((m3.out[k] - f64[k]) ** 2)))
+        assert rms <= 0.5, f"{k} RMS {rms:.3g} W/m2"
+    rad.close()
+    srv.close()

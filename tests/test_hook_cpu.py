@@ -105,3 +105,31 @@ def test_columns_rejects_bad_rows_and_shapes():
         rrtm_call.columns({**args, "J_S": JM + 3}, CPU)
     with pytest.raises(ValueError):
         rrtm_call.columns({**args, "PD": args["PD"][:-1]}, CPU)
+
+
+def test_tile_sized_from_free_memory():
+    """The hook's context tile fits the free device memory beside the call's own column arrays
+    (the reference sizes its batches from free bytes, rrtm_call.py:183-203)."""
+    from woof_nam_amd import _lib
+    L = _lib.load()
+    ncol, lm = 796590, 60
+    per_call = rrtm_call.call_bytes_per_column(lm) * ncol
+    whole = L.wr_context_bytes(ncol)
+    assert whole > 20e9                                 # tens of KB per column of work space
+    assert rrtm_call.tile_columns(ncol, lm, int((whole + per_call) / 0.9) + 1) == ncol
+    free = int(0.5 * (whole + per_call))                # half of what the whole grid needs
+    m = rrtm_call.tile_columns(ncol, lm, free)
+    assert 4096 <= m < ncol
+    assert L.wr_context_bytes(m) + per_call <= 0.9 * free
+    assert L.wr_context_bytes(m + 2048) + per_call > 0.9 * free   # within the bisection step
+    with pytest.raises(MemoryError):
+        rrtm_call.tile_columns(ncol, lm, int(per_call / 0.9))
+    assert L.wr_context_bytes(0) == 0
+
+
+def test_mcica_route_refused_by_the_hook():
+    ctx = make_ctx()
+    ctx.cfg.rrtm_mcica = 1
+    args = rad_driver.rrtm_args(ctx, ctx.tables["JDAT"], 3, 60.0, rad_driver.MP_FLAGS["gfs"], 1361.0)
+    with pytest.raises(NotImplementedError, match="RSWINC"):
+        rrtm_call.run(ctx, args)

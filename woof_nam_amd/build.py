@@ -6,10 +6,17 @@ gfx950 and links ``woof_nam_amd/libwoofrad.so``.  Flags:
 * ``-ffp-contract=off``: no FMA contraction, so the FP64 path evaluates exactly the operation
   sequence written (table indices and McICA bits are reproducible bit for bit);
 * ``-Xclang -target-feature -Xclang -packed-fp32-ops``: no packed FP32 instructions.
+
+Every kernel's private segment (register spills and private arrays, "ScratchSize") is read from the
+compiler's resource-usage remarks and the build fails when one exceeds ``SCRATCH_MAX`` bytes per lane:
+the runtime backs the private segment of every queue that runs a kernel, for every lane in flight,
+and large per-lane arrays have exhausted it (an HSA out-of-resources queue abort).  The per-kernel
+table is written to ``_build/resource_usage.txt``.
 """
 from __future__ import annotations
 
 import os
+import re
 import shutil
 import subprocess
 import sys
@@ -49,12 +56,34 @@ UNIT_FLAGS = {
 }
 
 
-def _compile(src: Path, obj: Path) -> None:
+# Largest private segment a kernel may use, bytes per lane (the two-wave band kernels spill up to
+# about 1.1 KB; the private arrays behind the queue aborts were 13-45 KB)
+SCRATCH_MAX = int(os.environ.get("WR_SCRATCH_MAX", "1536"))
+_REMARK = re.compile(r"remark:\s+(Function Name|VGPRs|AGPRs|ScratchSize \[bytes/lane\]|Occupancy \[waves/SIMD\]):\s*(\S+)")
+
+
+def resource_usage(stderr: str) -> list:
+    """[(kernel, vgprs, agprs, scratch bytes/lane, waves/SIMD)] from -Rpass-analysis=kernel-resource-usage."""
+    rows, cur = [], None
+    for m in _REMARK.finditer(stderr):
+        key, val = m.group(1), m.group(2)
+        if key == "Function Name":
+            cur = {"name": val}
+            rows.append(cur)
+        elif cur is not None:
+            cur[key.split()[0]] = int(val)
+    return [(r["name"], r.get("VGPRs", 0), r.get("AGPRs", 0), r.get("ScratchSize", 0), r.get("Occupancy", 0))
+            for r in rows]
+
+
+def _compile(src: Path, obj: Path) -> list:
     extra = os.environ.get("WR_CFLAGS_EXTRA", "").split()      # tuning builds, e.g. -DWR_BAND_WAVES=4
-    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), *extra, "-c", str(src), "-o", str(obj)]
+    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), *extra, "-Rpass-analysis=kernel-resource-usage",
+           "-c", str(src), "-o", str(obj)]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")
+    return [(src.name, *row) for row in resource_usage(r.stderr)]
 
 
 def build(verbose: bool = False, out: Path | None = None) -> Path:
@@ -63,7 +92,15 @@ def build(verbose: bool = False, out: Path | None = None) -> Path:
     objdir.mkdir(exist_ok=True)
     objs = [objdir / (Path(u).stem + ".o") for u in UNITS]
     with ThreadPoolExecutor(max_workers=min(len(UNITS), 8)) as ex:
-        list(ex.map(lambda p: _compile(CSRC / p[0], p[1]), zip(UNITS, objs)))
+        usage = [row for rows in ex.map(lambda p: _compile(CSRC / p[0], p[1]), zip(UNITS, objs)) for row in rows]
+    with open(objdir / "resource_usage.txt", "w") as f:
+        f.write("unit kernel vgprs agprs scratch_bytes_per_lane waves_per_simd\n")
+        for row in usage:
+            f.write(" ".join(map(str, row)) + "\n")
+    over = [row for row in usage if row[4] > SCRATCH_MAX]
+    if over:
+        raise RuntimeError("private segment above %d bytes/lane: %s" % (
+            SCRATCH_MAX, "; ".join(f"{u}:{k} {sc} B" for u, k, _, _, sc, _ in over)))
     tmp = out.with_suffix(".so.tmp")
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", str(tmp), *map(str, objs)]
     r = subprocess.run(cmd, capture_output=True, text=True)

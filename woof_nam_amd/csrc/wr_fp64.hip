@@ -41,3 +41,8 @@ int wr_launch_fp64(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs 
     if (par.mcica == 0) return launch_tile_nam<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
     return launch_tile<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
 }
+
+int wr_launch_diag_fp64(wr_ctx *c, const ColIn &in, const RunPar &par, const wr_diag *dg, int col0, int n, hipStream_t s)
+{
+    return launch_diag<double>(c, c->tabs_d, in, par, dg, col0, n, s);
+}

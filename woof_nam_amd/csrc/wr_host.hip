@@ -301,63 +301,6 @@ __global__ void k_d2f(const double *__restrict__ a, float *__restrict__ b, int n
     if (j < n) b[j] = (float)a[j];
 }
 
-// ---- diagnostics: setcoef indices and McICA words of one tile ---------------------------------
-__global__ void __launch_bounds__(64) k_diag(const Tabs<double> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                             ColScal cs, wr_diag dg)
-{
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= valid_cols(in, col0, n)) return;
-    const int i = col0 + t;
-    const RadCol rc{in, i, in.nlay + 1};
-    const int nrl = rc.nrl;
-    const size_t nc = (size_t)in.ncol;
-    const int day = cs.day[t];
-    for (int k = 0; k < nrl; ++k) {
-        const int kin = nrl - 1 - k;
-        const double la = log_d(rc.plyr(kin));
-        LwLay<double> L;
-        lw_layer<double>(T, rc, par, kin, la, L);
-        if (dg.lw_jp) dg.lw_jp[k * nc + i] = L.jp;
-        if (dg.lw_jt) dg.lw_jt[k * nc + i] = L.jt;
-        if (dg.lw_jt1) dg.lw_jt1[k * nc + i] = L.jt1;
-        SwLay<double> S;
-        if (day) sw_layer<double>(T, rc, par, kin, la, S);
-        if (dg.sw_jp) dg.sw_jp[k * nc + i] = day ? S.jp : -1;
-        if (dg.sw_jt) dg.sw_jt[k * nc + i] = day ? S.jt : -1;
-        if (dg.sw_jt1) dg.sw_jt1[k * nc + i] = day ? S.jt1 : -1;
-    }
-    if (dg.lw_laytrop) dg.lw_laytrop[i] = cs.lw_laytrop[t];
-    if (dg.sw_laytrop) dg.sw_laytrop[i] = day ? cs.sw_laytrop[t] : -1;
-    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
-    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
-    if (dg.lw_mask) {
-        const int lcf1 = cs.lcf1[t];
-        for (int k = 0; k < nrl; ++k)
-            for (int w = 0; w < 5; ++w) dg.lw_mask[((size_t)k * 5 + w) * nc + i] = 0u;
-        for (int g = 0; g < (par.mcica ? 140 : 0); ++g) {
-            const uint64_t mb = mcica_g(col, seed, 0, g, nrl, par.iovr,
-                                        [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
-            for (int k = 0; k < nrl; ++k)
-                if ((mb >> k) & 1) dg.lw_mask[((size_t)k * 5 + (g >> 5)) * nc + i] |= 1u << (g & 31);
-        }
-    }
-    if (dg.sw_mask) {
-        for (int k = 0; k < nrl; ++k)
-            for (int w = 0; w < 4; ++w) dg.sw_mask[((size_t)k * 4 + w) * nc + i] = 0u;
-        if (day && par.mcica) {
-            const double cf1 = cs.zcf1[t];
-            for (int g = 0; g < 112; ++g) {
-                const uint64_t mb = mcica_g(col, seed, 1, g, nrl, par.iovr, [&](int k) {
-                    const double ct = rc.cfrac(nrl - 1 - k);
-                    return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
-                });
-                for (int k = 0; k < nrl; ++k)
-                    if ((mb >> k) & 1) dg.sw_mask[((size_t)k * 4 + (g >> 5)) * nc + i] |= 1u << (g & 31);
-            }
-        }
-    }
-}
-
 // ---- CAL_CLDFRA (phys_rad_cal_cldfra.cu:29-55) ----------------------------------------------------
 __global__ void k_cal_cldfra(int im, int jm, int lm, int i_lo, int i_hi, int j_lo, int j_hi, int f_qc, int f_qi,
                              float *__restrict__ cldfra, const float *__restrict__ qc, const float *__restrict__ qi)
@@ -481,13 +424,48 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
             rc = wr_launch_fp64(c, ci, par, op, col0, n, band, s);
         if (rc != WR_OK) return rc;
         if (dg) {
-            tbeg(c, KID_DIAG, s);
-            hipLaunchKernelGGL(k_diag, dim3((n + 63) / 64), dim3(64), 0, s, c->tabs_d, ci, par, col0, n, colscal(c), *dg);
-            tend(c, s);
-            if (hipGetLastError() != hipSuccess) return WR_ERR_HIP;
+            // indices and bits from the layer setup of the precision that ran (the FP32 path's own
+            // instantiation for mixed calls), with the layer logs k_colprep left for this tile
+            rc = mode == MODE_MIXED ? wr_launch_diag_mixed(c, ci, par, dg, col0, n, s)
+                                    : wr_launch_diag_fp64(c, ci, par, dg, col0, n, s);
+            if (rc != WR_OK) return rc;
         }
     }
     return WR_OK;
+}
+
+// Device bytes of a context's tile work space for tiles of m columns, and the band-stream count.
+struct Sizes {
+    size_t acc, iws, dws, rtr, sflux_d, sflux_f, pool;
+    int naux;
+    size_t total() const { return acc + iws + dws + rtr + sflux_d + sflux_f + pool * (size_t)(naux > 1 ? naux : 1); }
+};
+Sizes sizes(int32_t max_cols)
+{
+    const size_t m = (size_t)max_cols;
+    Sizes z;
+    z.acc = sizeof(double) * (60 * (NL + 1) + 14) * m;                        // band flux planes
+    z.iws = sizeof(int) * 7 * m;                                              // 6 column words + day-list segments
+    z.dws = sizeof(double) * (4 + 2 * (size_t)NL) * m;                        // column words, layer logs, tz
+    z.rtr = (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m;      // NAM-route overlap factors
+    z.sflux_d = sizeof(double) * 112 * m;
+    z.sflux_f = sizeof(float) * 112 * m;
+    // level pool: room for every 64-column block of a tile in the largest slot layout (FP64 SW:
+    // 2 values, FP32 SW: 7 values, x 16 g x levels), at most WR_POOL_GB (default 6) GB per stream
+    const int nl_max = 64;
+    const size_t lane_max = std::max(cache_slot_bytes(SW_NV, 16, nl_max + 1, (int)sizeof(double)),
+                                     cache_slot_bytes(sw_nv_mc<float>(), 16, nl_max + 1, (int)sizeof(float)));
+    const size_t need = (m + 63) / 64 * lane_max;
+    const char *pe = getenv("WR_POOL_GB");
+    const double gb = pe ? atof(pe) : 6.0;
+    size_t cap = (size_t)(gb * 1e9);
+    if (cap < lane_max) cap = lane_max;
+    z.pool = need < cap ? need : cap;
+    // WR_BAND_STREAMS=0 keeps the band kernels on the caller's stream
+    const char *e = getenv("WR_BAND_STREAMS");
+    const int na = e ? atoi(e) : WR_NAUX;
+    z.naux = na < 0 ? 0 : (na > WR_NAUX ? WR_NAUX : na);
+    return z;
 }
 
 }  // namespace
@@ -528,38 +506,29 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
             hipLaunchKernelGGL(k_d2f, dim3((NTBL + 256) / 256), dim3(256), 0, nullptr, src[q], dst[q], NTBL + 1);
         if (hipGetLastError() != hipSuccess) rc = WR_ERR_HIP;
     }
-    const size_t m = (size_t)max_cols;
+    const Sizes z = sizes(max_cols);
     if (rc == WR_OK) {
-        if (hipMalloc((void **)&c->acc, sizeof(double) * (60 * (NL + 1) + 14) * m) != hipSuccess ||
-            hipMalloc((void **)&c->iws, sizeof(int) * 7 * m) != hipSuccess ||  // 6 column words + day-list segments
-            hipMalloc((void **)&c->dws, sizeof(double) * (4 + 2 * (size_t)NL) * m) != hipSuccess ||
-            hipMalloc(&c->rtr, (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m) != hipSuccess ||
-            hipMalloc((void **)&c->sflux_d, sizeof(double) * 112 * m) != hipSuccess ||
-            hipMalloc((void **)&c->sflux_f, sizeof(float) * 112 * m) != hipSuccess)
+        if (hipMalloc((void **)&c->acc, z.acc) != hipSuccess || hipMalloc((void **)&c->iws, z.iws) != hipSuccess ||
+            hipMalloc((void **)&c->dws, z.dws) != hipSuccess || hipMalloc(&c->rtr, z.rtr) != hipSuccess ||
+            hipMalloc((void **)&c->sflux_d, z.sflux_d) != hipSuccess ||
+            hipMalloc((void **)&c->sflux_f, z.sflux_f) != hipSuccess)
             rc = WR_ERR_NOMEM;
     }
     if (rc == WR_OK) {
-        // level pool: room for every 64-column block of a tile in the largest slot layout (FP64 SW:
-        // 2 values, FP32 SW: 7 values, x 16 g x levels), at most WR_POOL_GB (default 6) GB per stream
-        const size_t lane_max = std::max(cache_slot_bytes(SW_NV, 16, c->nl_max + 1, (int)sizeof(double)),
-                                         cache_slot_bytes(sw_nv_mc<float>(), 16, c->nl_max + 1, (int)sizeof(float)));
-        const size_t need = (m + 63) / 64 * lane_max;
-        const char *pe = getenv("WR_POOL_GB");
-        const double gb = pe ? atof(pe) : 6.0;
-        size_t cap = (size_t)(gb * 1e9);
-        if (cap < lane_max) cap = lane_max;
-        c->pool_bytes = need < cap ? need : cap;
+        c->pool_bytes = z.pool;
         if (hipMalloc((void **)&c->pool[0], c->pool_bytes) != hipSuccess) rc = WR_ERR_NOMEM;
     }
     if (rc == WR_OK) {
         // WR_BAND_STREAMS=0 keeps the band kernels on the caller's stream
-        const char *e = getenv("WR_BAND_STREAMS");
-        int na = e ? atoi(e) : WR_NAUX;
-        c->naux = na < 0 ? 0 : (na > WR_NAUX ? WR_NAUX : na);
+        c->naux = z.naux;
         for (int q = 0; q < c->naux && rc == WR_OK; ++q)
             if (hipStreamCreateWithFlags(&c->aux[q], hipStreamNonBlocking) != hipSuccess ||
                 hipEventCreateWithFlags(&c->join[q], hipEventDisableTiming) != hipSuccess)
                 rc = WR_ERR_HIP;
+        // one level pool per band stream (stream 0 shares the caller's), all allocated here: no later
+        // call allocates
+        for (int q = 1; q < c->naux && rc == WR_OK; ++q)
+            if (hipMalloc((void **)&c->pool[q], c->pool_bytes) != hipSuccess) rc = WR_ERR_NOMEM;
         if (rc == WR_OK && c->naux > 0 && hipEventCreateWithFlags(&c->fork, hipEventDisableTiming) != hipSuccess)
             rc = WR_ERR_HIP;
     }
@@ -570,6 +539,12 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
     }
     *out = c;
     return WR_OK;
+}
+
+uint64_t wr_context_bytes(int32_t max_cols)
+{
+    if (max_cols <= 0) return 0;
+    return (uint64_t)sizes(max_cols).total() + WR_TABLE_BYTES;
 }
 
 int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, const wr_diag *diag,

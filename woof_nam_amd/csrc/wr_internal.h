@@ -6,6 +6,9 @@
 #include "wr_kernels.h"
 
 #define WR_NAUX 4  // auxiliary streams the band kernels of a tile are spread over
+// upper bound of the device memory of the two table blobs (FP64 and FP32 tables, about 3.5 MB with
+// their look-up tables) and their structs
+#define WR_TABLE_BYTES (8ull << 20)
 
 struct wr_ctx {
     int device;
@@ -30,7 +33,7 @@ struct wr_ctx {
     void *sh1, *sh2;                 // hidden activations, bfloat16 [S][128]
     int *sflag;                      // [max_cols] per-column table-optics flag
     // per-stream cache pools of the band kernels (pool[0]: the caller's stream and aux stream 0;
-    // pool[q]: aux stream q, allocated on first use), pool_bytes each
+    // pool[q]: aux stream q), pool_bytes each, all allocated by wr_create
     char *pool[WR_NAUX];
     size_t pool_bytes;
     // band-kernel streams (default priority, non-blocking) and their fork/join events
@@ -87,8 +90,10 @@ int wr_launch_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const 
                    int band, hipStream_t s);
 int wr_launch_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
                     int band, hipStream_t s);
-int wr_launch_diag(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
-                   hipStream_t s);
+int wr_launch_diag_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
+                        hipStream_t s);
+int wr_launch_diag_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
+                         hipStream_t s);
 int wr_init_luts(wr_ctx *c, hipStream_t s);
 int wr_launch_surr(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
                    int32_t *flag_out, hipStream_t s);

@@ -127,25 +127,15 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
     }
 }
 
-// True when the band kernels may leave stream s: auxiliary streams exist and every one has its
-// cache pool.  Pools are allocated here on first use outside graph capture; under capture the fork
-// and join events bring the auxiliary streams into the graph (a captured tile then holds the same
-// band concurrency as an eager one), or the tile stays on s when a pool is missing.
+// True when the band kernels may leave stream s: the context has auxiliary streams (their pools are
+// allocated by wr_create).  Under graph capture the fork and join events bring the auxiliary streams
+// into the graph, so a captured tile holds the same band concurrency as an eager one.
 inline bool spread_bands(wr_ctx *c, hipStream_t s)
 {
+    (void)s;
     if (c->naux <= 0) return false;
-    hipStreamCaptureStatus st = hipStreamCaptureStatusNone;
-    if (hipStreamIsCapturing(s, &st) != hipSuccess) return false;
-    const bool capturing = st != hipStreamCaptureStatusNone;
-    for (int q = 1; q < c->naux; ++q) {
-        if (c->pool[q]) continue;
-        if (capturing) return false;
-        if (hipMalloc((void **)&c->pool[q], c->pool_bytes) != hipSuccess) {
-            c->pool[q] = nullptr;
-            (void)hipGetLastError();
-            return false;
-        }
-    }
+    for (int q = 1; q < c->naux; ++q)
+        if (!c->pool[q]) return false;
     return true;
 }
 
@@ -226,6 +216,77 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
     hipLaunchKernelGGL(k_post<R>, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
+    tend(c, s);
+    return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
+}
+
+// ---- diagnostics: setcoef indices and McICA words of one tile ---------------------------------
+// Runs after the tile, on its stream.  The layer setup is the instantiation of the precision that ran
+// (lw_layer<R>, sw_layer<R>: the code of that path's band kernels) on the layer log pressures that
+// k_colprep left in the work space for this tile.
+template <typename R>
+__global__ void __launch_bounds__(64) k_diag(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
+                                             ColScal cs, wr_diag dg)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= valid_cols(in, col0, n)) return;
+    const int i = col0 + t;
+    const RadCol rc{in, i, in.nlay + 1};
+    const int nrl = rc.nrl;
+    const size_t nc = (size_t)in.ncol;
+    const int day = cs.day[t];
+    for (int k = 0; k < nrl; ++k) {
+        const int kin = nrl - 1 - k;
+        const double la = cs.la[(size_t)kin * n + t];
+        LwLay<R> L;
+        lw_layer<R>(T, rc, par, kin, la, L);
+        if (dg.lw_jp) dg.lw_jp[k * nc + i] = L.jp;
+        if (dg.lw_jt) dg.lw_jt[k * nc + i] = L.jt;
+        if (dg.lw_jt1) dg.lw_jt1[k * nc + i] = L.jt1;
+        SwLay<R> S;
+        if (day) sw_layer<R>(T, rc, par, kin, la, S);
+        if (dg.sw_jp) dg.sw_jp[k * nc + i] = day ? S.jp : -1;
+        if (dg.sw_jt) dg.sw_jt[k * nc + i] = day ? S.jt : -1;
+        if (dg.sw_jt1) dg.sw_jt1[k * nc + i] = day ? S.jt1 : -1;
+    }
+    if (dg.lw_laytrop) dg.lw_laytrop[i] = cs.lw_laytrop[t];
+    if (dg.sw_laytrop) dg.sw_laytrop[i] = day ? cs.sw_laytrop[t] : -1;
+    const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
+    const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
+    if (dg.lw_mask) {
+        const int lcf1 = cs.lcf1[t];
+        for (int k = 0; k < nrl; ++k)
+            for (int w = 0; w < 5; ++w) dg.lw_mask[((size_t)k * 5 + w) * nc + i] = 0u;
+        for (int g = 0; g < (par.mcica ? 140 : 0); ++g) {
+            const uint64_t mb = mcica_g(col, seed, 0, g, nrl, par.iovr,
+                                        [&](int k) { return lcf1 ? rc.cfrac(nrl - 1 - k) : 0.0; });
+            for (int k = 0; k < nrl; ++k)
+                if ((mb >> k) & 1) dg.lw_mask[((size_t)k * 5 + (g >> 5)) * nc + i] |= 1u << (g & 31);
+        }
+    }
+    if (dg.sw_mask) {
+        for (int k = 0; k < nrl; ++k)
+            for (int w = 0; w < 4; ++w) dg.sw_mask[((size_t)k * 4 + w) * nc + i] = 0u;
+        if (day && par.mcica) {
+            const double cf1 = cs.zcf1[t];
+            for (int g = 0; g < 112; ++g) {
+                const uint64_t mb = mcica_g(col, seed, 1, g, nrl, par.iovr, [&](int k) {
+                    const double ct = rc.cfrac(nrl - 1 - k);
+                    return (cf1 > 0.0) ? ((ct < FTINY) ? 0.0 : ct) : 0.0;
+                });
+                for (int k = 0; k < nrl; ++k)
+                    if ((mb >> k) & 1) dg.sw_mask[((size_t)k * 4 + (g >> 5)) * nc + i] |= 1u << (g & 31);
+            }
+        }
+    }
+}
+
+template <typename R>
+int launch_diag(wr_ctx *c, const Tabs<R> *tabs, const ColIn &in, const RunPar &par, const wr_diag *dg, int col0, int n,
+                hipStream_t s)
+{
+    tbeg(c, KID_DIAG, s);
+    hipLaunchKernelGGL(k_diag<R>, dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, colscal(c), *dg);
     tend(c, s);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }

@@ -13,3 +13,8 @@ int wr_launch_mixed(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs
     if (par.mcica == 0) return launch_tile_nam<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
     return launch_tile<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
 }
+
+int wr_launch_diag_mixed(wr_ctx *c, const ColIn &in, const RunPar &par, const wr_diag *dg, int col0, int n, hipStream_t s)
+{
+    return launch_diag<float>(c, c->tabs_f, in, par, dg, col0, n, s);
+}

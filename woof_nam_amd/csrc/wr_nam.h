@@ -25,6 +25,15 @@ __device__ __forceinline__ double lw_cldfrc(const RadCol &rc, int kk)
     return rc.cfrac(rc.nrl - kk);
 }
 
+// The six factors of one index while the sweep that sets them is at it.
+struct Fac6 {
+    double cld1 = 0.0, cld2 = 0.0, clr1 = 0.0, clr2 = 0.0, cmb1 = 0.0, cmb2 = 0.0;
+};
+
+// Step k of a sweep sets the factors of the next index (k + 1 upward, k - 1 downward) and may clear
+// clr2/cld2 of index k itself; no later step touches index k.  So the sweeps keep two indices in
+// registers (cur = k, nxt) and three cloud fractions (cfm, cfk, cfp), and write each index as soon as
+// its step is done: no per-lane arrays, no private memory.
 template <typename R>
 __global__ void __launch_bounds__(64) k_rtr_fac(ColIn in, int col0, int n, Rtr<R> rt)
 {
@@ -34,119 +43,105 @@ __global__ void __launch_bounds__(64) k_rtr_fac(ColIn in, int col0, int n, Rtr<R
     const RadCol rc{in, col0 + t, in.nlay + 1};
     const int nl = rc.nrl;
     const double eps = 1.0e-6;
-    double cf[NL + 3];
-    for (int k = 0; k <= nl + 1; ++k) cf[k] = lw_cldfrc(rc, k);
-    double cld1[NL + 3], cld2[NL + 3], clr1[NL + 3], clr2[NL + 3], cmb1[NL + 3], cmb2[NL + 3];
-    uint64_t lst = 0;
-    // upward factors
-    for (int k = 0; k <= nl + 2; ++k) cld1[k] = cld2[k] = clr1[k] = clr2[k] = cmb1[k] = cmb2[k] = 0.0;
-    bool lk = cf[1] > eps;  // lstu[k]
-    if (lk) lst |= 1ull;
-    double rat1 = 0.0, rat2 = 0.0;
-    for (int k = 1; k <= nl - 1; ++k) {
-        const bool lk1 = cf[k + 1] > eps && cf[k] <= eps;
-        if (lk1) lst |= 1ull << k;
-        if (cf[k] > eps) {
-            if (cf[k + 1] >= cf[k]) {
-                if (lk) {
-                    if (cf[k] < 1.0) clr2[k + 1] = (cf[k + 1] - cf[k]) / (1.0 - cf[k]);
-                    clr2[k] = 0.0;
-                    cld2[k] = 0.0;
-                } else {
-                    const double fmx = fmax(cf[k], cf[k - 1]);
-                    if (cf[k + 1] > fmx) {
-                        clr1[k + 1] = rat2;
-                        clr2[k + 1] = (cf[k + 1] - fmx) / (1.0 - fmx);
-                    } else if (cf[k + 1] < fmx) {
-                        clr1[k + 1] = (cf[k + 1] - cf[k]) / (cf[k - 1] - cf[k]);
-                    } else {
-                        clr1[k + 1] = rat2;
-                    }
-                }
-                if (clr1[k + 1] > 0.0 || clr2[k + 1] > 0.0) rat1 = 1.0, rat2 = 0.0;
-                else rat1 = 0.0, rat2 = 0.0;
+    auto put = [&](int base, int idx, const Fac6 &f) {
+        rt.at(base + 0, idx, t) = (R)f.cld1, rt.at(base + 1, idx, t) = (R)f.cld2, rt.at(base + 2, idx, t) = (R)f.clr1;
+        rt.at(base + 3, idx, t) = (R)f.clr2, rt.at(base + 4, idx, t) = (R)f.cmb1, rt.at(base + 5, idx, t) = (R)f.cmb2;
+    };
+    // one step of either sweep: k's own fraction cfk, the one it moves to (cfn) and the one it came
+    // from (cfo); cur = index k, nxt = the index the step sets
+    auto step = [&](double cfo, double cfk, double cfn, bool lk, double &rat1, double &rat2, Fac6 &cur, Fac6 &nxt) {
+        if (!(cfk > eps)) return;
+        if (cfn >= cfk) {
+            if (lk) {
+                if (cfk < 1.0) nxt.clr2 = (cfn - cfk) / (1.0 - cfk);
+                cur.clr2 = 0.0;
+                cur.cld2 = 0.0;
             } else {
-                if (lk) {
-                    cld2[k + 1] = (cf[k] - cf[k + 1]) / cf[k];
-                    clr2[k] = 0.0;
-                    cld2[k] = 0.0;
+                const double fmx = fmax(cfk, cfo);
+                if (cfn > fmx) {
+                    nxt.clr1 = rat2;
+                    nxt.clr2 = (cfn - fmx) / (1.0 - fmx);
+                } else if (cfn < fmx) {
+                    nxt.clr1 = (cfn - cfk) / (cfo - cfk);
                 } else {
-                    const double fmn = fmin(cf[k], cf[k - 1]);
-                    if (cf[k + 1] <= fmn) {
-                        cld1[k + 1] = rat1;
-                        cld2[k + 1] = (fmn - cf[k + 1]) / fmn;
-                    } else {
-                        cld1[k + 1] = (cf[k] - cf[k + 1]) / (cf[k] - fmn);
-                    }
+                    nxt.clr1 = rat2;
                 }
-                if (cld1[k + 1] > 0.0 || cld2[k + 1] > 0.0) rat1 = 0.0, rat2 = 1.0;
-                else rat1 = 0.0, rat2 = 0.0;
             }
-            cmb1[k + 1] = clr1[k + 1] * cld2[k] * cf[k - 1];
-            cmb2[k + 1] = cld1[k + 1] * clr2[k] * (1.0 - cf[k - 1]);
-        }
-        lk = lk1;
-    }
-    for (int k = 0; k <= nl + 2; ++k) {
-        rt.at(FCLD1U, k, t) = (R)cld1[k], rt.at(FCLD2U, k, t) = (R)cld2[k], rt.at(FCLR1U, k, t) = (R)clr1[k];
-        rt.at(FCLR2U, k, t) = (R)clr2[k], rt.at(FCMB1U, k, t) = (R)cmb1[k], rt.at(FCMB2U, k, t) = (R)cmb2[k];
-    }
-    rt.lstu[t] = lst;
-    // downward factors
-    for (int k = 0; k <= nl + 2; ++k) cld1[k] = cld2[k] = clr1[k] = clr2[k] = cmb1[k] = cmb2[k] = 0.0;
-    lst = 0;
-    lk = cf[nl] > eps;  // lstd[k]
-    if (lk) lst |= 1ull << (nl - 1);
-    rat1 = 0.0, rat2 = 0.0;
-    for (int k = nl; k >= 2; --k) {
-        const bool lkm = cf[k - 1] > eps && cf[k] <= eps;
-        if (lkm) lst |= 1ull << (k - 2);
-        if (cf[k] > eps) {
-            if (cf[k - 1] >= cf[k]) {
-                if (lk) {
-                    if (cf[k] < 1.0) clr2[k - 1] = (cf[k - 1] - cf[k]) / (1.0 - cf[k]);
-                    clr2[k] = 0.0;
-                    cld2[k] = 0.0;
-                } else {
-                    const double fmx = fmax(cf[k], cf[k + 1]);
-                    if (cf[k - 1] > fmx) {
-                        clr1[k - 1] = rat2;
-                        clr2[k - 1] = (cf[k - 1] - fmx) / (1.0 - fmx);
-                    } else if (cf[k - 1] < fmx) {
-                        clr1[k - 1] = (cf[k - 1] - cf[k]) / (cf[k + 1] - cf[k]);
-                    } else {
-                        clr1[k - 1] = rat2;
-                    }
-                }
-                if (clr1[k - 1] > 0.0 || clr2[k - 1] > 0.0) rat1 = 1.0, rat2 = 0.0;
-                else rat1 = 0.0, rat2 = 0.0;
+            if (nxt.clr1 > 0.0 || nxt.clr2 > 0.0) rat1 = 1.0, rat2 = 0.0;
+            else rat1 = 0.0, rat2 = 0.0;
+        } else {
+            if (lk) {
+                nxt.cld2 = (cfk - cfn) / cfk;
+                cur.clr2 = 0.0;
+                cur.cld2 = 0.0;
             } else {
-                if (lk) {
-                    cld2[k - 1] = (cf[k] - cf[k - 1]) / cf[k];
-                    clr2[k] = 0.0;
-                    cld2[k] = 0.0;
+                const double fmn = fmin(cfk, cfo);
+                if (cfn <= fmn) {
+                    nxt.cld1 = rat1;
+                    nxt.cld2 = (fmn - cfn) / fmn;
                 } else {
-                    const double fmn = fmin(cf[k], cf[k + 1]);
-                    if (cf[k - 1] <= fmn) {
-                        cld1[k - 1] = rat1;
-                        cld2[k - 1] = (fmn - cf[k - 1]) / fmn;
-                    } else {
-                        cld1[k - 1] = (cf[k] - cf[k - 1]) / (cf[k] - fmn);
-                    }
+                    nxt.cld1 = (cfk - cfn) / (cfk - fmn);
                 }
-                if (cld1[k - 1] > 0.0 || cld2[k - 1] > 0.0) rat1 = 0.0, rat2 = 1.0;
-                else rat1 = 0.0, rat2 = 0.0;
             }
-            cmb1[k - 1] = clr1[k - 1] * cld2[k] * cf[k + 1];
-            cmb2[k - 1] = cld1[k - 1] * clr2[k] * (1.0 - cf[k + 1]);
+            if (nxt.cld1 > 0.0 || nxt.cld2 > 0.0) rat1 = 0.0, rat2 = 1.0;
+            else rat1 = 0.0, rat2 = 0.0;
         }
-        lk = lkm;
+        nxt.cmb1 = nxt.clr1 * cur.cld2 * cfo;
+        nxt.cmb2 = nxt.cld1 * cur.clr2 * (1.0 - cfo);
+    };
+    const Fac6 zero;
+    // upward factors: index 0 stays zero, step k = 1..nl-1 sets index k + 1
+    {
+        uint64_t lst = 0;
+        double cfm = lw_cldfrc(rc, 0), cfk = lw_cldfrc(rc, 1);
+        bool lk = cfk > eps;
+        if (lk) lst |= 1ull;
+        double rat1 = 0.0, rat2 = 0.0;
+        Fac6 cur;
+        put(FCLD1U, 0, zero);
+        for (int k = 1; k <= nl - 1; ++k) {
+            const double cfp = lw_cldfrc(rc, k + 1);
+            const bool lk1 = cfp > eps && cfk <= eps;
+            if (lk1) lst |= 1ull << k;
+            Fac6 nxt;
+            step(cfm, cfk, cfp, lk, rat1, rat2, cur, nxt);
+            put(FCLD1U, k, cur);
+            cur = nxt;
+            lk = lk1;
+            cfm = cfk;
+            cfk = cfp;
+        }
+        put(FCLD1U, nl, cur);
+        put(FCLD1U, nl + 1, zero);
+        put(FCLD1U, nl + 2, zero);
+        rt.lstu[t] = lst;
     }
-    for (int k = 0; k <= nl + 2; ++k) {
-        rt.at(FCLD1D, k, t) = (R)cld1[k], rt.at(FCLD2D, k, t) = (R)cld2[k], rt.at(FCLR1D, k, t) = (R)clr1[k];
-        rt.at(FCLR2D, k, t) = (R)clr2[k], rt.at(FCMB1D, k, t) = (R)cmb1[k], rt.at(FCMB2D, k, t) = (R)cmb2[k];
+    // downward factors: indices nl+1, nl+2 stay zero, step k = nl..2 sets index k - 1
+    {
+        uint64_t lst = 0;
+        double cfp = lw_cldfrc(rc, nl + 1), cfk = lw_cldfrc(rc, nl);
+        bool lk = cfk > eps;
+        if (lk) lst |= 1ull << (nl - 1);
+        double rat1 = 0.0, rat2 = 0.0;
+        Fac6 cur;
+        put(FCLD1D, nl + 2, zero);
+        put(FCLD1D, nl + 1, zero);
+        for (int k = nl; k >= 2; --k) {
+            const double cfm = lw_cldfrc(rc, k - 1);
+            const bool lkm = cfm > eps && cfk <= eps;
+            if (lkm) lst |= 1ull << (k - 2);
+            Fac6 nxt;
+            step(cfp, cfk, cfm, lk, rat1, rat2, cur, nxt);
+            put(FCLD1D, k, cur);
+            cur = nxt;
+            lk = lkm;
+            cfp = cfk;
+            cfk = cfm;
+        }
+        put(FCLD1D, 1, cur);
+        put(FCLD1D, 0, zero);
+        rt.lstd[t] = lst;
     }
-    rt.lstd[t] = lst;
 }
 
 // total-sky transmittance, source factor and absorptance of odtot (rtrnmr's cloudy branch)

@@ -27,7 +27,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from . import abi, solar
+from . import _lib, abi, solar
 from .radiation import Radiation
 
 KEY_CTX = "woof_nam_amd.radiation"
@@ -52,11 +52,43 @@ def _cfg(ctx, name, default):
     return getattr(cfg, name, default) if cfg is not None else default
 
 
-def _context(ctx, device, ncol):
+# device bytes per column the hook itself holds during a call besides the library context: the FP64
+# column inputs (6 layer fields + interfaces + 4 surface values) and outputs (2 heating rates, 4 flux
+# profiles, 1 surface value) at LM layers, with the progcld2 paths and radii (5 layer fields)
+def call_bytes_per_column(lm: int) -> int:
+    return 8 * ((6 + 5) * lm + (lm + 1) + 4 + 2 * lm + 4 * (lm + 2) + 1)
+
+
+def tile_columns(ncol: int, lm: int, free_bytes: int, cap: int = 1048576, margin: float = 0.9) -> int:
+    """Largest tile (columns per library context) that fits the free device memory beside the call's
+    own arrays, as namgpu/physics/rrtm/rrtm_call.py:183-203 (batch_count) sizes its batches from free
+    bytes.  Raises MemoryError when not even a 4,096-column tile fits."""
+    L = _lib.load()
+    avail = int(margin * free_bytes) - call_bytes_per_column(lm) * ncol
+    hi = max(1, min(ncol, cap))
+    if L.wr_context_bytes(hi) <= avail:
+        return hi
+    lo = min(4096, hi)
+    if L.wr_context_bytes(lo) > avail:
+        raise MemoryError(f"rrtm-call: {ncol} columns need {call_bytes_per_column(lm) * ncol / 1e9:.1f} GB of "
+                          f"column arrays and a {lo}-column context of {L.wr_context_bytes(lo) / 1e9:.1f} GB; "
+                          f"{free_bytes / 1e9:.1f} GB free")
+    while hi - lo > 1024:                       # context bytes grow with the tile: bisect
+        mid = (lo + hi) // 2
+        if L.wr_context_bytes(mid) <= avail:
+            lo = mid
+        else:
+            hi = mid
+    return lo
+
+
+def _context(ctx, device, ncol, lm):
     tables = ctx.tables
     rad = tables.get(KEY_CTX)
     if rad is None or rad.device != device:
-        rad = Radiation(device.index or 0, max_cols=min(max(ncol, 1), int(_cfg(ctx, "rrtm_tile", 1048576))))
+        free, _ = torch.cuda.mem_get_info(device)
+        m = tile_columns(max(ncol, 1), lm, free, cap=int(_cfg(ctx, "rrtm_tile", 1048576)))
+        rad = Radiation(device.index or 0, max_cols=m)
         tables[KEY_CTX] = rad
     return rad
 
@@ -117,6 +149,11 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
                                      "solcon": float(args["SOLCON"]), "sw_run": bool(sw and lsswr)}
     if not (lsswr or lslwr):
         return
+    mcica = int(_cfg(ctx, "rrtm_mcica", 0))
+    if mcica != 0:
+        # the McICA band kernels do not compute the clear-sky surface SW flux that RSWINC takes
+        raise NotImplementedError("rrtm-call: rrtm_mcica = 1 leaves RSWINC (clear-sky surface SW) uncomputed; "
+                                  "the NAM route (rrtm_mcica = 0, isubcsw = isubclw = 0) is the hook's route")
     if stamp is not None:
         stamp("driver-prep")
     T = _arr(args, "T")
@@ -126,9 +163,8 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     glon = _arr(args, "GLON", (jm, im)).reshape(-1)[c0:c1].to(device)
     cz = solar.coszen(args["JDAT"], nts * dt, nrads * dt, glat, glon)
     cols["cosz"] = cz if (sw and lsswr) else torch.full_like(cz, -1.0)
-    params = Radiation.default_params(mcica=int(_cfg(ctx, "rrtm_mcica", 0)), solcon=float(args["SOLCON"]),
-                                      seed=int(_cfg(ctx, "rrtm_seed", 0)))
-    rad = _context(ctx, device, c1 - c0)
+    params = Radiation.default_params(mcica=mcica, solcon=float(args["SOLCON"]), seed=int(_cfg(ctx, "rrtm_seed", 0)))
+    rad = _context(ctx, device, c1 - c0, lm)
     if stamp is not None:
         stamp("grrad")
     out, _ = rad.run(cols, params=params, precision=str(_cfg(ctx, "rrtm_precision", "fp64")))
