@@ -112,6 +112,7 @@ def test_autotune_picks_a_measured_shape(gpu_device):
 
 def grid_dev(nx, ny, seed, dx_km, device):
     c = gen.grid(nx, ny, seed=seed, dx_km=dx_km)
+    assert all(np.isfinite(v).all() for v in c.values()), "synthetic grid has non-finite values"
     return {k: torch.from_numpy(np.ascontiguousarray(v)).to(device) for k, v in c.items()}, c
 
 

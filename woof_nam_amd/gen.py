@@ -115,7 +115,9 @@ def _finish(p_lev, t, q, o3, cldf, qc, qi, rng, cosz):
 
 def _grf(rng, nx, ny, dx_km, scale_km, nfields):
     """Unit-variance Gaussian random fields [nfields, ny, nx] with a Gaussian spectrum of the given
-    length scale, drawn by FFT filtering of white noise."""
+    length scale, drawn by FFT filtering of white noise.  The scale is capped at a quarter of the
+    domain's shorter side: a longer one leaves only the mean, a field of zero variance."""
+    scale_km = min(scale_km, 0.25 * min(nx, ny) * dx_km)
     kx = np.fft.rfftfreq(nx, d=dx_km)
     ky = np.fft.fftfreq(ny, d=dx_km)
     k2 = ky[:, None] ** 2 + kx[None, :] ** 2
