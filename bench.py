@@ -6,7 +6,8 @@ resident in HBM.  Prints ONE JSON line (rank 0).  Also reported beside the throu
   * accuracy: heating-rate (K/day) and flux (W/m2) RMS against the CPU FP64 oracle on a sample;
   * roofline: the dominant kernel's algorithmic HBM bytes / its mean launch duration (HIP events
     on the launching stream, recorded by the library around every launch);
-  * cpu_baseline: the CPU FP64 oracle on a bounded sample of the same grid, on this host's cores.
+  * cpu_baseline: the CPU FP64 oracle on the 4,096-column CPU-reference workload, on this host's
+    cores (median of 5 after 1 warm-up), with nproc and the CPU model.
 """
 from __future__ import annotations
 
@@ -76,6 +77,55 @@ def kernel_bytes(name: str, ncol: int, nday: int) -> float:
     return 0.0
 
 
+# Dense vector peaks of MI355X (TFLOP/s): FP64 78.6, FP32 157.3
+VALU_PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3}
+
+# FLOPs per column (LW) or per day column (SW) of the band kernels, from rocprofv3 SQ_INSTS_VALU_{ADD,MUL,
+# FMA,TRANS}_F64 / _F32 counts (x 64 lanes, FMA counted twice) on a 160,000-column call; empty until
+# measured for a kernel.
+FLOPS_FP64 = {}
+FLOPS_MIXED = {}
+
+
+def kernel_flops(name: str, precision: str, ncol: int, nday: int):
+    table = FLOPS_FP64 if precision == "fp64" else FLOPS_MIXED
+    if name not in table:
+        return None
+    return table[name] * (ncol if name.startswith("k_lw") else nday)
+
+
+def serial_pass(d, precision, max_cols):
+    """One call with the band kernels on the caller's stream (a context without band streams): the
+    duration of each kernel alone on the GPU."""
+    import torch
+
+    from woof_nam_amd.radiation import Radiation
+    prev = os.environ.get("WR_BAND_STREAMS")
+    os.environ["WR_BAND_STREAMS"] = "0"
+    try:
+        rad = Radiation(0, max_cols=max_cols)
+    finally:
+        if prev is None:
+            os.environ.pop("WR_BAND_STREAMS")
+        else:
+            os.environ["WR_BAND_STREAMS"] = prev
+    out, _ = rad.run(d, precision=precision)                # warm-up
+    torch.cuda.synchronize()
+    rad.timing(True)
+    t0 = time.perf_counter()
+    rad.run(d, precision=precision, out=out)
+    torch.cuda.synchronize()
+    call_ms = (time.perf_counter() - t0) * 1e3
+    ms = {}
+    for name, t in rad.timing_read():
+        ms[name] = ms.get(name, 0.0) + t
+    rad.timing(False)
+    rad.close()
+    del out
+    print(f"# serial pass: {call_ms:.1f} ms per call with the band kernels on one stream", file=sys.stderr, flush=True)
+    return {"call_ms": call_ms, "ms": ms}
+
+
 def served_block(d, dev, ncol, seconds):
     """Config-3 serving: Poisson arrivals at 40 req/s of the parent grid, 3 km / 6 km nests and
     ensemble members (one member per request, seeds 1..8), mixed precision, 200 GB HBM budget."""
@@ -114,6 +164,60 @@ def served_block(d, dev, ncol, seconds):
     return r
 
 
+def cpu_info() -> dict:
+    """Host cores and CPU model (nproc and /proc/cpuinfo), and the threads this run may use: the
+    OMP_NUM_THREADS share when the host sets one, else every core."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    nproc = os.cpu_count() or 1
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    return {"nproc": nproc, "cpu_model": model, "threads": share if share > 0 else nproc}
+
+
+def cpu_baseline(cols, out, ncol):
+    """The CPU FP64 oracle (the C restatement, OpenMP over columns) on the 4,096-column CPU-reference
+    workload (gen.columns(4096, seed=0), McICA): 1 warm-up, then the median of 5 timed runs on this
+    host's cores.  The full-grid figure is extrapolated from it.  Beside it, the GPU FP64 outputs
+    against the oracle on the first columns of the grid."""
+    from oracle import pyoracle
+    from woof_nam_amd import gen
+    info = cpu_info()
+    threads = info["threads"]
+    c0 = gen.columns(4096, seed=0)
+    pyoracle.run(c0, nthreads=threads)                      # warm-up
+    times = []
+    for _ in range(5):
+        t1 = time.perf_counter()
+        pyoracle.run(c0, nthreads=threads)
+        times.append(time.perf_counter() - t1)
+    med = float(np.median(times))
+    base = {"value": 4096 / med, "unit": "columns/s", "cores": threads, "kind": "port",
+            "sample": "4,096 independent columns x 60 layers (the CPU-reference workload, seed 0), LW+SW McICA, "
+                      f"CPU FP64 oracle (C, OpenMP); median of 5 runs after 1 warm-up: {med * 1e3:.1f} ms",
+            "runs_ms": [t * 1e3 for t in times], "nproc": info["nproc"], "cpu_model": info["cpu_model"],
+            "grid_extrapolated_s": ncol / (4096 / med),
+            "grid_note": f"the {ncol}-column grid is extrapolated from the 4,096-column rate, not run"}
+    ns = min(ncol, 8192)
+    sub = {k: np.ascontiguousarray(v[..., :ns]) for k, v in cols.items()}
+    ref, _ = pyoracle.run(sub, nthreads=threads)
+    acc = {}
+    for k in ("hr_lw", "hr_sw"):
+        diff = out[k][:, :ns].cpu().numpy() - ref[k]
+        acc[k + "_rms_kday"] = float(np.sqrt(np.mean(diff ** 2)) * KDAY)
+    for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+        diff = out[k][:, :ns].cpu().numpy() - ref[k]
+        acc[k + "_rms_wm2"] = float(np.sqrt(np.mean(diff ** 2)))
+    acc["vs"] = f"CPU FP64 oracle on the first {ns} columns of the grid"
+    return base, acc
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.splitlines()[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -123,12 +227,12 @@ def main(argv=None):
     ap.add_argument("--max-cols", type=int, default=0, help="tile columns (0 = the whole grid in one tile)")
     ap.add_argument("--nx", type=int, default=954)
     ap.add_argument("--ny", type=int, default=835)
-    ap.add_argument("--cpu-cols", type=int, default=0, help="CPU baseline sample (0 = sized from cores)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--acc-cols", type=int, default=65536, help="mixed-precision error-budget sample")
     ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision comparison block")
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="seconds of Poisson arrivals (40 req/s) for the served block (0 = skip)")
+    ap.add_argument("--no-serial", action="store_true", help="skip the pass with the band kernels on one stream")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB between two device buffers (reference for HBM byte counters)")
     a = ap.parse_args(argv)
@@ -196,6 +300,7 @@ def main(argv=None):
     traffic = kernel_traffic(dom, a.precision, ncol * a.steps / cnt[dom], nday * a.steps / cnt[dom])
     shares = sorted(((v / sum(tot.values()), k) for k, v in tot.items()), reverse=True)[:6]
     print("# kernel time shares: " + ", ".join(f"{k} {100 * f:.1f}%" for f, k in shares), file=sys.stderr, flush=True)
+    serial = serial_pass(d, a.precision, a.max_cols) if not a.no_serial else None
 
     res = {
         "metric": METRIC, "value": value, "unit": "columns/s", "n_gpus": 1, "steps": a.steps,
@@ -215,6 +320,19 @@ def main(argv=None):
                                      "the time a kernel shares the CUs with the others",
                      "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
     }
+    if serial is not None:
+        rf = res["roofline"]
+        rf["serial_launch_ms"] = serial["ms"].get(dom)
+        rf["serial_frac"] = (bytes_per_launch / (rf["serial_launch_ms"] * 1e-3) / 1e9 / HBM_PEAK_GBS
+                             if rf["serial_launch_ms"] else None)
+        rf["serial_note"] = "the same kernel with the band kernels on one stream (each alone on the GPU)"
+        fl = kernel_flops(dom, a.precision, ncol, nday)
+        if fl is not None and rf["serial_launch_ms"]:
+            peak = VALU_PEAK_TFLOPS["fp64" if a.precision == "fp64" else "fp32"]
+            ach = fl / (rf["serial_launch_ms"] * 1e-3) / 1e12
+            rf["valu"] = {"achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
+                          "flops_per_launch": fl, "from": "rocprofv3 SQ_INSTS_VALU_* counts per column x 64 lanes"}
+        res["serial"] = {"ms_per_call": serial["call_ms"], "kernel_ms": serial["ms"]}
 
     if a.precision == "fp64" and not a.no_mixed:
         # the same workload on the mixed-precision path: throughput, speedup, errors against FP64
@@ -263,30 +381,7 @@ def main(argv=None):
         }
 
     if not a.no_cpu:
-        from oracle import pyoracle
-        threads = min(16, os.cpu_count() or 1)
-        # size the sample for about 15 s of CPU work from a short calibration run
-        cal = {k: np.ascontiguousarray(v[..., :4000]) for k, v in cols.items()}
-        t1 = time.perf_counter()
-        pyoracle.run(cal, nthreads=threads)
-        rate = 4000 / (time.perf_counter() - t1)
-        ns = a.cpu_cols or int(min(ncol, max(4000, 15.0 * rate)))
-        sub = {k: np.ascontiguousarray(v[..., :ns]) for k, v in cols.items()}
-        t1 = time.perf_counter()
-        ref, _ = pyoracle.run(sub, nthreads=threads)
-        tc = time.perf_counter() - t1
-        res["cpu_baseline"] = {"value": ns / tc, "unit": "columns/s", "cores": threads, "kind": "port",
-                               "sample": f"first {ns} columns of the same grid, CPU FP64 oracle (C, OpenMP), "
-                                         f"{tc:.1f} s"}
-        acc = {}
-        for k in ("hr_lw", "hr_sw"):
-            diff = out[k][:, :ns].cpu().numpy() - ref[k]
-            acc[k + "_rms_kday"] = float(np.sqrt(np.mean(diff ** 2)) * KDAY)
-        for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
-            diff = out[k][:, :ns].cpu().numpy() - ref[k]
-            acc[k + "_rms_wm2"] = float(np.sqrt(np.mean(diff ** 2)))
-        acc["vs"] = f"CPU FP64 oracle on the first {ns} columns"
-        res["accuracy"] = acc
+        res["cpu_baseline"], res["accuracy"] = cpu_baseline(cols, out, ncol)
     else:
         res["cpu_baseline"] = None
     print(json.dumps(res), flush=True)
