@@ -38,7 +38,8 @@ enum {
     WR_ERR_TABLES = 2,     /* table file missing or malformed  -> FileNotFoundError   */
     WR_ERR_HIP = 3,        /* HIP runtime error                -> RuntimeError        */
     WR_ERR_NOMEM = 4,      /* device allocation failed         -> MemoryError         */
-    WR_ERR_UNSUPPORTED = 5 /* option not implemented           -> NotImplementedError */
+    WR_ERR_UNSUPPORTED = 5,/* option not implemented           -> NotImplementedError */
+    WR_ERR_TOO_LARGE = 6   /* call beyond a static work space  -> ValueError          */
 };
 
 /* Column inputs (model layers nlay, interfaces nlay+1; index 0 = model top). */
@@ -130,6 +131,19 @@ int wr_run_fp64(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const w
  * and for pressure/temperature index arithmetic.  Same inputs/outputs as wr_run_fp64. */
 int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
                  const wr_diag *diag, void *stream);
+
+/* wr_run_mixed that allocates nothing, not even through its context: no counterpart in the
+ * reference.  For callers that cannot allocate device memory at call time (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid allocation).  Same inputs, outputs and results
+ * as wr_run_mixed (the same kernels), but the tile work space is a static device array of the library
+ * (219 MB per device, held until the process ends) instead of the context's, and the band
+ * kernels run on the caller's stream.  At most WR_NOALLOC_MAX_COLS columns per call (more:
+ * WR_ERR_TOO_LARGE) and 63 model layers.  The context supplies the tables; its own work space is not
+ * touched.  Every call on one device shares that device's array: calls that overlap on one device
+ * (other streams, other threads, other contexts) race and must be serialised by the caller. */
+#define WR_NOALLOC_MAX_COLS 4096
+int wr_run_mixed_noalloc(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out,
+                         void *stream);
 
 /* One band alone, for per-band error budgets (no counterpart in the reference): as wr_run_fp64
  * (mixed = 0) or wr_run_mixed (mixed = 1), but only band `band` contributes to the fluxes and

@@ -15,7 +15,7 @@ LIB_PATH = Path(os.environ.get("WR_LIB", Path(__file__).resolve().parent / "libw
 TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 
 # symbols include/woofrad.h declares (the CPU test suite checks every one is exported)
-EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", "wr_run_mixed", "wr_run_band", "wr_cal_cldfra",
+EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", "wr_run_mixed", "wr_run_mixed_noalloc", "wr_run_band", "wr_cal_cldfra",
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
            "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate")
@@ -26,6 +26,7 @@ _ERRORS = {
     abi.WR_ERR_HIP: RuntimeError,
     abi.WR_ERR_NOMEM: MemoryError,
     abi.WR_ERR_UNSUPPORTED: NotImplementedError,
+    abi.WR_ERR_TOO_LARGE: ValueError,
 }
 
 _lib = None
@@ -51,6 +52,9 @@ def load():
         getattr(L, f).argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams),
                                   C.POINTER(abi.WrOutputs), C.POINTER(abi.WrDiag), vp]
         getattr(L, f).restype = C.c_int
+    L.wr_run_mixed_noalloc.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams),
+                                       C.POINTER(abi.WrOutputs), vp]
+    L.wr_run_mixed_noalloc.restype = C.c_int
     L.wr_run_band.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
                               C.c_int32, C.c_int32, vp]
     L.wr_run_band.restype = C.c_int

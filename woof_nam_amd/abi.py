@@ -8,7 +8,8 @@ import ctypes as C
 
 NBANDS_LW, NGPT_LW, NBANDS_SW, NGPT_SW = 16, 140, 14, 112
 
-WR_OK, WR_ERR_ARG, WR_ERR_TABLES, WR_ERR_HIP, WR_ERR_NOMEM, WR_ERR_UNSUPPORTED = range(6)
+WR_OK, WR_ERR_ARG, WR_ERR_TABLES, WR_ERR_HIP, WR_ERR_NOMEM, WR_ERR_UNSUPPORTED, WR_ERR_TOO_LARGE = range(7)
+NOALLOC_MAX_COLS = 4096  # WR_NOALLOC_MAX_COLS
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int32)

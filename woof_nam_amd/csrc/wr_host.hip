@@ -10,6 +10,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -468,6 +469,55 @@ Sizes sizes(int32_t max_cols)
     return z;
 }
 
+// ---- work space of wr_run_mixed_noalloc ---------------------------------------------------------
+// The mixed-precision tile work space for up to WR_NOALLOC_MAX_COLS columns of up to 64 radiation
+// layers, in one static device array (every device the library runs on holds its own copy until the
+// process ends).  Layout (256-byte aligned parts): FP32 band flux planes, column words, layer logs and
+// level temperatures, NAM-route overlap factors, FP32 solar source, and one band stream's level pool
+// with a slot for every 64-column block.  The kernels write every word of it that they read, so
+// nothing needs clearing between calls; calls that overlap on one device race on it.
+struct NaLayout {
+    size_t acc, iws, dws, rtr, sflux, pool, total;
+};
+constexpr size_t na_al(size_t x) { return (x + 255) & ~(size_t)255; }
+constexpr NaLayout na_layout()
+{
+    const size_t m = WR_NOALLOC_MAX_COLS;
+    NaLayout z{};
+    size_t off = 0;
+    z.acc = off, off += na_al(sizeof(float) * (60 * (NL + 1) + 14) * m);
+    z.iws = off, off += na_al(sizeof(int) * 7 * m);
+    z.dws = off, off += na_al(sizeof(double) * (4 + 2 * (size_t)NL) * m);
+    z.rtr = off, off += na_al((sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m);
+    z.sflux = off, off += na_al(sizeof(float) * 112 * m);
+    const size_t lane_max = std::max(cache_slot_bytes(sw_nv_mc<float>(), 16, 64 + 1, (int)sizeof(float)),
+                                     cache_slot_bytes(LW_NV_F32, 16, 64 + 1, (int)sizeof(float)));
+    z.pool = off, off += (m + 63) / 64 * lane_max;
+    z.total = off;
+    return z;
+}
+constexpr NaLayout NA = na_layout();
+static_assert(NA.total <= ((size_t)256 << 20), "the no-alloc work space stays under 256 MiB");
+
+static __device__ __align__(256) char scratch[NA.total];
+
+// Device address of `scratch` on each device, looked up once per device (hipGetSymbolAddress with the
+// device current) by wr_create, or by the first wr_run_mixed_noalloc on a device without a context.
+constexpr int NA_MAXDEV = 64;
+std::mutex na_mu;
+char *na_ptr[NA_MAXDEV];
+
+int noalloc_setup(int device)
+{
+    if (device < 0 || device >= NA_MAXDEV) return WR_ERR_ARG;
+    std::lock_guard<std::mutex> lk(na_mu);
+    if (na_ptr[device]) return WR_OK;
+    void *p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return WR_ERR_HIP;
+    na_ptr[device] = (char *)p;
+    return WR_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -497,6 +547,7 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
             hipMemcpy(c->tabs_f, &c->host_f, sizeof(Tabs<float>), hipMemcpyHostToDevice) != hipSuccess)
             rc = WR_ERR_HIP;
     }
+    if (rc == WR_OK) rc = noalloc_setup(device);
     if (rc == WR_OK) rc = wr_init_luts(c, nullptr);
     if (rc == WR_OK) {
         const double *src[4] = {c->host_d.exp_tbl, c->host_d.tfn_tbl, c->host_d.tau_tbl, c->host_d.sw_exp_tbl};
@@ -557,6 +608,37 @@ int wr_run_mixed(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const 
                  void *stream)
 {
     return run(ctx, in, par, out, diag, stream, MODE_MIXED);
+}
+
+int wr_run_mixed_noalloc(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, void *stream)
+{
+    if (!ctx || !in) return WR_ERR_ARG;
+    if (in->ncol > WR_NOALLOC_MAX_COLS) return WR_ERR_TOO_LARGE;
+    if (in->nlay + 1 > 64) return WR_ERR_ARG;
+    int rc = WR_OK;
+    {
+        DevGuard g(ctx->device);
+        rc = noalloc_setup(ctx->device);
+    }
+    if (rc != WR_OK) return rc;
+    char *base = na_ptr[ctx->device];
+    // the context's tables, streams and timing with the work space of the static array; the band
+    // kernels stay on the caller's stream (one level pool)
+    wr_ctx t = *ctx;
+    t.max_cols = WR_NOALLOC_MAX_COLS;
+    t.acc = (double *)(base + NA.acc);
+    t.iws = (int *)(base + NA.iws);
+    t.dws = (double *)(base + NA.dws);
+    t.rtr = base + NA.rtr;
+    t.sflux_d = nullptr;
+    t.sflux_f = (float *)(base + NA.sflux);
+    for (int q = 0; q < WR_NAUX; ++q) t.pool[q] = nullptr;
+    t.pool[0] = base + NA.pool;
+    t.pool_bytes = NA.total - NA.pool;
+    t.naux = 0;
+    rc = run(&t, in, par, out, nullptr, stream, MODE_MIXED);
+    ctx->ev_n = t.ev_n;
+    return rc;
 }
 
 int wr_run_band(wr_ctx *ctx, const wr_columns *in, const wr_params *par, const wr_outputs *out, int32_t band,
@@ -693,6 +775,7 @@ const char *wr_strerror(int code)
     case WR_ERR_HIP: return "HIP runtime error";
     case WR_ERR_NOMEM: return "device allocation failed";
     case WR_ERR_UNSUPPORTED: return "option not supported by this build";
+    case WR_ERR_TOO_LARGE: return "more columns than the static work space of wr_run_mixed_noalloc holds";
     default: return "unknown error";
     }
 }

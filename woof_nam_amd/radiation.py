@@ -180,6 +180,25 @@ class Radiation:
         _lib.check(rc, f"wr_run_{precision}")
         return out, dg  # mask words are returned as int32 bit patterns
 
+    def run_noalloc(self, cols: dict, params: abi.WrParams | None = None, out: dict | None = None, stream=None) -> dict:
+        """wr_run_mixed_noalloc: the mixed-precision call with the library's static work space (at most
+        abi.NOALLOC_MAX_COLS columns; calls on one device must not overlap)."""
+        t = cols["t_lay"]
+        nlay, ncol = int(t.shape[0]), int(t.shape[1])
+        for k, shp in input_shapes(ncol, nlay).items():
+            _check_field(k, cols[k], shp, torch.float64, self.device)
+        if out is None:
+            out = {k: torch.empty(s, dtype=torch.float64, device=self.device)
+                   for k, s in abi.output_shapes(ncol, nlay).items()}
+        ci = _colstruct(cols, ncol, nlay, self.device)
+        co = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
+        params = params or self.default_params()
+        s = stream or torch.cuda.current_stream(self.device)
+        rc = _lib.load().wr_run_mixed_noalloc(self._h, C.byref(ci), C.byref(params), C.byref(co),
+                                              C.c_void_p(s.cuda_stream))
+        _lib.check(rc, "wr_run_mixed_noalloc")
+        return out
+
 
 def cal_cldfra(cldfra: torch.Tensor, qc: torch.Tensor, qi: torch.Tensor | None, i_lo: int, i_hi: int,
                j_lo: int, j_hi: int, f_qc: bool = True, f_qi: bool = True, stream=None) -> None:
