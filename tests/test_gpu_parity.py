@@ -77,6 +77,34 @@ def test_fp64_bit_identical(rad):
             f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
 
 
+def test_glane_lw_kernels(gpu_device, monkeypatch):
+    """The opt-in g-point-lane LW kernels (WR_LW_GLANE=1, read when a context is created): FP64
+    bit-identical to the oracle; in the mixed path the LW fluxes within 0.01 W/m2 and the LW heating
+    rates within 0.01 K/day of the lane-per-column mixed kernels at every level (both FP32 optics and
+    sums in g order; the lane-per-column kernel keeps the up sweep's transmittance and source from
+    the down sweep where the g-lane kernel recomputes them)."""
+    cols = gen.columns(1500, seed=17)
+    plain = Radiation(0, max_cols=2048)
+    monkeypatch.setenv("WR_LW_GLANE", "1")
+    r = Radiation(0, max_cols=2048)
+    try:
+        out, _, ref, _ = run_both(r, cols)
+        for k in abi.OUTPUT_FIELDS:
+            a = out[k].cpu().numpy()
+            assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), \
+                f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
+        dcols = to_dev(cols, r.device)
+        mg, _ = r.run(dcols, precision="mixed")
+        mc, _ = plain.run(dcols, precision="mixed")
+        torch.cuda.synchronize()
+        for k in ("flw_up", "flw_dn"):
+            assert (mg[k] - mc[k]).abs().max().item() <= 0.01, k
+        assert (mg["hr_lw"] - mc["hr_lw"]).abs().max().item() * 86400.0 <= 0.01
+    finally:
+        r.close()
+        plain.close()
+
+
 def test_day_list_over_several_segments(gpu_device):
     """One 7,000-column tile: the day list is compacted from four 2,048-column segments, in order."""
     cols = gen.columns(7000, seed=13)

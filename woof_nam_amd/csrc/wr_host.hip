@@ -15,6 +15,7 @@
 #include <unordered_map>
 #include <vector>
 
+#include "wr_glw.h"
 #include "wr_internal.h"
 
 using namespace wr;
@@ -435,11 +436,24 @@ int run(wr_ctx *c, const wr_columns *in, const wr_params *p, const wr_outputs *o
     return WR_OK;
 }
 
+// WR_LW_GLANE=1 runs the McICA route's LW bands as the g-point-lane kernels over per-tile layer
+// records (wr_glw.h) instead of the lane-per-column kernels; the records then take about 15 KB per
+// column of context memory.  Off by default: measured on the 954x835x60 grid, the g-lane kernels
+// took twice the time of the lane-per-column ones.
+bool lw_glane_env()
+{
+    const char *e = getenv("WR_LW_GLANE");
+    return e && atoi(e) != 0;
+}
+
 // Device bytes of a context's tile work space for tiles of m columns, and the band-stream count.
 struct Sizes {
-    size_t acc, iws, dws, rtr, sflux_d, sflux_f, pool;
+    size_t acc, iws, dws, rtr, sflux_d, sflux_f, lwrec, pool;
     int naux;
-    size_t total() const { return acc + iws + dws + rtr + sflux_d + sflux_f + pool * (size_t)(naux > 1 ? naux : 1); }
+    size_t total() const
+    {
+        return acc + iws + dws + rtr + sflux_d + sflux_f + lwrec + pool * (size_t)(naux > 1 ? naux : 1);
+    }
 };
 Sizes sizes(int32_t max_cols)
 {
@@ -451,6 +465,7 @@ Sizes sizes(int32_t max_cols)
     z.rtr = (sizeof(double) * 12 * (NL + 3) + 2 * sizeof(uint64_t)) * m;      // NAM-route overlap factors
     z.sflux_d = sizeof(double) * 112 * m;
     z.sflux_f = sizeof(float) * 112 * m;
+    z.lwrec = lw_glane_env() ? lwrec_bytes_per_col<double>(64 + 1) * m + 3 * 256 : 0;  // LW layer records
     // level pool: room for every 64-column block of a tile in the largest slot layout (FP64 SW:
     // 2 values, FP32 SW: 7 values, x 16 g x levels), at most WR_POOL_GB (default 6) GB per stream
     const int nl_max = 64;
@@ -562,8 +577,10 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
         if (hipMalloc((void **)&c->acc, z.acc) != hipSuccess || hipMalloc((void **)&c->iws, z.iws) != hipSuccess ||
             hipMalloc((void **)&c->dws, z.dws) != hipSuccess || hipMalloc(&c->rtr, z.rtr) != hipSuccess ||
             hipMalloc((void **)&c->sflux_d, z.sflux_d) != hipSuccess ||
-            hipMalloc((void **)&c->sflux_f, z.sflux_f) != hipSuccess)
+            hipMalloc((void **)&c->sflux_f, z.sflux_f) != hipSuccess ||
+            (z.lwrec && hipMalloc(&c->lwrec, z.lwrec) != hipSuccess))
             rc = WR_ERR_NOMEM;
+        c->lw_glane = z.lwrec > 0;
     }
     if (rc == WR_OK) {
         c->pool_bytes = z.pool;
@@ -632,6 +649,8 @@ int wr_run_mixed_noalloc(wr_ctx *ctx, const wr_columns *in, const wr_params *par
     t.rtr = base + NA.rtr;
     t.sflux_d = nullptr;
     t.sflux_f = (float *)(base + NA.sflux);
+    t.lwrec = nullptr;  // lane-per-column LW kernels
+    t.lw_glane = 0;
     for (int q = 0; q < WR_NAUX; ++q) t.pool[q] = nullptr;
     t.pool[0] = base + NA.pool;
     t.pool_bytes = NA.total - NA.pool;
@@ -749,6 +768,7 @@ const char *wr_kernel_name(int32_t kid)
     if (kid == KID_FEAT) return "k_features";
     if (kid == KID_MLP) return "mlp_gemm";
     if (kid == KID_RTRFAC) return "k_rtr_fac";
+    if (kid == KID_LWREC) return "k_lwrec";
     return "?";
 }
 
@@ -795,6 +815,7 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->rtr);
     (void)hipFree(c->sflux_d);
     (void)hipFree(c->sflux_f);
+    (void)hipFree(c->lwrec);
     (void)hipFree(c->sx);
     (void)hipFree(c->sy);
     (void)hipFree(c->sh1);

@@ -26,6 +26,8 @@ struct wr_ctx {
     void *rtr;                       // maximum-random overlap factors of the NAM LW route
     double *sflux_d;                 // [112][max_cols]
     float *sflux_f;
+    void *lwrec;                     // LW layer records of a tile (g-lane LW kernels), sized for FP64
+    int lw_glane;                    // 1: McICA LW bands as g-lane kernels over the layer records
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
     int has_sur;
     wr_surrogate sur;
@@ -48,7 +50,7 @@ struct wr_ctx {
 
 // kernel ids reported by wr_timing_read
 enum { KID_ZERO = 0, KID_COLPREP = 1, KID_DAYLIST = 2, KID_LW0 = 3, KID_SW0 = 19, KID_POST = 33, KID_DIAG = 34,
-       KID_FEAT = 35, KID_MLP = 36, KID_RTRFAC = 37, KID_COUNT = 38 };
+       KID_FEAT = 35, KID_MLP = 36, KID_RTRFAC = 37, KID_LWREC = 38, KID_COUNT = 39 };
 
 inline void tbeg(wr_ctx *c, int kid, hipStream_t s)
 {

@@ -8,6 +8,7 @@
 // band order, so the result does not depend on which stream ran which band.
 #pragma once
 #include "wr_internal.h"
+#include "wr_glw.h"
 #include "wr_nam.h"
 
 namespace wr {
@@ -75,7 +76,25 @@ struct BandJob {
     BandAcc ba;
     Surr sg;
     Rtr<R> rt;
+    LwRec<R> rec;
 };
+
+// The LW layer records of a tile in the context's record space: [nk][n] index words, then
+// [REC_R][nk][n] values of precision R, then [REC_D][nk][n] FP64 values (256-byte aligned parts).
+template <typename R>
+inline LwRec<R> tile_lwrec(wr_ctx *c, int n, int nk)
+{
+    LwRec<R> r;
+    char *b = (char *)c->lwrec;
+    const size_t a = ((size_t)nk * n * sizeof(int32_t) + 255) & ~(size_t)255;
+    const size_t v = ((size_t)REC_R * nk * n * sizeof(R) + 255) & ~(size_t)255;
+    r.idx = gptr((int32_t *)b);
+    r.r = gptr((R *)(b + a));
+    r.d = gptr((double *)(b + a + v));
+    r.nk = nk;
+    r.n = n;
+    return r;
+}
 
 // McICA route band kernels (k_lw / k_sw), dispatched from a runtime id to the template instance.
 // Grid of a band kernel: one 64-lane block per 64 columns, at most one block per pool slot (the
@@ -96,6 +115,15 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
         if (id != B) return launch_band<R, SURR, B + 1>(j, id, s, pool);
         const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
+        if constexpr (B < 16) {
+            if (j.c->lw_glane) {
+                constexpr int CPW = 64 / pow2_at_least(LW_NG[B]);
+                hipLaunchKernelGGL((k_glw<R, B, SURR>), dim3((j.n + CPW - 1) / CPW), dim3(64), 0, s, j.tabs, j.in,
+                                   j.par, j.col0, j.n, j.cs, acc, j.sg, j.rec);
+                tend(j.c, s);
+                return;
+            }
+        }
         if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1)
             hipLaunchKernelGGL((k_lwc<B, SURR>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg, pool);
@@ -175,13 +203,19 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
                 int col0, int n, int band, hipStream_t s, const Surr &sg = Surr{})
 {
     const int nrl = in.nlay + 1;
-    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), sg, Rtr<R>{}};
+    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), sg, Rtr<R>{},
+                 tile_lwrec<R>(c, n, nrl)};
     tbeg(c, KID_COLPREP, s);
     hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
     tbeg(c, KID_DAYLIST, s);
     launch_daylist(j.cs, n, s);
     tend(c, s);
+    if (c->lw_glane && (band < 0 || band < 16)) {
+        tbeg(c, KID_LWREC, s);
+        hipLaunchKernelGGL((k_lwrec<R>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, j.cs, j.rec);
+        tend(c, s);
+    }
     int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
@@ -202,7 +236,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     rt.lstu = gptr((uint64_t *)((char *)c->rtr + sizeof(double) * 12 * (NL + 3) * (size_t)c->max_cols));
     rt.lstd = rt.lstu + c->max_cols;
     rt.n = n;
-    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), Surr{}, rt};
+    BandJob<R> j{c, tabs, sflux, in, par, col0, n, colscal(c), tile_planes(c, n, nrl, band), Surr{}, rt, LwRec<R>{}};
     tbeg(c, KID_COLPREP, s);
     hipLaunchKernelGGL((k_colprep<R>), dim3((n + 255) / 256), dim3(256), 0, s, tabs, in, par, col0, n, j.cs, sflux);
     tend(c, s);
