@@ -63,6 +63,14 @@ typedef struct {
     const int32_t *ncol_dev; /* optional device pointer to the number of valid columns (<= ncol);
                                 columns from it on are skipped and their outputs left untouched
                                 (lets one graph captured for ncol columns serve shorter tiles)    */
+    /* Optional path form of the clouds, progcld2's outputs (wr_nmmb_prep writes them).  When cwp is
+     * set, each layer's cloud is cldf with these paths and radii, and qc, qi and wr_params.re_liq /
+     * re_ice are not read (qc, qi may be NULL); cip, rew and rei are then required, crp optional. */
+    const double *cwp;       /* [nlay][ncol] cloud liquid water path, g/m2                        */
+    const double *cip;       /* [nlay][ncol] cloud ice path, g/m2                                 */
+    const double *crp;       /* [nlay][ncol] rain path, g/m2 (NULL: none)                          */
+    const double *rew;       /* [nlay][ncol] liquid effective radius, micron                      */
+    const double *rei;       /* [nlay][ncol] ice effective radius, micron                         */
 } wr_columns;
 
 /* Call parameters (NAM values by default; see wr_default_params). */
@@ -163,6 +171,102 @@ int wr_cal_cldfra(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi
  * be NULL).  Bits equal those wr_run_* use. */
 int wr_mcica_masks(int32_t ncol, int32_t nrl, const double *cldf, const uint64_t *col_id,
                    uint64_t seed, int32_t iovr, uint32_t *lw_mask, uint32_t *sw_mask, void *stream);
+
+/* ---- NMMB-state side of the call (the hook's prep and post, on the device) ----------------------
+ *
+ * wr_nmmb_prep turns the model state of the radiated rows into the column inputs above, with the
+ * cloud properties in the path form (wr_columns.cwp ...).  It restates, per column:
+ *   - driver prep (phys_rrtm_driver_prep.cu:82-98): interface pressures P8W = PT + cumulative
+ *     (PDSG1 + DSG2 PD) and layer pressures P_PHY = SGML2 PD + PSGML1, in single precision as there;
+ *   - driver call (phys_rrtm_driver_call.cu:200-275): total condensate CW with the Ferrier fractions
+ *     F_ICE, F_RAIN, and the subgrid cloud water of layers without condensate at RH > 95 % (SUBGRID,
+ *     the saturation pressure FPVS of module_MP_FER_HIRES.F90:2874-2971);
+ *   - grrad setup (phys_rrtm_grrad_setup.cu:102-215): the water-vapour floor 1e-7, the virtual
+ *     temperature, the layer above the model top (cloud free);
+ *   - progcld2 (phys_rrtm_progcld2.cu:92-359) in the NAM configuration (top-first, no lcrick / lcnorm
+ *     / lnoprec / lsashal, condensate cloud fraction): ice / liquid / rain split, cloud fraction, the
+ *     paths of cloud water, ice and rain (g/m2), the diagnosed liquid and ice effective radii, and
+ *     gethml's low / middle / high / total / boundary-layer cloud cover with their top and base layers.
+ * The state is single precision; the prep evaluates in FP64 (the CPU oracle's operation sequence).
+ * wr_nmmb_post is driver post (phys_rrtm_driver_post.cu:71-150): RLWTT, RSWTT, the surface and top
+ * fluxes, CZMEAN, SIGT4, and the cloud outputs CLDFRA (progcld2's layer cloud fraction), CFRACL/M/H
+ * with the ACFRST/NCFRST (and ACFRCV/NCFRCV) accumulators. */
+typedef struct {
+    int32_t im, jm, lm;      /* fields [lm][jm][im] (level 0 = model top), surface fields [jm][im]    */
+    int32_t j0, j1;          /* radiated rows j0 <= j < j1 (0-based); column c = (j - j0) * im + i     */
+    float pt;                /* pressure at the model top, Pa                                         */
+    const float *pd;         /* [jm][im] PD, Pa                                                       */
+    const float *pdsg1, *dsg2, *psgml1, *sgml2;  /* [lm] hybrid coordinate (pressure and sigma parts) */
+    const float *t, *q, *o3; /* [lm][jm][im] temperature K, specific humidity, ozone (kg/kg)          */
+    const float *cw;         /* [lm][jm][im] total condensate CW, kg/kg                               */
+    const float *f_ice;      /* [lm][jm][im] ice fraction of the condensate (Ferrier F_ICE)           */
+    const float *f_rain;     /* [lm][jm][im] rain fraction of the liquid (Ferrier F_RAIN)             */
+    const float *glat;       /* [jm][im] latitude, radians                                            */
+} wr_nmmb_state;
+
+typedef struct {
+    int32_t llyr;            /* top of the boundary-layer cloud: 1-based top-first radiation layer,
+                                cld_init's rule (radiation_clouds_nmmb.f:327-344, ivflip = 0)         */
+    int32_t iovr;            /* gethml cloud overlap: 0 random, 1 maximum-random                      */
+    int32_t subgrid;         /* 1: subgrid cloud water where RH > 95 % and no condensate (the NAM's)   */
+} wr_nmmb_opts;
+
+typedef struct {             /* device arrays the prep writes; ncol = (j1 - j0) * im                  */
+    double *p_lev;           /* [lm+1][ncol] interface pressure, Pa                                   */
+    double *t_lay, *q_lay, *o3_lay;               /* [lm][ncol]                                       */
+    double *cldf, *cwp, *cip, *crp, *rew, *rei;   /* [lm][ncol] progcld2 layer cloud fraction, liquid /
+                                                     ice / rain paths (g/m2), liquid / ice radii (um) */
+    float *clds;             /* [5][ncol] gethml cover: low, middle, high, total, boundary layer      */
+    int32_t *mtop, *mbot;    /* [3][ncol] top / base layer of low, middle, high (1-based top-first
+                                radiation layers, the added layer above the model top is layer 1)    */
+    float *p_phy;            /* optional [lm][jm][im] state field P_PHY (radiated rows), or NULL      */
+} wr_nmmb_cols;
+
+typedef struct {             /* state outputs (float32 device arrays; int32 counters)                */
+    float *rlwtt, *rswtt, *cldfra;                          /* [lm][jm][im]                           */
+    float *rlwin, *rlwtoa, *sigt4;                           /* [jm][im], LW steps                    */
+    float *rswin, *rswout, *rswinc, *rswtoa;                 /* [jm][im], SW steps                    */
+    float *czmean;                                           /* [jm][im], every call                  */
+    float *cfracl, *cfracm, *cfrach, *acfrst, *acfrcv;       /* [jm][im], every call                  */
+    int32_t *ncfrst, *ncfrcv;                                /* [jm][im], every call                  */
+} wr_nmmb_out;
+
+int wr_nmmb_prep(const wr_nmmb_state *st, const wr_nmmb_opts *opt, const wr_nmmb_cols *cols, void *stream);
+
+/* rad: the radiation call's outputs on the same columns (fsw_dn_sfc_clr required on SW steps);
+ * czmean [ncol]: the call's mean cosine of the zenith angle; lw_step / sw_step: which parts ran. */
+int wr_nmmb_post(const wr_nmmb_state *st, const wr_nmmb_cols *cols, const wr_outputs *rad, const double *czmean,
+                 int32_t lw_step, int32_t sw_step, const wr_nmmb_out *out, void *stream);
+
+/* ---- solar geometry around the call ---------------------------------------------------------------
+ * wr_sol_update: sol_update_nmmb (phys_rrtm_sol_update.cu:16-160): the sun's position for a date
+ * (UTC) from its orbit (Kepler's equation by Newton iteration) and the solar constant solc0 / r^2 at
+ * the earth-sun distance r; nstp and anginc are the time steps sol_update sets for a SW interval of
+ * deltsw seconds with model step deltim.  Host computation, no device work.
+ * wr_coszmn: coszmn_nmmb (phys_rrtm_coszmn.cu:18-62): per column (device arrays [ncol], radians) the
+ * cosine of the zenith angle at nrads instants over the SW interval (dtswav hours from the hour solhr
+ * UTC), averaged over the sunlit instants (coszen, the radiation's cos z and the driver's CZMEAN) and
+ * over all of them (coszdg, optional).
+ * wr_rdtemp: RDTEMP (drv_rdtemp.cu:10-145), every model step: the sun's declination and hour angle at
+ * step ntsd of length dt from the start date (julday of julyr, hour ihrst; leap: February has 29
+ * days), czen [jm][im] of every column, and T += (RSWTT czen / CZMEAN + RLWTT) dt on the columns
+ * i_lo..i_hi, j_lo..j_hi (0-based inclusive) of the [lm][jm][im] float32 state. */
+typedef struct {
+    double sollag;           /* equation of time (solar lag), radians */
+    double sindec, cosdec;   /* sine and cosine of the declination     */
+    double anginc;           /* solar angle increment per sub-step      */
+    double solcon;           /* solar constant at the date, W/m2       */
+    int32_t nstp;            /* sub-steps per SW interval               */
+} wr_sol;
+
+int wr_sol_update(int32_t yr, int32_t mon, int32_t day, int32_t hr, int32_t mn, int32_t sec, double deltsw,
+                  double deltim, double solc0, wr_sol *out);
+int wr_coszmn(int32_t ncol, const double *glat, const double *glon, double solhr, double dtswav, int32_t nrads,
+              const wr_sol *sol, double *coszen, double *coszdg, void *stream);
+int wr_rdtemp(int32_t im, int32_t jm, int32_t lm, int32_t i_lo, int32_t i_hi, int32_t j_lo, int32_t j_hi, int32_t ntsd,
+              double dt, int32_t julday, int32_t julyr, int32_t ihrst, int32_t leap, const float *glat,
+              const float *glon, const float *czmean, const float *rswtt, const float *rlwtt, float *t, float *czen,
+              void *stream);
 
 /* ---- gas-optics surrogate (no counterpart in the reference: a learned replacement of taugb/taumol)
  *

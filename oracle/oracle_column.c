@@ -226,6 +226,19 @@ static void o_prep(const wr_columns *in, const wr_params *p, int i, Rad *r)
     for (int l = 1; l < nrl; ++l) {
         const int k0 = l - 1;
         double ct = F(cldf, k0);
+        if (in->cwp) { /* path form: progcld2's layer values as given */
+            const int cl = ct >= 0.001;
+            r->cld[l][0] = cl ? ct : 0.0;
+            r->cld[l][1] = cl ? F(cwp, k0) : 0.0;
+            r->cld[l][2] = F(rew, k0);
+            r->cld[l][3] = cl ? F(cip, k0) : 0.0;
+            r->cld[l][4] = F(rei, k0);
+            r->cld[l][5] = (cl && in->crp) ? F(crp, k0) : 0.0;
+            r->cld[l][6] = 100.0;
+            r->cld[l][7] = 0.0;
+            r->cld[l][8] = 0.0;
+            continue;
+        }
         const double qc = F(qc, k0), qi = F(qi, k0);
         double qcwat, qcice;
         if (r->tlyr[l] - 273.15 > -40.0) {

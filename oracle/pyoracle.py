@@ -96,17 +96,100 @@ def _ptr(a, ctype=C.c_double):
     return a.ctypes.data_as(C.POINTER(ctype)) if a is not None else None
 
 
+def _nmmb_state(st: dict, im, jm, lm, j0, j1, pt):
+    """WrNmmbState over host float32 arrays (kept alive by the caller's dict)."""
+    fp = C.c_float
+    return abi.WrNmmbState(im, jm, lm, j0, j1, pt, *[_ptr(st[k], fp) for k in
+                                                      ("pd", "pdsg1", "dsg2", "psgml1", "sgml2", "t", "q", "o3", "cw",
+                                                       "f_ice", "f_rain", "glat")])
+
+
+def nmmb_prep(st: dict, dims, opts):
+    """oracle_nmmb_prep on host float32 state arrays; dims = (im, jm, lm, j0, j1, pt), opts = (llyr, iovr,
+    subgrid).  Returns the column dict (float64 [k][ncol]) with clds, mtop, mbot and p_phy."""
+    im, jm, lm, j0, j1, pt = dims
+    ncol = (j1 - j0) * im
+    cols = {k: np.zeros(s) for k, s in abi.nmmb_col_shapes(ncol, lm).items()}
+    cols["clds"] = np.zeros((5, ncol), dtype=np.float32)
+    cols["mtop"] = np.zeros((3, ncol), dtype=np.int32)
+    cols["mbot"] = np.zeros((3, ncol), dtype=np.int32)
+    cols["p_phy"] = np.full((lm, jm, im), -1.0, dtype=np.float32)
+    cst = abi.WrNmmbCols(*[_ptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _ptr(cols["clds"], C.c_float),
+                         _ptr(cols["mtop"], C.c_int32), _ptr(cols["mbot"], C.c_int32), _ptr(cols["p_phy"], C.c_float))
+    L = lib()
+    L.oracle_nmmb_prep.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbOpts), C.POINTER(abi.WrNmmbCols)]
+    rc = L.oracle_nmmb_prep(C.byref(_nmmb_state(st, *dims)), C.byref(abi.WrNmmbOpts(*opts)), C.byref(cst))
+    if rc != 0:
+        raise ValueError(f"oracle_nmmb_prep: bad arguments ({rc})")
+    return cols
+
+
+def nmmb_post(st: dict, dims, cols: dict, rad: dict, czmean, lw_step: bool, sw_step: bool, out: dict):
+    """oracle_nmmb_post: writes the state outputs in `out` (float32 / int32 host arrays) in place."""
+    fp = C.c_float
+    cst = abi.WrNmmbCols(*[_ptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _ptr(cols["clds"], fp), None, None, None)
+    ro = abi.WrOutputs(*[_ptr(rad[k]) for k in abi.OUTPUT_FIELDS])
+    oo = abi.WrNmmbOut(*[_ptr(out[k], fp) for k in abi.NMMB_OUT_FIELDS], _ptr(out["ncfrst"], C.c_int32),
+                       _ptr(out["ncfrcv"], C.c_int32))
+    L = lib()
+    L.oracle_nmmb_post.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbCols), C.POINTER(abi.WrOutputs),
+                                   C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(abi.WrNmmbOut)]
+    cz = np.ascontiguousarray(czmean, dtype=np.float64)
+    rc = L.oracle_nmmb_post(C.byref(_nmmb_state(st, *dims)), C.byref(cst), C.byref(ro), _ptr(cz), int(lw_step),
+                            int(sw_step), C.byref(oo))
+    if rc != 0:
+        raise ValueError(f"oracle_nmmb_post: bad arguments ({rc})")
+
+
+def sol_update(yr, mon, day, hr, mn, sec, deltsw, deltim, solc0):
+    o = abi.WrSol()
+    L = lib()
+    L.oracle_sol_update.argtypes = [C.c_int] * 6 + [C.c_double] * 3 + [C.POINTER(abi.WrSol)]
+    if L.oracle_sol_update(yr, mon, day, hr, mn, sec, deltsw, deltim, solc0, C.byref(o)) != 0:
+        raise ValueError("oracle_sol_update: bad arguments")
+    return o
+
+
+def coszmn(glat, glon, solhr, dtswav, nrads, sol):
+    lat = np.ascontiguousarray(glat, dtype=np.float64)
+    lon = np.ascontiguousarray(glon, dtype=np.float64)
+    cz, cd = np.zeros_like(lat), np.zeros_like(lat)
+    L = lib()
+    dp = C.POINTER(C.c_double)
+    L.oracle_coszmn.argtypes = [C.c_int, dp, dp, C.c_double, C.c_double, C.c_int, C.POINTER(abi.WrSol), dp, dp]
+    L.oracle_coszmn(lat.size, _ptr(lat), _ptr(lon), solhr, dtswav, nrads, C.byref(sol), _ptr(cz), _ptr(cd))
+    return cz, cd
+
+
+def rdtemp(t, czen, glat, glon, czmean, rswtt, rlwtt, ntsd, dt, start, box):
+    """oracle_rdtemp in place on host float32 arrays; start: datetime of the run start; box = (i_lo, i_hi,
+    j_lo, j_hi)."""
+    lm, jm, im = t.shape
+    fp = C.POINTER(C.c_float)
+    L = lib()
+    L.oracle_rdtemp.argtypes = [C.c_int] * 8 + [C.c_double] + [C.c_int] * 4 + [fp] * 7
+    rc = L.oracle_rdtemp(im, jm, lm, box[0], box[1], box[2], box[3], ntsd, dt, start.timetuple().tm_yday, start.year,
+                         start.hour, int(start.year % 4 == 0), _ptr(glat, C.c_float), _ptr(glon, C.c_float),
+                         _ptr(czmean, C.c_float), _ptr(rswtt, C.c_float), _ptr(rlwtt, C.c_float), _ptr(t, C.c_float),
+                         _ptr(czen, C.c_float))
+    if rc != 0:
+        raise ValueError("oracle_rdtemp: bad arguments")
+
+
 def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = None, diag: bool = False):
     """Run the oracle on host float64 columns (gen.py layout).  Returns (outputs, diag|None)."""
     L = lib()
     params = params or abi.make_params()
     t = np.ascontiguousarray(cols["t_lay"], dtype=np.float64)
     nlay, ncol = t.shape
-    arrs = {k: np.ascontiguousarray(cols[k], dtype=np.float64) for k in abi.INPUT_FIELDS}
+    path = cols.get("cwp") is not None
+    arrs = {k: np.ascontiguousarray(cols[k], dtype=np.float64) for k in abi.INPUT_FIELDS + abi.PATH_FIELDS
+            if cols.get(k) is not None}
     ids = cols.get("col_id")
     ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint64)
-    ci = abi.WrColumns(ncol, nlay, *[_ptr(arrs[k]) for k in abi.INPUT_FIELDS],
-                       _ptr(ids, C.c_uint64) if ids is not None else None)
+    ci = abi.WrColumns(ncol, nlay, *[_ptr(arrs.get(k)) for k in abi.INPUT_FIELDS],
+                       _ptr(ids, C.c_uint64) if ids is not None else None, None, None,
+                       *[_ptr(arrs.get(k)) if path else None for k in abi.PATH_FIELDS])
     outs = {k: np.zeros(s) for k, s in abi.output_shapes(ncol, nlay).items()}
     co = abi.WrOutputs(*[_ptr(outs[k]) for k in abi.OUTPUT_FIELDS])
     dg = None

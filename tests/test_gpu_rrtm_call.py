@@ -1,97 +1,168 @@
-"""The NMMB hook: rad_driver.run -> rrtm_call.run(ctx, args) on a small NMMB-shaped state.
+"""The NMMB side of the call on the GPU against the oracle, bit for bit, on a Ferrier-style state:
+total condensate CW with ice-only layers (F_ICE = 1), rain (F_RAIN > 0), layers below -40 C and humid
+layers without condensate (the subgrid cloud).
 
-Checks the state -> column mapping (interface pressures from PT/PD/PDSG1/DSG2, interior rows only),
-the outputs written against a direct column call on the same columns, the cadence and the
-per-step zeroing of the driver."""
-import math
+* wr_nmmb_prep (driver prep, driver call, grrad setup, progcld2 with gethml) against oracle_nmmb_prep;
+* wr_nmmb_post (driver post) against oracle_nmmb_post;
+* the hook: rad_driver.run -> rrtm_call.run against the oracle chain (prep, the FP64 column call on
+  the path-form columns, post), every state output;
+* the driver's cadence and per-step zeroing."""
 from types import SimpleNamespace
 
 import numpy as np
 import pytest
 import torch
 
-from woof_nam_amd import gen, rad_driver, rrtm_call, solar
-from woof_nam_amd.radiation import Radiation
+from oracle import pyoracle
+from woof_nam_amd import abi, gen, rad_driver, rrtm_call, solar
 
 pytestmark = pytest.mark.gpu
 
 IM, JM, LM = 24, 20, 60
 
 
-def make_ctx(device, nrads=3, nradl=3):
-    rng = np.random.default_rng(0)
-    c = gen.columns(IM * JM, seed=77)
+def ferrier_state(seed=0):
+    """Host float32 NMMB state: generated profiles with Ferrier condensate fields."""
+    rng = np.random.default_rng(seed)
+    c = gen.columns(IM * JM, seed=77 + seed)
     sg1, sg2, pdtop = gen._sigma()
-    ps = c["p_lev"][-1]
-    pd = ps - gen.PT - pdtop
-    f32 = lambda a: torch.from_numpy(np.ascontiguousarray(a, dtype=np.float32)).to(device)
-    lay = lambda a: f32(a.reshape(LM, JM, IM))
-    sfc = lambda a: f32(a.reshape(JM, IM))
-    st = {"T": lay(c["t_lay"]), "Q": lay(c["q_lay"]), "O3": lay(c["o3_lay"]), "QC": lay(c["qc"]), "QI": lay(c["qi"]),
-          "PD": sfc(pd), "TSKIN": sfc(c["t_sfc"]), "ALBEDO": sfc(c["alb"]), "SM": sfc(np.ones(IM * JM)),
-          "GLAT": sfc(np.radians(np.repeat(np.linspace(20, 60, JM), IM))),
-          "GLON": sfc(np.radians(np.tile(np.linspace(-120, -60, IM), JM))),
-          "DSG2": f32(np.diff(sg2)), "SGML2": f32(0.5 * (sg2[1:] + sg2[:-1]))}
-    for k in ("RLWTT", "RSWTT", "CLDFRA"):
+    pd = c["p_lev"][-1] - gen.PT - pdtop
+    f32 = lambda a: np.ascontiguousarray(a, dtype=np.float32)
+    lay = lambda a: f32(np.asarray(a).reshape(LM, JM, IM))
+    t = lay(c["t_lay"])
+    cw = lay(c["qc"] + c["qi"])
+    cw[rng.uniform(size=cw.shape) < 0.05] = 3.0e-4                # extra cloudy layers
+    f_ice = f32(rng.uniform(0, 1, cw.shape))
+    f_ice[rng.uniform(size=cw.shape) < 0.3] = 1.0                  # ice-only layers
+    f_rain = f32(rng.uniform(0, 1, cw.shape) * (rng.uniform(size=cw.shape) < 0.4))
+    q = lay(c["q_lay"])
+    humid = (cw == 0) & (rng.uniform(size=cw.shape) < 0.1)          # RH near saturation, no condensate
+    q[humid] = q[humid] * 3.0
+    return {"T": t, "Q": q, "O3": lay(c["o3_lay"]), "CW": cw, "F_ICE": f_ice, "F_RAIN": f_rain,
+            "PD": f32(pd.reshape(JM, IM)), "GLAT": f32(np.radians(np.repeat(np.linspace(20, 60, JM), IM)).reshape(JM, IM)),
+            "GLON": f32(np.radians(np.tile(np.linspace(-120, -60, IM), JM)).reshape(JM, IM)),
+            "PDSG1": f32(np.diff(sg1) * pdtop), "DSG2": f32(np.diff(sg2)),
+            "PSGML1": f32(0.5 * (sg1[1:] + sg1[:-1]) * pdtop + gen.PT), "SGML2": f32(0.5 * (sg2[1:] + sg2[:-1])),
+            "TSKIN": f32(c["t_sfc"].reshape(JM, IM)), "ALBEDO": f32(c["alb"].reshape(JM, IM)),
+            "SM": f32(np.ones((JM, IM)))}
+
+
+def oracle_state(h):
+    m = {"pd": "PD", "pdsg1": "PDSG1", "dsg2": "DSG2", "psgml1": "PSGML1", "sgml2": "SGML2", "t": "T", "q": "Q",
+         "o3": "O3", "cw": "CW", "f_ice": "F_ICE", "f_rain": "F_RAIN", "glat": "GLAT"}
+    return {k: h[v] for k, v in m.items()}
+
+
+def make_ctx(device, h, nrads=3, nradl=3):
+    st = {k: torch.from_numpy(v).to(device) for k, v in h.items() if k not in ("PDSG1", "PSGML1")}
+    for k in ("RLWTT", "RSWTT", "CLDFRA", "P_PHY"):
         st[k] = torch.full((LM, JM, IM), 7.0, dtype=torch.float32, device=device)
     for k in ("RLWIN", "RSWIN", "RSWINC", "RSWOUT", "RLWTOA", "RSWTOA", "CZMEAN", "SIGT4", "CFRACL", "CFRACM",
-              "CFRACH"):
+              "CFRACH", "ACFRST", "ACFRCV"):
         st[k] = torch.full((JM, IM), 7.0, dtype=torch.float32, device=device)
-    grid = SimpleNamespace(im=IM, jm=JM, lm=LM, pt=gen.PT, pdsg1=np.diff(sg1) * pdtop, psgml1=np.zeros(LM))
-    cfg = SimpleNamespace(nrads=nrads, nradl=nradl, nphs=1, shortwave="rrtm", longwave="rrtm", microphysics="gfs",
+    for k in ("NCFRST", "NCFRCV"):
+        st[k] = torch.full((JM, IM), 5, dtype=torch.int32, device=device)
+    grid = SimpleNamespace(im=IM, jm=JM, lm=LM, pt=gen.PT, pdsg1=h["PDSG1"], psgml1=h["PSGML1"])
+    cfg = SimpleNamespace(nrads=nrads, nradl=nradl, nphs=1, shortwave="rrtm", longwave="rrtm", microphysics="fer_hires",
                           cldfraction="default", rrtm_precision="fp64")
     return SimpleNamespace(cfg=cfg, st=st, grid=grid, dt=60.0, ntimestep_rad=3,
                            tables={"JDAT": [2025, 3, 20, 0, 12, 0, 0, 0]}, stream=None)
 
 
-def test_hook_matches_column_call(gpu_device):
-    print("case rrtm_call hook", flush=True)
-    ctx = make_ctx(gpu_device)
-    captured = {}
+def dev_args(h, device):
+    a = {k: torch.from_numpy(v).to(device) for k, v in h.items()}
+    a["PT"] = float(np.float32(gen.PT))
+    return a
 
-    def hook(c, args):
-        captured["args"] = args
-        rrtm_call.run(c, args)
 
-    rad_driver.run(ctx, rrtm_call=hook)
+def test_prep_matches_oracle(gpu_device):
+    print("case nmmb prep", flush=True)
+    h = ferrier_state(1)
+    llyr = rrtm_call.boundary_layer_top(rrtm_call.sigma_interfaces(dev_args(h, gpu_device), LM))
+    args = dev_args(h, gpu_device)
+    args["P_PHY"] = torch.full((LM, JM, IM), -1.0, dtype=torch.float32, device=gpu_device)
+    cols, _ = rrtm_call.prep(args, gpu_device, (1, JM - 1), llyr, iovr=1, subgrid=True)
     torch.cuda.synchronize()
-    args = captured["args"]
-    cols, (c0, c1), _ = rrtm_call.columns(args, gpu_device)
-    assert (c0, c1) == (IM, (JM - 1) * IM)
-    sg1, sg2, pdtop = gen._sigma()
-    pd = args["PD"].double().reshape(-1)[c0:c1]
-    direct = torch.from_numpy(sg1).to(gpu_device)[:, None] * pdtop + torch.from_numpy(sg2).to(gpu_device)[:, None] \
-        * pd[None, :] + gen.PT
-    # the driver hands PDSG1/DSG2 over as float32 vectors (as the reference's driver does), so the
-    # cumulative interface pressures agree with the float64 sigma formula to float32 rounding
-    err = float((cols["p_lev"] - direct).abs().max())
-    assert err < 1.0e-2, f"interface pressure mismatch {err:.3e} Pa"
-    glat = args["GLAT"].reshape(-1)[c0:c1]
-    glon = args["GLON"].reshape(-1)[c0:c1]
-    cols["cosz"] = solar.coszen(args["JDAT"], 3 * 60.0, 3 * 60.0, glat, glon)
-    rad = Radiation(0, max_cols=4096)
-    out, _ = rad.run(cols, params=Radiation.default_params(mcica=0, solcon=args["SOLCON"]))
+    ref = pyoracle.nmmb_prep(oracle_state(h), (IM, JM, LM, 1, JM - 1, np.float32(gen.PT)), (llyr, 1, 1))
+    for k in abi.NMMB_COL_FIELDS + ("clds", "mtop", "mbot"):
+        a = cols[k].cpu().numpy()
+        assert np.array_equal(a, ref[k]), f"{k}: {np.count_nonzero(a != ref[k])} elements differ"
+    assert np.array_equal(args["P_PHY"].cpu().numpy(), ref["p_phy"])
+    # the state exercises every branch: rain, ice-only, subgrid cloud
+    assert (ref["crp"] > 0).any() and ((ref["cip"] > 0) & (ref["cwp"] == 0)).any() and (ref["cldf"] > 0).mean() > 0.05
+    assert (ref["clds"][3] > 0).mean() > 0.3
+
+
+def test_post_matches_oracle(gpu_device):
+    print("case nmmb post", flush=True)
+    h = ferrier_state(2)
+    h["CW"][:, 5, 7] = 0.0                                       # a column without condensate
+    args = dev_args(h, gpu_device)
+    cols, _ = rrtm_call.prep(args, gpu_device, (1, JM - 1), 10)
+    ncol = (JM - 2) * IM
+    rng = np.random.default_rng(3)
+    rad_h = {k: rng.uniform(-1, 300, s) for k, s in abi.output_shapes(ncol, LM).items()}
+    rad_d = {k: torch.from_numpy(v).to(gpu_device) for k, v in rad_h.items()}
+    cz = rng.uniform(0, 1, ncol)
+    out_h = {k: np.full((LM, JM, IM), 7.0, np.float32) for k in ("rlwtt", "rswtt", "cldfra")}
+    out_h.update({k: np.full((JM, IM), 7.0, np.float32) for k in abi.NMMB_OUT_FIELDS if k not in out_h})
+    out_h.update({k: np.full((JM, IM), 5, np.int32) for k in ("ncfrst", "ncfrcv")})
+    for k, v in out_h.items():
+        args[k.upper()] = torch.from_numpy(v.copy()).to(gpu_device)
+    for lw_step, sw_step in ((True, False), (False, True), (True, True)):
+        rrtm_call.post(args, gpu_device, (1, JM - 1), cols, rad_d, torch.from_numpy(cz).to(gpu_device), lw_step, sw_step)
+        ref_cols = {k: v.cpu().numpy() for k, v in cols.items() if k != "_keep"}
+        pyoracle.nmmb_post(oracle_state(h), (IM, JM, LM, 1, JM - 1, np.float32(gen.PT)), ref_cols, rad_h, cz, lw_step,
+                           sw_step, out_h)
+        torch.cuda.synchronize()
+        for k, v in out_h.items():
+            a = args[k.upper()].cpu().numpy()
+            assert np.array_equal(a, v), f"{k} (lw {lw_step}, sw {sw_step}): {np.count_nonzero(a != v)} differ"
+    assert out_h["ncfrst"][3, 3] == 8 and (out_h["cldfra"][:, 5, 7] == 0).all()
+
+
+def test_hook_matches_oracle_chain(gpu_device):
+    """rad_driver.run -> rrtm_call.run against the oracle chain on the same inputs."""
+    print("case rrtm_call hook vs oracle chain", flush=True)
+    h = ferrier_state(3)
+    ctx = make_ctx(gpu_device, h)
+    rad_driver.run(ctx, rrtm_call=rrtm_call.run)
     torch.cuda.synchronize()
     st = ctx.st
-    inner = lambda a: a.reshape(a.shape[0], -1)[:, c0:c1] if a.dim() == 3 else a.reshape(-1)[c0:c1]
-    assert torch.equal(inner(st["RLWTT"]), out["hr_lw"].float())
-    assert torch.equal(inner(st["RSWTT"]), out["hr_sw"].float())
-    assert torch.equal(inner(st["RLWIN"]), out["flw_dn"][-1].float())
-    assert torch.equal(inner(st["RSWOUT"]), out["fsw_up"][-1].float())
-    assert torch.equal(inner(st["RSWINC"]), out["fsw_dn_sfc_clr"].float())
-    assert torch.equal(inner(st["RLWTOA"]), out["flw_up"][0].float())
-    # boundary rows untouched, zeroed fields zero outside the interior
-    assert torch.all(st["RLWTT"][:, 0] == 7.0) and torch.all(st["RLWTT"][:, -1] == 7.0)
-    assert torch.all(st["CFRACL"] == 0.0) and torch.all(st["CZMEAN"][0] == 0.0)
-    # daytime: clear-sky surface SW at least the all-sky value where the sun is up
-    day = inner(st["CZMEAN"]) > 0.05
-    assert bool(day.any()) and torch.all(inner(st["RSWINC"])[day] >= inner(st["RSWIN"])[day] - 1e-3)
-    assert float(inner(st["SIGT4"]).min()) > 200.0
-    rad.close()
+    # the oracle chain on the interior rows (J_S = 2, J_E = JM - 1)
+    d = (IM, JM, LM, 1, JM - 1, np.float32(gen.PT))
+    args = dev_args(h, gpu_device)
+    llyr = rrtm_call.boundary_layer_top(rrtm_call.sigma_interfaces(args, LM))
+    cols = pyoracle.nmmb_prep(oracle_state(h), d, (llyr, 1, 1))
+    c0, c1 = IM, (JM - 1) * IM
+    glat = args["GLAT"].reshape(-1)[c0:c1]
+    glon = args["GLON"].reshape(-1)[c0:c1]
+    cz = solar.coszen(ctx.tables["JDAT"], 3 * 60.0, 3, 60.0, glat, glon)[0].cpu().numpy()
+    flat = lambda a: np.ascontiguousarray(a.reshape(-1)[c0:c1], dtype=np.float64)
+    rc = {k: cols[k] for k in abi.NMMB_COL_FIELDS}
+    rc.update({"cosz": cz, "t_sfc": flat(h["TSKIN"]), "alb": flat(h["ALBEDO"]), "emis": np.ones(c1 - c0),
+               "col_id": np.arange(c0, c1, dtype=np.uint64)})
+    from woof_nam_amd.radiation import Radiation
+    solcon = ctx.tables["rrtm_call_flags"]["solcon"]
+    rad, _ = pyoracle.run(rc, params=Radiation.default_params(mcica=0, solcon=solcon))
+    out = {k: np.full((LM, JM, IM), 7.0, np.float32) for k in ("rlwtt", "rswtt", "cldfra")}
+    out.update({k: np.full((JM, IM), 7.0, np.float32) for k in abi.NMMB_OUT_FIELDS if k not in out})
+    out.update({k: np.full((JM, IM), 5, np.int32) for k in ("ncfrst", "ncfrcv")})
+    for k in ("cldfra", "cfracl", "cfracm", "cfrach", "czmean", "sigt4"):
+        out[k][...] = 0.0                                         # the driver zeroes these every step
+    pyoracle.nmmb_post(oracle_state(h), d, cols, rad, cz, True, True, out)
+    for k, v in out.items():
+        a = st[k.upper()].cpu().numpy()
+        assert np.array_equal(a, v), f"{k.upper()}: {np.count_nonzero(a != v)} elements differ from the oracle chain"
+    assert np.array_equal(st["P_PHY"].cpu().numpy()[:, 1:JM - 1], cols["p_phy"][:, 1:JM - 1])
+    # what the state shows: heating where there is cloud, ice-only and rainy layers radiated
+    assert (st["CLDFRA"].cpu().numpy()[:, 1:-1] > 0).mean() > 0.05
+    assert float(st["RSWTT"].abs().max()) > 0 and float(st["RLWTT"].abs().max()) > 0
 
 
 def test_cadence_and_zeroing(gpu_device):
-    ctx = make_ctx(gpu_device, nrads=3, nradl=3)
+    h = ferrier_state(4)
+    ctx = make_ctx(gpu_device, h, nrads=3, nradl=3)
     calls = []
     ctx.ntimestep_rad = 4                       # not a radiation step
     rad_driver.run(ctx, rrtm_call=lambda c, a: calls.append(a))

@@ -1,6 +1,6 @@
-"""Host logic of the NMMB hook on CPU tensors (no GPU): the driver's cadence and per-step zeroing,
-and the state -> column mapping of rrtm_call.columns (interface pressures, interior rows, the NAM
-condensate cloud fraction, column ids)."""
+"""Host logic of the NMMB hook on CPU tensors (no GPU): the driver's cadence and per-step zeroing, the
+boundary-layer cloud top, the checks of the state arguments, the tile sized from free memory and the
+refused McICA route."""
 from types import SimpleNamespace
 
 import numpy as np
@@ -21,6 +21,7 @@ def make_ctx(nrads=3, nradl=3):
     lay = lambda a: f32(a.reshape(LM, JM, IM))
     sfc = lambda a: f32(a.reshape(JM, IM))
     st = {"T": lay(c["t_lay"]), "Q": lay(c["q_lay"]), "O3": lay(c["o3_lay"]), "QC": lay(c["qc"]), "QI": lay(c["qi"]),
+          "CW": lay(c["qc"] + c["qi"]), "F_ICE": lay(np.zeros(LM * IM * JM)), "F_RAIN": lay(np.zeros(LM * IM * JM)),
           "PD": sfc(pd), "TSKIN": sfc(c["t_sfc"]), "ALBEDO": sfc(c["alb"]), "SM": sfc(np.ones(IM * JM)),
           "GLAT": sfc(np.radians(np.repeat(np.linspace(20, 60, JM), IM))),
           "GLON": sfc(np.radians(np.tile(np.linspace(-120, -60, IM), JM))),
@@ -28,8 +29,10 @@ def make_ctx(nrads=3, nradl=3):
     for k in ("RLWTT", "RSWTT", "CLDFRA"):
         st[k] = torch.full((LM, JM, IM), 7.0)
     for k in ("RLWIN", "RSWIN", "RSWINC", "RSWOUT", "RLWTOA", "RSWTOA", "CZMEAN", "SIGT4", "CFRACL", "CFRACM",
-              "CFRACH"):
+              "CFRACH", "ACFRST", "ACFRCV"):
         st[k] = torch.full((JM, IM), 7.0)
+    for k in ("NCFRST", "NCFRCV"):
+        st[k] = torch.zeros((JM, IM), dtype=torch.int32)
     grid = SimpleNamespace(im=IM, jm=JM, lm=LM, pt=gen.PT, pdsg1=np.diff(sg1) * pdtop, psgml1=np.zeros(LM))
     cfg = SimpleNamespace(nrads=nrads, nradl=nradl, nphs=1, shortwave="rrtm", longwave="rrtm", microphysics="gfs",
                           cldfraction="default")
@@ -67,44 +70,36 @@ def test_unsupported_schemes_raise():
         rad_driver.run(ctx, rrtm_call=lambda c, a: None)
 
 
-def test_columns_mapping():
+def test_boundary_layer_top_rule():
+    """cld_init's llyr (radiation_clouds_nmmb.f:327-344, ivflip = 0): the last k = LM..2 before the first
+    sigma interface below 0.9, counting from the surface."""
+    si = np.linspace(0.0, 1.0, 61)                   # si(k) = (k - 1) / 60
+    assert rrtm_call.boundary_layer_top(si) == 54    # si(54) = 0.883 < 0.9 <= si(55)
+    assert rrtm_call.boundary_layer_top(np.full(61, 0.95)) == 2     # none below 0.9: the loop ends at 2
+    ctx = make_ctx()
+    ctx.ntimestep_rad = 1
+    got = []
+    rad_driver.run(ctx, rrtm_call=lambda c, a: got.append(a))
+    si = rrtm_call.sigma_interfaces(got[0], LM)
+    assert si[0] == 0.0 and si[-1] == pytest.approx(1.0) and np.all(np.diff(si) > 0)
+    llyr = rrtm_call.boundary_layer_top(si)
+    assert si[llyr - 1] < 0.9 <= si[llyr] and 40 < llyr < LM
+
+
+def test_state_arguments_checked_before_any_device_work():
     ctx = make_ctx()
     ctx.ntimestep_rad = 1
     got = []
     rad_driver.run(ctx, rrtm_call=lambda c, a: got.append(a))
     args = got[0]
-    cols, (c0, c1), (im, jm, lm) = rrtm_call.columns(args, CPU)
-    assert (c0, c1) == (IM, (JM - 1) * IM) and (im, jm, lm) == (IM, JM, LM)
-    # interface pressures: PT + cumulative (PDSG1 + DSG2 PD) over the interior rows
-    pd = ctx.st["PD"].double().reshape(-1)[c0:c1].numpy()
-    pdsg1 = np.asarray(args["PDSG1"].double())
-    dsg2 = ctx.st["DSG2"].double().numpy()
-    ref = gen.PT + np.concatenate([np.zeros((1, c1 - c0)), np.cumsum(pdsg1[:, None] + dsg2[:, None] * pd[None, :], 0)])
-    assert np.allclose(cols["p_lev"].numpy(), ref, rtol=0, atol=1e-6)
-    assert np.all(np.diff(cols["p_lev"].numpy(), axis=0) > 0)
-    # NAM condensate cloud fraction (progcld2, ivflip = 0): min(1, 1e5 clw p/1000) where clw > 1e-8 p/1000
-    pl = 0.5 * (ref[1:] + ref[:-1]) * 1.0e-2
-    clw = (ctx.st["QC"].double().reshape(LM, -1)[:, c0:c1] + ctx.st["QI"].double().reshape(LM, -1)[:, c0:c1]).numpy()
-    cf = np.where(clw > 1.0e-8 * (pl * 1.0e-3), np.minimum(1.0e5 * clw * pl * 1.0e-3, 1.0), 0.0)
-    assert np.allclose(cols["cldf"].numpy(), cf, rtol=1e-12, atol=0)
-    assert 0.0 < float(cols["cldf"].mean()) < 1.0
-    # layer fields are the interior rows of the state, in float64, top first
-    t = ctx.st["T"].double().reshape(LM, -1)[:, c0:c1]
-    assert torch.equal(cols["t_lay"], t)
-    assert torch.equal(cols["col_id"], torch.arange(c0, c1))
-    assert torch.all(cols["emis"] == 1.0)
-
-
-def test_columns_rejects_bad_rows_and_shapes():
-    ctx = make_ctx()
-    ctx.ntimestep_rad = 1
-    got = []
-    rad_driver.run(ctx, rrtm_call=lambda c, a: got.append(a))
-    args = dict(got[0])
+    st, keep, dims = rrtm_call.nmmb_state(args, CPU, (1, JM - 1))
+    assert dims == (IM, JM, LM) and (st.j0, st.j1) == (1, JM - 1)
     with pytest.raises(ValueError):
-        rrtm_call.columns({**args, "J_S": JM + 3}, CPU)
+        rrtm_call.nmmb_state({**args, "CW": args["CW"].double()}, CPU, (1, JM - 1))     # dtype
     with pytest.raises(ValueError):
-        rrtm_call.columns({**args, "PD": args["PD"][:-1]}, CPU)
+        rrtm_call.nmmb_state({**args, "PD": args["PD"][:-1]}, CPU, (1, JM - 1))          # shape
+    with pytest.raises(KeyError):
+        rrtm_call.nmmb_state({k: v for k, v in args.items() if k != "F_RAIN"}, CPU, (1, JM - 1))
 
 
 def test_tile_sized_from_free_memory():

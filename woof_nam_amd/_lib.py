@@ -18,7 +18,8 @@ TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", "wr_run_mixed", "wr_run_mixed_noalloc", "wr_run_band", "wr_cal_cldfra",
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
-           "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate")
+           "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate", "wr_nmmb_prep", "wr_nmmb_post",
+           "wr_sol_update", "wr_coszmn", "wr_rdtemp")
 
 _ERRORS = {
     abi.WR_ERR_ARG: ValueError,
@@ -85,6 +86,18 @@ def load():
     L.wr_run_surrogate.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrParams), C.POINTER(abi.WrOutputs),
                                    vp, vp]
     L.wr_run_surrogate.restype = C.c_int
+    L.wr_nmmb_prep.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbOpts), C.POINTER(abi.WrNmmbCols), vp]
+    L.wr_nmmb_prep.restype = C.c_int
+    L.wr_nmmb_post.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbCols), C.POINTER(abi.WrOutputs), vp,
+                               C.c_int32, C.c_int32, C.POINTER(abi.WrNmmbOut), vp]
+    L.wr_nmmb_post.restype = C.c_int
+    i32, f64 = C.c_int32, C.c_double
+    L.wr_sol_update.argtypes = [i32] * 6 + [f64, f64, f64, C.POINTER(abi.WrSol)]
+    L.wr_sol_update.restype = C.c_int
+    L.wr_coszmn.argtypes = [i32, vp, vp, f64, f64, i32, C.POINTER(abi.WrSol), vp, vp, vp]
+    L.wr_coszmn.restype = C.c_int
+    L.wr_rdtemp.argtypes = [i32] * 8 + [f64] + [i32] * 4 + [vp] * 7 + [vp]
+    L.wr_rdtemp.restype = C.c_int
     L.wr_strerror.argtypes = [C.c_int]
     L.wr_strerror.restype = C.c_char_p
     L.wr_destroy.argtypes = [vp]

@@ -21,7 +21,8 @@ class WrColumns(C.Structure):
                 ("p_lev", _dp), ("t_lay", _dp), ("q_lay", _dp), ("o3_lay", _dp), ("cldf", _dp),
                 ("qc", _dp), ("qi", _dp), ("t_sfc", _dp), ("emis", _dp), ("alb", _dp), ("cosz", _dp),
                 ("col_id", C.POINTER(C.c_uint64)), ("seed_dev", C.POINTER(C.c_uint64)),
-                ("ncol_dev", C.POINTER(C.c_int32))]
+                ("ncol_dev", C.POINTER(C.c_int32)),
+                ("cwp", _dp), ("cip", _dp), ("crp", _dp), ("rew", _dp), ("rei", _dp)]
 
 
 class WrParams(C.Structure):
@@ -47,6 +48,47 @@ class WrSurrogate(C.Structure):
 
 
 INPUT_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi", "t_sfc", "emis", "alb", "cosz")
+# the optional path form of the clouds (progcld2's paths and radii; cwp set: qc/qi are not read)
+PATH_FIELDS = ("cwp", "cip", "crp", "rew", "rei")
+
+_fp = C.POINTER(C.c_float)
+
+
+class WrSol(C.Structure):
+    _fields_ = [("sollag", C.c_double), ("sindec", C.c_double), ("cosdec", C.c_double), ("anginc", C.c_double),
+                ("solcon", C.c_double), ("nstp", C.c_int32)]
+
+
+class WrNmmbState(C.Structure):
+    _fields_ = [("im", C.c_int32), ("jm", C.c_int32), ("lm", C.c_int32), ("j0", C.c_int32), ("j1", C.c_int32),
+                ("pt", C.c_float)] + [(n, _fp) for n in ("pd", "pdsg1", "dsg2", "psgml1", "sgml2", "t", "q", "o3",
+                                                          "cw", "f_ice", "f_rain", "glat")]
+
+
+class WrNmmbOpts(C.Structure):
+    _fields_ = [("llyr", C.c_int32), ("iovr", C.c_int32), ("subgrid", C.c_int32)]
+
+
+NMMB_COL_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "cwp", "cip", "crp", "rew", "rei")
+
+
+class WrNmmbCols(C.Structure):
+    _fields_ = [(n, _dp) for n in NMMB_COL_FIELDS] + [("clds", _fp), ("mtop", _ip), ("mbot", _ip), ("p_phy", _fp)]
+
+
+NMMB_OUT_FIELDS = ("rlwtt", "rswtt", "cldfra", "rlwin", "rlwtoa", "sigt4", "rswin", "rswout", "rswinc", "rswtoa",
+                   "czmean", "cfracl", "cfracm", "cfrach", "acfrst", "acfrcv")
+
+
+class WrNmmbOut(C.Structure):
+    _fields_ = [(n, _fp) for n in NMMB_OUT_FIELDS] + [("ncfrst", _ip), ("ncfrcv", _ip)]
+
+
+def nmmb_col_shapes(ncol: int, lm: int) -> dict:
+    s = {"p_lev": (lm + 1, ncol)}
+    for k in NMMB_COL_FIELDS[1:]:
+        s[k] = (lm, ncol)
+    return s
 OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn", "fsw_dn_sfc_clr")
 DIAG_FIELDS = ("lw_jp", "lw_jt", "lw_jt1", "lw_laytrop", "sw_jp", "sw_jt", "sw_jt1", "sw_laytrop",
                "lw_mask", "sw_mask")

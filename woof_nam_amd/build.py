@@ -12,6 +12,11 @@ compiler's resource-usage remarks and the build fails when one exceeds ``SCRATCH
 the runtime backs the private segment of every queue that runs a kernel, for every lane in flight,
 and large per-lane arrays have exhausted it (an HSA out-of-resources queue abort).  The per-kernel
 table is written to ``_build/resource_usage.txt``.
+
+The build also fails when a kernel that the host pass registers (by its host-side mangled name) has no
+device symbol of that name: a kernel parameter whose type differs between the passes (a pointer
+qualified with a device address space, declared only in the device pass) mangles differently, and
+launching it aborts the process because the runtime finds no such kernel in the code object.
 """
 from __future__ import annotations
 
@@ -29,7 +34,7 @@ LIB = PKG / "libwoofrad.so"
 OBJ = PKG / "_build"
 ARCH = os.environ.get("WR_OFFLOAD_ARCH", "gfx950")
 
-UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_mixed.hip", "wr_mlp.hip", "wr_surr.hip"]
+UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_mixed.hip", "wr_mlp.hip", "wr_surr.hip", "wr_nmmb.hip", "wr_solar.hip"]
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -76,6 +81,20 @@ def resource_usage(stderr: str) -> list:
             for r in rows]
 
 
+_REGISTER = re.compile(r"@__hipRegisterFunction\(ptr [^,]+, ptr (?:nonnull )?@\"?([\w.$]+)\"?,")
+
+
+def registered_kernels(src: Path) -> list:
+    """Kernel names the host pass of a unit registers with the runtime (its host-side mangling)."""
+    extra = os.environ.get("WR_CFLAGS_EXTRA", "").split()
+    cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), *extra, "--cuda-host-only", "-emit-llvm", "-S",
+           "-o", "-", str(src)]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc (host pass) failed for {src.name}:\n{r.stderr}")
+    return _REGISTER.findall(r.stdout)
+
+
 def _compile(src: Path, obj: Path) -> list:
     extra = os.environ.get("WR_CFLAGS_EXTRA", "").split()      # tuning builds, e.g. -DWR_BAND_WAVES=4
     cmd = [hipcc(), *CFLAGS, *UNIT_FLAGS.get(src.name, []), *extra, "-Rpass-analysis=kernel-resource-usage",
@@ -83,7 +102,13 @@ def _compile(src: Path, obj: Path) -> list:
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src.name}:\n{r.stderr}")
-    return [(src.name, *row) for row in resource_usage(r.stderr)]
+    rows = [(src.name, *row) for row in resource_usage(r.stderr)]
+    device = {row[1] for row in rows}
+    missing = [k for k in registered_kernels(src) if k not in device]
+    if missing:
+        raise RuntimeError(f"{src.name}: kernels registered by the host pass with no device symbol of the same "
+                           f"name (a parameter type that differs between the passes): {missing}")
+    return rows
 
 
 def build(verbose: bool = False, out: Path | None = None) -> Path:

@@ -103,6 +103,7 @@ struct ColIn {
     const WR_G uint64_t *col_id;
     const WR_G uint64_t *seed_dev;  // optional device copy of the McICA seed
     const WR_G int32_t *ncol_dev;   // optional device count of valid columns (a shorter replayed tile)
+    const WR_G double *cwp, *cip, *crp, *rew, *rei;  // optional path form of the clouds (cwp set)
     int ncol;  // stride of the [k][column] fields
     int nlay;  // model layers
 };
@@ -175,6 +176,15 @@ struct RadCol {
         if (l == 0) return;
         const int k0 = l - 1;
         ct = fld(in.cldf, k0);
+        if (in.cwp) {  // path form: progcld2's paths as given
+            if (ct < 0.001) {
+                ct = 0.0;
+                return;
+            }
+            cwp = fld(in.cwp, k0);
+            cip = fld(in.cip, k0);
+            return;
+        }
         const double qc = fld(in.qc, k0), qi = fld(in.qi, k0);
         double qcwat, qcice;
         if (tlyr(l) - 273.15 > -40.0) {
@@ -193,6 +203,17 @@ struct RadCol {
             ct = 0.0;
             cwp = 0.0;
             cip = 0.0;
+        }
+    }
+    // effective radii (micron) and rain path (g/m2) of cloudy layer l: the call's radii and no rain, or
+    // the path form's per-layer values
+    __device__ __forceinline__ void props(int l, double re_liq, double re_ice, double &rl, double &ri, double &rp) const
+    {
+        rl = re_liq, ri = re_ice, rp = 0.0;
+        if (in.cwp && l > 0) {
+            rl = fld(in.rew, l - 1);
+            ri = fld(in.rei, l - 1);
+            if (in.crp) rp = fld(in.crp, l - 1);
         }
     }
     __device__ __forceinline__ LayV lay(int l) const

@@ -176,7 +176,7 @@ __global__ void __launch_bounds__(64, WR_GLW_WAVES) k_glw(const Tabs<R> *__restr
                 lw_optics<R, B, 1>(T, L, &tau, &fr, gg);
             const R blay = planck<R>(T, B, L.tavel);
             const R dplnkd = planck<R>(T, B, tz_bot) - blay;
-            const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+            const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
             const GasT<R> q = lw_gas<R>(T, sdif, tau + (R)0);
             const R plfrac = fr;
             const R bbdgas = plfrac * (blay + dplnkd * q.gasfac);
@@ -225,7 +225,7 @@ __global__ void __launch_bounds__(64, WR_GLW_WAVES) k_glw(const Tabs<R> *__restr
                 lw_optics<R, B, 1>(T, L, &tau, &fr, gg);
             const R blay = planck<R>(T, B, L.tavel);
             const R dplnku = planck<R>(T, B, tzt) - blay;
-            const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+            const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
             const GasT<R> q = lw_gas<R>(T, sdif, tau + (R)0);
             const R plfrac = fr;
             const R bbugas = plfrac * (blay + dplnku * q.gasfac);

@@ -366,9 +366,10 @@ int check_args(const wr_ctx *c, const wr_columns *in, const wr_params *p, const 
     if (!c || !in || !p || !o) return WR_ERR_ARG;
     if (in->ncol < 0 || in->nlay < 2 || in->nlay + 1 > c->nl_max) return WR_ERR_ARG;
     if (in->ncol == 0) return WR_OK;
-    if (!in->p_lev || !in->t_lay || !in->q_lay || !in->o3_lay || !in->cldf || !in->qc || !in->qi || !in->t_sfc ||
-        !in->emis || !in->alb || !in->cosz)
+    if (!in->p_lev || !in->t_lay || !in->q_lay || !in->o3_lay || !in->cldf || !in->t_sfc || !in->emis || !in->alb ||
+        !in->cosz)
         return WR_ERR_ARG;
+    if (in->cwp ? (!in->cip || !in->rew || !in->rei) : (!in->qc || !in->qi)) return WR_ERR_ARG;
     if (!o->hr_lw || !o->hr_sw || !o->flw_up || !o->flw_dn || !o->fsw_up || !o->fsw_dn) return WR_ERR_ARG;
     if (p->iovr < 0 || p->iovr > 2) return WR_ERR_ARG;
     if (!(p->re_liq > 0.0) || !(p->re_ice > 0.0) || !(p->solcon > 0.0)) return WR_ERR_ARG;
@@ -382,6 +383,7 @@ ColIn to_colin(const wr_columns *in)
     ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
     ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
     ci.seed_dev = gptr(in->seed_dev), ci.ncol_dev = gptr(in->ncol_dev);
+    ci.cwp = gptr(in->cwp), ci.cip = gptr(in->cip), ci.crp = gptr(in->crp), ci.rew = gptr(in->rew), ci.rei = gptr(in->rei);
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     return ci;
 }

@@ -541,7 +541,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
-        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
         R s = (R)0;
 #pragma unroll
         for (int jj = 0; jj < NGL; ++jj) {
@@ -617,7 +617,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             s0 = gsum<GS>(s0);
             if (lead) acc.lw_up[t] = s0;
         }
-        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
         R s = (R)0;
 #pragma unroll
         for (int jj = 0; jj < NGL; ++jj) {
@@ -701,7 +701,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R dplnku = planck<R>(T, B, tz_top) - blay;
-        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
         R s = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -829,7 +829,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
         sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
     }
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
-    if (cf1 > 0.0 && L.ct > FTINY) sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
+    if (cf1 > 0.0 && L.ct > FTINY) sw_cloud_l<R>(T, B, L, rc, kin, par, ctau, cssa, casy);
 #pragma unroll
     for (int g = 0; g < NG; ++g)
         layer_rt_eval<R>(T, tr[g], tg[g], (mb[g] >> k) & 1, ctau, cssa, casy, cz, snt, o.rb[g], o.rd[g], o.tb[g],

@@ -134,7 +134,7 @@ template <typename R> __device__ __forceinline__ R planck(const Tabs<R> *__restr
 // LW cloud optical depth of band b (phys_rrtm_lw_cldprop.cu:79, ilwcliq = 1, ilwcice = 1)
 template <typename R>
 __device__ __forceinline__ R lw_taucld(const Tabs<R> *__restrict__ T, int b, int lcf1, double ct, double cldliq,
-                                       double cldice, double refliq, double refice)
+                                       double cldice, double refliq, double refice, double cldran = 0.0)
 {
     if (!lcf1 || !(ct > 0.0)) return (R)0;
     R tauliq = 0, tauice = 0;
@@ -151,7 +151,20 @@ __device__ __forceinline__ R lw_taucld(const Tabs<R> *__restrict__ T, int b, int
         const int ia0 = T->ipat[b] - 1;
         tauice = fmax((R)0, (R)cldice * (T->absice1[ia0 * 2 + 0] + T->absice1[ia0 * 2 + 1] / ri));
     }
-    return tauice + tauliq + (R)0 + (R)0;  // + tauran (rain path 0) + tausnw (snow path 0)
+    const R tauran = (R)(0.33e-3 * cldran);  // absrain (NCAR) x rain path
+    return tauice + tauliq + tauran + (R)0;  // + tausnw (snow path 0)
+}
+
+// lw_taucld of radiation layer kin of column rc, with the layer's radii and rain path (path form)
+// loaded only for a cloudy layer
+template <typename R, typename L_T, typename RC_T, typename P_T>
+__device__ __forceinline__ R lw_taucld_l(const Tabs<R> *__restrict__ T, int b, int lcf1, const L_T &L, const RC_T &rc,
+                                         int kin, const P_T &par)
+{
+    if (!lcf1 || !(L.ct > 0.0)) return (R)0;
+    double rl, ri, rp;
+    rc.props(kin, par.re_liq, par.re_ice, rl, ri, rp);
+    return lw_taucld<R>(T, b, 1, L.ct, L.clwp, L.ciwp, rl, ri, rp);
 }
 
 // ---- band optics ------------------------------------------------------------------------------

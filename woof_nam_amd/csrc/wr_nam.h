@@ -246,7 +246,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, L, tau, fr, 0);
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
-        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
         const R clfr = (R)lw_cldfrc(rc, kk);
         const bool top = (lstd >> (kk - 1)) & 1;
         const R c1 = rt.at(FCLR1D, kk - 1, t), d1 = rt.at(FCLD1D, kk - 1, t), m1 = rt.at(FCMB1D, kk - 1, t);
@@ -329,7 +329,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
             }
             acc.lw_up[t] = s0;
         }
-        const R taucld = lw_taucld<R>(T, B, lcf1, L.ct, L.clwp, L.ciwp, par.re_liq, par.re_ice);
+        const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
         const R clfr = (R)lw_cldfrc(rc, kk);
         const bool bot = (lstu >> (kk - 1)) & 1;
         const R c1 = rt.at(FCLR1U, kk + 1, t), d1 = rt.at(FCLD1U, kk + 1, t), m1 = rt.at(FCMB1U, kk + 1, t);
@@ -389,7 +389,7 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
     if (!(zc1d > FTINY)) return;
     const R zc1 = (R)zc1d, zc0 = (R)(1.0 - zc1d);
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
-    sw_cloud<R>(T, B, L.cwp, par.re_liq, L.cip, par.re_ice, ctau, cssa, casy);
+    sw_cloud_l<R>(T, B, L, rc, kin, par, ctau, cssa, casy);
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
         R rb, rd, tb, td, ld;

@@ -324,7 +324,7 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
 // cloud optical properties of band B for one layer (taucw, ssacw = tau*ssa, asycw = tau*ssa*g)
 template <typename R>
 __device__ __forceinline__ void sw_cloud(const Tabs<R> *__restrict__ T, int B, double cldliq, double refliq, double cldice,
-                                         double refice, R &tau, R &ssa, R &asy)
+                                         double refice, R &tau, R &ssa, R &asy, double cldran = 0.0)
 {
     R tauliq = 0, ssaliq = 0, asyliq = 0, tauice = 0, ssaice = 0, asyice = 0;
     if (!(cldliq <= 0.0)) {
@@ -352,10 +352,25 @@ __device__ __forceinline__ void sw_cloud(const Tabs<R> *__restrict__ T, int B, d
         ssaice = tauice * ssaco;
         asyice = ssaice * asyco;
     }
-    // rain path is zero on this path: tauran = ssaran = asyran = 0; snow path zero
-    tau = tauliq + tauice + (R)0 + (R)0;
-    ssa = ssaliq + ssaice + (R)0 + (R)0;
-    asy = asyliq + asyice + (R)0 + (R)0;
+    // rain (a0r, b0r, c0r of radsw_datatb); snow path zero
+    constexpr double B0R[14] = {0.466, 0.437, 0.416, 0.391, 0.374, 0.352, 0.183, 0.048, 0.012, 0.0, 0.0, 0.0, 0.0, 0.496};
+    constexpr double C0R[14] = {0.975, 0.965, 0.960, 0.955, 0.952, 0.950, 0.944, 0.894, 0.884, 0.883, 0.883, 0.883, 0.883, 0.980};
+    const double tauran = cldran * 3.07e-3;
+    const double ssaran = tauran * (1.0 - B0R[B]);
+    const double asyran = ssaran * C0R[B];
+    tau = tauliq + tauice + (R)tauran + (R)0;
+    ssa = ssaliq + ssaice + (R)ssaran + (R)0;
+    asy = asyliq + asyice + (R)asyran + (R)0;
+}
+
+// sw_cloud of radiation layer kin of column rc with the layer's radii and rain path (path form)
+template <typename R>
+__device__ __forceinline__ void sw_cloud_l(const Tabs<R> *__restrict__ T, int B, const SwLay<R> &L, const RadCol &rc, int kin,
+                                           const RunPar &par, R &tau, R &ssa, R &asy)
+{
+    double rl, ri, rp;
+    rc.props(kin, par.re_liq, par.re_ice, rl, ri, rp);
+    sw_cloud<R>(T, B, L.cwp, rl, L.cip, ri, tau, ssa, asy, rp);
 }
 
 template <typename R> __device__ __forceinline__ R expm(R zb, const WR_G R *__restrict__ tbl)

@@ -28,7 +28,8 @@ ZEROED = ("CLDFRA", "CFRACH", "CFRACL", "CFRACM", "CZMEAN", "SIGT4")
 
 
 def default_radupdate(ctx, *, jdat, dtsw, dtx, lssw):
-    return {"SOLCON": solar.solcon(jdat)}
+    """The solar constant at the date of the step (sol_update_nmmb, JDAT + NTIMESTEP DT)."""
+    return {"SOLCON": solar.solcon(jdat, float(ctx.ntimestep_rad) * dtx, dtsw, dtx)}
 
 
 def rrtm_args(ctx, jdat, nts, dt, flags, solcon):
@@ -48,16 +49,17 @@ def rrtm_args(ctx, jdat, nts, dt, flags, solcon):
          "F_QR": bool(flags["F_QR"]), "F_QG": bool(flags["F_QG"]), "F_NI": bool(flags["F_NI"]),
          "NUM_WATER": int(flags["num_water"]), "CLD_FRACTION": 0,
          "IMS": 1, "IME": im, "JMS": 1, "JME": jm, "I_S": 1, "I_E": im, "J_S": 2, "J_E": jm - 1, "LM": lm}
-    for k in ("GLAT", "GLON", "DSG2", "SGML2", "PD", "T", "Q", "O3", "ALBEDO", "TSKIN", "SM", "CLDFRA",
-              "RLWTT", "RSWTT", "RLWIN", "RSWIN", "RSWINC", "RSWOUT", "RLWTOA", "RSWTOA", "CZMEAN", "SIGT4",
-              "CFRACL", "CFRACM", "CFRACH"):
+    for k in ("GLAT", "GLON", "DSG2", "SGML2", "PD", "T", "Q", "O3", "CW", "F_ICE", "F_RAIN", "ALBEDO", "TSKIN",
+              "SM", "CLDFRA", "RLWTT", "RSWTT", "RLWIN", "RSWIN", "RSWINC", "RSWOUT", "RLWTOA", "RSWTOA", "CZMEAN",
+              "SIGT4", "CFRACL", "CFRACM", "CFRACH", "ACFRST", "ACFRCV", "NCFRST", "NCFRCV"):
         if k not in st:
             raise KeyError(f"rad-driver: state field {k!r} missing (its producer has not run)")
         a[k] = st[k]
     a["QCW"] = st["QC"] if "QC" in st else zero3
     a["QCI"] = st["QI"] if (flags["F_QI"] and "QI" in st) else zero3
-    if "EMISS" in st:
-        a["EMISS"] = st["EMISS"]
+    for k in ("EMISS", "P_PHY"):
+        if k in st:
+            a[k] = st[k]
     return a
 
 

@@ -30,13 +30,18 @@ def _check_field(name, t, shape, dtype, device):
         raise ValueError(f"{name}: must be contiguous")
 
 
-def input_shapes(ncol: int, nlay: int) -> dict:
+def input_shapes(ncol: int, nlay: int, path: bool = False) -> dict:
+    """Shapes of the column inputs: qc/qi, or with path=True the path form (cwp, cip, crp, rew, rei)."""
     s = {"p_lev": (nlay + 1, ncol)}
-    for k in ("t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi"):
+    for k in ("t_lay", "q_lay", "o3_lay", "cldf") + (abi.PATH_FIELDS if path else ("qc", "qi")):
         s[k] = (nlay, ncol)
     for k in ("t_sfc", "emis", "alb", "cosz"):
         s[k] = (ncol,)
     return s
+
+
+def _is_path(cols: dict) -> bool:
+    return cols.get("cwp") is not None
 
 
 def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
@@ -50,10 +55,11 @@ def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
     if nd is not None:
         _check_field("ncol_dev", nd, (1,), torch.int32, device)
     u64 = C.POINTER(C.c_uint64)
-    return abi.WrColumns(ncol, nlay, *[_dptr(cols[k]) for k in abi.INPUT_FIELDS],
+    return abi.WrColumns(ncol, nlay, *[_dptr(cols.get(k)) for k in abi.INPUT_FIELDS],
                          C.cast(C.c_void_p(ids.data_ptr()), u64) if ids is not None else None,
                          C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None,
-                         C.cast(C.c_void_p(nd.data_ptr()), C.POINTER(C.c_int32)) if nd is not None else None)
+                         C.cast(C.c_void_p(nd.data_ptr()), C.POINTER(C.c_int32)) if nd is not None else None,
+                         *[_dptr(cols.get(k)) for k in abi.PATH_FIELDS])
 
 
 class Radiation:
@@ -143,7 +149,9 @@ class Radiation:
             raise ValueError(f"precision must be 'fp64', 'mixed' or 'surrogate', got {precision!r}")
         t = cols["t_lay"]
         nlay, ncol = int(t.shape[0]), int(t.shape[1])
-        for k, shp in input_shapes(ncol, nlay).items():
+        for k, shp in input_shapes(ncol, nlay, _is_path(cols)).items():
+            if k == "crp" and cols.get(k) is None:
+                continue
             if k not in cols:
                 raise KeyError(f"missing input field {k!r}")
             _check_field(k, cols[k], shp, torch.float64, self.device)

@@ -7,22 +7,25 @@ holding device arrays aliased to the model state ([LM, JM, IM] float32, level 1 
 or [JM, IM]); the hook writes the radiation outputs into the output arrays of ``args`` in place and
 returns None.  The input species arrays are only read.
 
-What it does on a call:
+What it does on a call, all of it on the device in library kernels:
   * cadence: SW on NTIMESTEP % NRADS == 0, LW on NTIMESTEP % NRADL == 0 (rrtm_call.py:253-254);
-  * driver prep: interface pressures P8W = PT + cumulative (PDSG1 + DSG2 * PD)
-    (phys_rrtm_driver_prep.cu:82), the interior rows J_S..J_E only (phys_rad_driver.py:165-166);
-  * clouds: with CLD_FRACTION = 0 the NAM condensate cloud fraction of progcld2
-    (phys_rrtm_progcld2.cu:218-231, ivflip = 0: min(1, 1e5 clw p/1000) where clw > 1e-8 p/1000,
-    p in hPa) from cloud water QCW and ice QCI; otherwise the CLDFRA field.  Effective radii are
-    the call parameters (re_liq, re_ice), not progcld2's diagnosed radii;
-  * solar geometry: cos zenith averaged over the SW interval (woof_nam_amd.solar);
-  * the radiation call (woof_nam_amd.radiation, the NAM route mcica = 0 unless ctx.cfg says
-    otherwise; FP64 unless ctx.cfg.rrtm_precision = "mixed");
-  * driver post (phys_rrtm_driver_post.cu:71-150): RLWTT, RSWTT (K/s), RLWIN, RSWIN, RSWOUT,
-    RSWINC, RLWTOA, RSWTOA, CZMEAN, and SIGT4 = sigma T(LM)^4 on LW steps.
-The cloud-cover diagnostics (CFRACL/M/H and their accumulators) are not produced.
+  * wr_nmmb_prep on the interior rows J_S..J_E (phys_rad_driver.py:165-166): driver prep (P8W,
+    P_PHY, phys_rrtm_driver_prep.cu:82), the driver call's condensate and subgrid cloud
+    (phys_rrtm_driver_call.cu:200-275), grrad setup, and progcld2 with gethml
+    (phys_rrtm_progcld2.cu:92-450) in the NAM configuration: Ferrier total condensate CW split by
+    F_ICE / F_RAIN into cloud water, ice and rain, the condensate cloud fraction, paths and diagnosed
+    effective radii, low / middle / high cloud cover;
+  * solar geometry: the sun's position for the date (sol_update) and each column's cos zenith averaged
+    over the sunlit instants of the SW interval (coszmn), on the library (woof_nam_amd.solar);
+  * the radiation call on the path-form columns (woof_nam_amd.radiation, the NAM route mcica = 0;
+    FP64 unless ctx.cfg.rrtm_precision = "mixed"), in tiles sized from free device memory;
+  * wr_nmmb_post (phys_rrtm_driver_post.cu:71-150): RLWTT, RSWTT (K/s), RLWIN, RSWIN, RSWOUT,
+    RSWINC, RLWTOA, RSWTOA, CZMEAN, SIGT4 = sigma T(LM)^4 on LW steps, and on every call CLDFRA,
+    CFRACL/M/H and the ACFRST/NCFRST (ACFRCV/NCFRCV) accumulators.
 """
 from __future__ import annotations
+
+import ctypes as C
 
 import numpy as np
 import torch
@@ -31,12 +34,14 @@ from . import _lib, abi, solar
 from .radiation import Radiation
 
 KEY_CTX = "woof_nam_amd.radiation"
-STBOLT = 5.67051e-8
-SW_OUT = ("RSWTT", "RSWIN", "RSWOUT", "RSWINC", "RSWTOA", "CZMEAN")
-LW_OUT = ("RLWTT", "RLWIN", "RLWTOA", "SIGT4")
+KEY_LLYR = "woof_nam_amd.llyr"
+STATE_3D = ("T", "Q", "O3", "CW", "F_ICE", "F_RAIN")
+OUT_3D = ("RLWTT", "RSWTT", "CLDFRA")
+OUT_2D = ("RLWIN", "RLWTOA", "SIGT4", "RSWIN", "RSWOUT", "RSWINC", "RSWTOA", "CZMEAN", "CFRACL", "CFRACM",
+          "CFRACH", "ACFRST", "ACFRCV")
 
 
-def _arr(args, name, shape=None):
+def _arr(args, name, shape=None, dtype=None):
     if name not in args:
         raise KeyError(f"rrtm-call: argument {name!r} missing (its producer has not run)")
     a = args[name]
@@ -44,6 +49,8 @@ def _arr(args, name, shape=None):
         a = torch.as_tensor(np.asarray(a))
     if shape is not None and tuple(a.shape) != tuple(shape):
         raise ValueError(f"rrtm-call: {name} has shape {tuple(a.shape)}, expected {tuple(shape)}")
+    if dtype is not None and a.dtype != dtype:
+        raise ValueError(f"rrtm-call: {name} has dtype {a.dtype}, expected {dtype}")
     return a
 
 
@@ -93,49 +100,113 @@ def _context(ctx, device, ncol, lm):
     return rad
 
 
-def columns(args, device):
-    """The column-API inputs of the interior rows, float64 [k][ncol], plus (rows slice, im, jm, lm)."""
+def boundary_layer_top(si) -> int:
+    """llyr of cld_init (radiation_clouds_nmmb.f:327-344, ivflip = 0; phys_rrtm_cld_init.cu): with the
+    sigma interfaces si (LM + 1 values, top first), the last k of k = LM, LM-1, ..., 2 before the first
+    si(k) < 0.9 (1-based); 2 when none is."""
+    si = np.asarray(si, dtype=np.float32)
+    nlay = si.size - 1
+    kl = 1
+    for k in range(nlay, 1, -1):
+        kl = k
+        if si[k - 1] < np.float32(0.9):
+            break
+    return kl
+
+
+def sigma_interfaces(args, lm):
+    """Interface sigma of the hybrid coordinate, (p - PT) / (p_sfc - PT), at the standard surface
+    pressure 101325 Pa (the NMMB's SGM when the grid does not carry it)."""
+    pt = float(args["PT"])
+    pdsg1 = _arr(args, "PDSG1", (lm,)).double().cpu().numpy()
+    dsg2 = _arr(args, "DSG2", (lm,)).double().cpu().numpy()
+    pdtop = float(np.sum(pdsg1))
+    pd = 101325.0 - pt - pdtop
+    p = pt + np.concatenate([[0.0], np.cumsum(pdsg1 + dsg2 * pd)])
+    return (p - pt) / (p[-1] - pt)
+
+
+def _llyr(ctx, args, lm):
+    v = _cfg(ctx, "rrtm_llyr", None)
+    if v is not None:
+        return int(v)
+    if KEY_LLYR not in ctx.tables:
+        sgm = getattr(getattr(ctx, "grid", None), "sgm", None)
+        ctx.tables[KEY_LLYR] = boundary_layer_top(sgm if sgm is not None else sigma_interfaces(args, lm))
+    return ctx.tables[KEY_LLYR]
+
+
+def _fptr(t):
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_float)) if t is not None else None
+
+
+def _dptr(t):
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_double)) if t is not None else None
+
+
+def _iptr(t):
+    return C.cast(C.c_void_p(t.data_ptr()), C.POINTER(C.c_int32)) if t is not None else None
+
+
+def nmmb_state(args, device, rows):
+    """WrNmmbState over the state arrays of ``args`` (float32, contiguous, on the device) and the arrays
+    it points to (kept alive by the caller)."""
     T = _arr(args, "T")
     if T.dim() != 3:
         raise ValueError(f"rrtm-call: T must be [LM, JM, IM], got {tuple(T.shape)}")
     lm, jm, im = (int(x) for x in T.shape)
-    j0, j1 = int(args.get("J_S", 2)) - 1, int(args.get("J_E", jm - 1))
-    if not (0 <= j0 < j1 <= jm):
-        raise ValueError(f"rrtm-call: rows J_S..J_E = {j0 + 1}..{j1} outside 1..{jm}")
-    c0, c1 = j0 * im, j1 * im
-    f64 = dict(dtype=torch.float64, device=device)
+    j0, j1 = rows
+    keep = {}
+    for k in STATE_3D:
+        keep[k] = _arr(args, k, (lm, jm, im), torch.float32)
+    for k in ("PD", "GLAT"):
+        keep[k] = _arr(args, k, (jm, im), torch.float32)
+    for k in ("PDSG1", "DSG2", "PSGML1", "SGML2"):
+        keep[k] = _arr(args, k, (lm,)).to(device=device, dtype=torch.float32)
+    for k, v in keep.items():
+        if v.device != device or not v.is_contiguous():
+            raise ValueError(f"rrtm-call: {k} must be a contiguous array on {device}")
+    st = abi.WrNmmbState(im, jm, lm, j0, j1, float(args["PT"]),
+                         *[_fptr(keep[k]) for k in ("PD", "PDSG1", "DSG2", "PSGML1", "SGML2", "T", "Q", "O3", "CW",
+                                                    "F_ICE", "F_RAIN", "GLAT")])
+    return st, keep, (im, jm, lm)
 
-    def lay(name):
-        a = _arr(args, name, (lm, jm, im))
-        return a.reshape(lm, jm * im)[:, c0:c1].to(**f64).contiguous()
 
-    def sfc(name):
-        a = _arr(args, name, (jm, im))
-        return a.reshape(jm * im)[c0:c1].to(**f64).contiguous()
+def prep(args, device, rows, llyr, iovr=1, subgrid=True, stream=None):
+    """wr_nmmb_prep: the path-form columns of the radiated rows (float64 device tensors) with gethml's
+    clds [5][ncol], mtop / mbot [3][ncol]; P_PHY is written when args holds it."""
+    st, keep, (im, jm, lm) = nmmb_state(args, device, rows)
+    ncol = (rows[1] - rows[0]) * im
+    cols = {k: torch.empty(s, dtype=torch.float64, device=device) for k, s in abi.nmmb_col_shapes(ncol, lm).items()}
+    cols["clds"] = torch.empty((5, ncol), dtype=torch.float32, device=device)
+    cols["mtop"] = torch.empty((3, ncol), dtype=torch.int32, device=device)
+    cols["mbot"] = torch.empty((3, ncol), dtype=torch.int32, device=device)
+    pphy = args.get("P_PHY")
+    if pphy is not None:
+        pphy = _arr(args, "P_PHY", (lm, jm, im), torch.float32)
+    cst = abi.WrNmmbCols(*[_dptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _fptr(cols["clds"]), _iptr(cols["mtop"]),
+                         _iptr(cols["mbot"]), _fptr(pphy))
+    s = stream or torch.cuda.current_stream(device)
+    rc = _lib.load().wr_nmmb_prep(C.byref(st), C.byref(abi.WrNmmbOpts(int(llyr), int(iovr), int(bool(subgrid)))),
+                                  C.byref(cst), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "wr_nmmb_prep")
+    cols["_keep"] = keep
+    return cols, (im, jm, lm)
 
-    pd = sfc("PD")
-    pdsg1 = _arr(args, "PDSG1", (lm,)).to(**f64)
-    dsg2 = _arr(args, "DSG2", (lm,)).to(**f64)
-    p_lev = torch.empty((lm + 1, c1 - c0), **f64)
-    p_lev[0] = float(args["PT"])
-    p_lev[1:] = float(args["PT"]) + torch.cumsum(pdsg1[:, None] + dsg2[:, None] * pd[None, :], dim=0)
-    t = lay("T")
-    q = lay("Q")
-    o3 = lay("O3")
-    qc = lay("QCW") if args.get("F_QC", True) else torch.zeros_like(t)
-    qi = lay("QCI") if args.get("F_QI", False) else torch.zeros_like(t)
-    if int(args.get("CLD_FRACTION", 0)) == 0:
-        pl = 0.5 * (p_lev[1:] + p_lev[:-1]) * 1.0e-2                 # layer pressure, hPa
-        clw = qc + qi
-        cldf = torch.where(clw > 1.0e-8 * (pl * 1.0e-3), torch.clamp(1.0e5 * clw * pl * 1.0e-3, max=1.0),
-                           torch.zeros_like(clw))
-    else:
-        cldf = lay("CLDFRA")
-    cols = {"p_lev": p_lev.contiguous(), "t_lay": t, "q_lay": q, "o3_lay": o3, "cldf": cldf.contiguous(),
-            "qc": qc, "qi": qi, "t_sfc": sfc("TSKIN"), "alb": sfc("ALBEDO")}
-    cols["emis"] = sfc("EMISS") if "EMISS" in args else torch.ones_like(cols["t_sfc"])
-    cols["col_id"] = torch.arange(c0, c1, dtype=torch.int64, device=device)
-    return cols, (c0, c1), (im, jm, lm)
+
+def post(args, device, rows, cols, out, czmean, lw_step, sw_step, stream=None):
+    """wr_nmmb_post: the radiation outputs and cloud diagnostics into the state arrays of ``args``."""
+    st, keep, (im, jm, lm) = nmmb_state(args, device, rows)
+    o = {k: _arr(args, k, (lm, jm, im), torch.float32) for k in OUT_3D}
+    o.update({k: _arr(args, k, (jm, im), torch.float32) for k in OUT_2D})
+    o.update({k: _arr(args, k, (jm, im), torch.int32) for k in ("NCFRST", "NCFRCV")})
+    cst = abi.WrNmmbCols(*[_dptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _fptr(cols["clds"]), None, None, None)
+    ro = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
+    oo = abi.WrNmmbOut(*[_fptr(o[k.upper()]) for k in abi.NMMB_OUT_FIELDS], _iptr(o["NCFRST"]), _iptr(o["NCFRCV"]))
+    s = stream or torch.cuda.current_stream(device)
+    rc = _lib.load().wr_nmmb_post(C.byref(st), C.byref(cst), C.byref(ro), _dptr(czmean), int(bool(lw_step)),
+                                  int(bool(sw_step)), C.byref(oo), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "wr_nmmb_post")
 
 
 def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
@@ -157,12 +228,26 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     if stamp is not None:
         stamp("driver-prep")
     T = _arr(args, "T")
+    if T.dim() != 3:
+        raise ValueError(f"rrtm-call: T must be [LM, JM, IM], got {tuple(T.shape)}")
+    lm, jm, im = (int(x) for x in T.shape)
     device = T.device if T.is_cuda else torch.device("cuda", 0)
-    cols, (c0, c1), (im, jm, lm) = columns(args, device)
-    glat = _arr(args, "GLAT", (jm, im)).reshape(-1)[c0:c1].to(device)
+    j0, j1 = int(args.get("J_S", 2)) - 1, int(args.get("J_E", jm - 1))
+    if not (0 <= j0 < j1 <= jm):
+        raise ValueError(f"rrtm-call: rows J_S..J_E = {j0 + 1}..{j1} outside 1..{jm}")
+    cols, _ = prep(args, device, (j0, j1), _llyr(ctx, args, lm), iovr=1, subgrid=bool(_cfg(ctx, "rrtm_subgrid", True)))
+    c0, c1 = j0 * im, j1 * im
+    glat = _arr(args, "GLAT", (jm, im)).reshape(-1)[c0:c1]
     glon = _arr(args, "GLON", (jm, im)).reshape(-1)[c0:c1].to(device)
-    cz = solar.coszen(args["JDAT"], nts * dt, nrads * dt, glat, glon)
-    cols["cosz"] = cz if (sw and lsswr) else torch.full_like(cz, -1.0)
+    cz = solar.coszen(args["JDAT"], nts * dt, nrads, dt, glat.to(device), glon)[0].contiguous()
+    sw_step = bool(sw and lsswr)
+    f64 = dict(dtype=torch.float64, device=device)
+    cols["cosz"] = cz if sw_step else torch.full_like(cz, -1.0)
+    cols["t_sfc"] = _arr(args, "TSKIN", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous()
+    cols["alb"] = _arr(args, "ALBEDO", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous()
+    cols["emis"] = (_arr(args, "EMISS", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous() if "EMISS" in args
+                    else torch.ones(c1 - c0, **f64))
+    cols["col_id"] = torch.arange(c0, c1, dtype=torch.int64, device=device)
     params = Radiation.default_params(mcica=mcica, solcon=float(args["SOLCON"]), seed=int(_cfg(ctx, "rrtm_seed", 0)))
     rad = _context(ctx, device, c1 - c0, lm)
     if stamp is not None:
@@ -170,25 +255,4 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     out, _ = rad.run(cols, params=params, precision=str(_cfg(ctx, "rrtm_precision", "fp64")))
     if stamp is not None:
         stamp("driver-post")
-
-    def put3(name, val):
-        a = _arr(args, name, (lm, jm, im))
-        a.view(lm, jm * im)[:, c0:c1] = val.to(a.dtype)
-
-    def put2(name, val):
-        a = _arr(args, name, (jm, im))
-        a.view(jm * im)[c0:c1] = val.to(a.dtype)
-
-    if lslwr:
-        put3("RLWTT", out["hr_lw"])
-        put2("RLWIN", out["flw_dn"][-1])
-        put2("RLWTOA", out["flw_up"][0])
-        tl = cols["t_lay"][-1]
-        put2("SIGT4", STBOLT * tl * tl * tl * tl)
-    if sw and lsswr:
-        put3("RSWTT", out["hr_sw"])
-        put2("RSWIN", out["fsw_dn"][-1])
-        put2("RSWOUT", out["fsw_up"][-1])
-        put2("RSWINC", out["fsw_dn_sfc_clr"])
-        put2("RSWTOA", out["fsw_up"][0])
-        put2("CZMEAN", cz)
+    post(args, device, (j0, j1), cols, out, cz.contiguous(), lslwr, sw_step)
