@@ -71,6 +71,16 @@ typedef struct {
     const double *crp;       /* [nlay][ncol] rain path, g/m2 (NULL: none)                          */
     const double *rew;       /* [nlay][ncol] liquid effective radius, micron                      */
     const double *rei;       /* [nlay][ncol] ice effective radius, micron                         */
+    /* Optional aerosol optical properties per radiation layer, in the layout setaer writes
+     * (phys_rrtm_setaer.cu:20-27, float32): [field][band][layer][aer_ld] with field 0 the optical
+     * depth, 1 the single-scattering albedo, 2 the asymmetry factor; layer 0 the layer added above
+     * the model top, then the model layers top-first (nlay + 1 layers); column i of the call at
+     * entry i of the last index.  aer_sw holds SW bands 16..29 (14), aer_lw LW bands 1..16 (16; the
+     * LW adds tau (1 - ssa) to the gas optical depth, lwrad).  aer_ld is the column stride, >= ncol
+     * (0: ncol), so the arrays may be a slice of a wider grid.  NULL: no aerosol. */
+    const float *aer_sw;
+    const float *aer_lw;
+    int64_t aer_ld;
 } wr_columns;
 
 /* Call parameters (NAM values by default; see wr_default_params). */

@@ -76,6 +76,8 @@ typedef struct {
     int nrl;
     double plvl[O_NL + 1], plyr[O_NL], tlyr[O_NL], tlvl[O_NL + 1], qlyr[O_NL], olyr[O_NL];
     double cld[O_NL][9]; /* clouds(:, n): frac, cwp, rew, cip, rei, crp, rer, snow path, snow radius */
+    double aerlw[2][16][O_NL]; /* aerosol optical depth, single-scattering albedo per LW band (0: none) */
+    double aersw[3][14][O_NL]; /* aerosol optical depth, ssa, asymmetry per SW band (0: none)           */
     double tskn;
     double vmr[10];
 } Rad;
@@ -93,6 +95,7 @@ typedef struct {
     double rfrate[O_NL][6][2];
     double pklay[O_NL + 1][16], pklev[O_NL + 1][16];
     double taucld[16][O_NL];
+    double tauaer[16][O_NL]; /* aerosol absorption optical depth tau (1 - ssa) */
 } LwCol;
 
 /* SW day column, SURFACE-FIRST */
@@ -107,6 +110,7 @@ typedef struct {
     double forfac[O_NL], forfrac[O_NL];
     double sfluxzen[112];
     double taucw[14][O_NL], ssacw[14][O_NL], asycw[14][O_NL];
+    double tauae[14][O_NL], ssaae[14][O_NL], asyae[14][O_NL]; /* aerosol optics */
     double cldfrc[O_NL], cldf[O_NL];
 } SwCol;
 
@@ -138,7 +142,7 @@ double osw_expm(const OT *t, double z, int clamped);
 void osw_layer_opt(const OT *t, double tau0, double ssa0, double asy0, double cz, double *rb, double *rd, double *tb,
                    double *td, double *tau1);
 void osw_gpoint(const double *tbl, int nlay, double cz, double snt, double albb, double albd, const double *tr,
-                const double *tg, int cloudy, const double *zc, const double *tc, const double *sc, const double *ac,
-                double *f0u, double *f0d, double *fcu, double *fcd);
+                const double *tg, const double *ta, const double *sa, const double *ya, int cloudy, const double *zc,
+                const double *tc, const double *sc, const double *ac, double *f0u, double *f0d, double *fcu, double *fcd);
 const OT *oracle_tables(void);
 #endif

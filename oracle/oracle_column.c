@@ -269,6 +269,16 @@ static void o_prep(const wr_columns *in, const wr_params *p, int i, Rad *r)
         r->cld[l][7] = 0.0;
         r->cld[l][8] = 0.0;
     }
+    /* aerosol optics, setaer's layout [field][band][radiation layer, top first][column] (float32) */
+    const size_t ld = in->aer_ld > 0 ? (size_t)in->aer_ld : (size_t)nc;
+    for (int l = 0; l < nrl; ++l) {
+        for (int b = 0; b < 16; ++b)
+            for (int f = 0; f < 2; ++f)
+                r->aerlw[f][b][l] = in->aer_lw ? (double)in->aer_lw[((size_t)(f * 16 + b) * nrl + l) * ld + i] : 0.0;
+        for (int b = 0; b < 14; ++b)
+            for (int f = 0; f < 3; ++f)
+                r->aersw[f][b][l] = in->aer_sw ? (double)in->aer_sw[((size_t)(f * 14 + b) * nrl + l) * ld + i] : 0.0;
+    }
 #undef F
 }
 

@@ -54,6 +54,8 @@ void olw_prep(const Rad *r, double sfemis, LwCol *c)
         c->pavel[k] = r->plyr[kin];
         c->tavel[k] = r->tlyr[kin];
         c->delp[k] = dlp;
+        /* aerosol: tauaer = tau (1 - ssa), band by band (phys_rrtm_lw_lwrad.cu:214-219) */
+        for (int b = 0; b < 16; ++b) c->tauaer[b][k] = r->aerlw[0][b][kin] * (1.0 - r->aerlw[1][b][kin]);
         c->tz[k + 1] = r->tlvl[nl - k - 1];
         const double h2ovmr = fmax(0.0, qv * O_AMDW / (1.0 - qv));
         const double o3vmr = fmax(0.0, ov * O_AMDO3);
@@ -741,7 +743,7 @@ static void lw_band(const OT *t, const LwCol *c, int b, int k, double *tau, doub
 #undef ABS4
 }
 
-/* all bands of one column: taug/fracs [140][nlay], tautot = taug + tauaer (tauaer = 0 here) */
+/* all bands of one column: taug/fracs [140][nlay], tautot = taug + tauaer (radlw taumol) */
 void olw_taumol(const OT *t, const LwCol *c, double (*taug)[O_NL], double (*fracs)[O_NL])
 {
     double tau[16], fr[16];
@@ -749,7 +751,7 @@ void olw_taumol(const OT *t, const LwCol *c, double (*taug)[O_NL], double (*frac
         for (int k = 0; k < c->nlay; ++k) {
             lw_band(t, c, b, k, tau, fr);
             for (int g = 0; g < t->lw[b].ng; ++g) {
-                taug[t->lw[b].ns + g][k] = tau[g] + 0.0;
+                taug[t->lw[b].ns + g][k] = tau[g] + c->tauaer[b][k];
                 fracs[t->lw[b].ns + g][k] = fr[g];
             }
         }

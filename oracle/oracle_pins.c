@@ -172,10 +172,12 @@ int oracle_sw_layer_opt(double tau0, double ssa0, double asy0, double cz, double
     return WR_OK;
 }
 
-/* One g-point of spcvrtc on a column of nlay layers (surface-first per-layer inputs), per unit
- * incident flux: f[0..3][nlay+1] = clear up, clear down, total up, total down, levels surface first. */
-int oracle_sw_gpoint(int nlay, double cz, double albb, double albd, const double *tr, const double *tg, int cloudy,
-                     const double *zc, const double *tc, const double *sc, const double *ac, double *f)
+/* One g-point of spcvrtc on a column of nlay layers (surface-first per-layer inputs; ta / sa / ya the
+ * aerosol optical depth, single-scattering albedo and asymmetry, NULL: none), per unit incident flux:
+ * f[0..3][nlay+1] = clear up, clear down, total up, total down, levels surface first. */
+int oracle_sw_gpoint(int nlay, double cz, double albb, double albd, const double *tr, const double *tg,
+                     const double *ta, const double *sa, const double *ya, int cloudy, const double *zc,
+                     const double *tc, const double *sc, const double *ac, double *f)
 {
     const OT *t = oracle_tables();
     if (!t) return WR_ERR_TABLES;
@@ -183,7 +185,8 @@ int oracle_sw_gpoint(int nlay, double cz, double albb, double albd, const double
     double f0u[O_NL + 3], f0d[O_NL + 3], fcu[O_NL + 3], fcd[O_NL + 3];
     memset(fcu, 0, sizeof fcu);
     memset(fcd, 0, sizeof fcd);
-    osw_gpoint(t->sw_exp_tbl, nlay, cz, 1.0 / cz, albb, albd, tr, tg, cloudy, zc, tc, sc, ac, f0u, f0d, fcu, fcd);
+    osw_gpoint(t->sw_exp_tbl, nlay, cz, 1.0 / cz, albb, albd, tr, tg, ta, sa, ya, cloudy, zc, tc, sc, ac, f0u, f0d, fcu,
+               fcd);
     const int n = nlay + 1;
     for (int k = 0; k < n; ++k) {
         f[0 * n + k] = f0u[k + 1];

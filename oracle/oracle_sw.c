@@ -35,6 +35,11 @@ void osw_prep(const Rad *r, double alb, double cosz, double solcon, SwCol *c)
         const int kk0 = nl - 1 - k;
         c->pavel[k] = r->plyr[kk0];
         c->tavel[k] = r->tlyr[kk0];
+        for (int b = 0; b < 14; ++b) { /* aerosol optics (phys_rrtm_sw_swrad.cu:224-229) */
+            c->tauae[b][k] = r->aersw[0][b][kk0];
+            c->ssaae[b][k] = r->aersw[1][b][kk0];
+            c->asyae[b][k] = r->aersw[2][b][kk0];
+        }
         const double dl = r->plvl[kk0 + 1] - r->plvl[kk0];
         c->delp[k] = dl;
         const double qv = r->qlyr[kk0], ov = r->olyr[kk0];
@@ -673,8 +678,8 @@ static const int SW_IDXSFC[14] = {1, 1, 1, 1, 1, 1, 1, 1, 0, 2, 2, 2, 2, 1}; /* 
  * the clear-sky level fluxes f0u/f0d and, when cloudy != 0, the total-sky ones fcu/fcd, per unit
  * incident flux, levels 1..nlay+1 (index 0 unused). */
 void osw_gpoint(const double *tbl, int nlay, double cz, double snt, double albb, double albd, const double *tr,
-                const double *tg, int cloudy, const double *zc, const double *tc, const double *sc, const double *ac,
-                double *f0u, double *f0d, double *fcu, double *fcd)
+                const double *tg, const double *ta, const double *sa, const double *ya, int cloudy, const double *zc,
+                const double *tc, const double *sc, const double *ac, double *f0u, double *f0d, double *fcu, double *fcd)
 {
     const int nlp1 = nlay + 1;
     double zrefb[O_NL + 3], zrefd[O_NL + 3], ztrab[O_NL + 3], ztrad[O_NL + 3], ztdbt[O_NL + 3], zldbt[O_NL + 3];
@@ -688,9 +693,11 @@ void osw_gpoint(const double *tbl, int nlay, double cz, double snt, double albb,
     ztrad[1] = 0.0;
     for (int k = nlay; k >= 1; --k) {
         const int kp = k + 1;
-        const double ztau0 = fmax(O_FTINY, tr[k - 1] + tg[k - 1] + 0.0);
-        const double zssa0 = tr[k - 1] + 0.0;
-        const double zasy0 = 0.0;
+        /* clear-sky optical parameters with the aerosol (phys_rrtm_sw_spcvrtc.cu:323-329) */
+        const double a_t = ta ? ta[k - 1] : 0.0, a_s = sa ? sa[k - 1] : 0.0, a_y = ya ? ya[k - 1] : 0.0;
+        const double ztau0 = fmax(O_FTINY, tr[k - 1] + tg[k - 1] + a_t);
+        const double zssa0 = tr[k - 1] + a_t * a_s;
+        const double zasy0 = a_y * a_s * a_t;
         ztaus[k] = ztau0;
         zssas[k] = zssa0;
         zasys[k] = zasy0;
@@ -763,7 +770,8 @@ void osw_spcvrtc(const OT *t, const SwCol *c, double (*taug)[O_NL], double (*tau
             const double zsolar = c->ssolar * c->sfluxzen[jg];
             for (int k = 0; k < nlay; ++k)
                 zc[k] = mcica ? (((mask[k * 4 + (jg >> 5)] >> (jg & 31)) & 1u) ? 1.0 : 0.0) : c->cldfrc[k];
-            osw_gpoint(tbl, nlay, cz, snt, albb, albd, taur[jg], taug[jg], cloudy, zc, c->taucw[bi], c->ssacw[bi],
+            osw_gpoint(tbl, nlay, cz, snt, albb, albd, taur[jg], taug[jg], c->tauae[bi], c->ssaae[bi], c->asyae[bi], cloudy,
+                       zc, c->taucw[bi], c->ssacw[bi],
                        c->asycw[bi], f0u, f0d, fcu, fcd);
             for (int k = 1; k <= nlp1; ++k) {
                 fxup0[bi][k - 1] = fxup0[bi][k - 1] + zsolar * f0u[k];

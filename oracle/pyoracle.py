@@ -51,7 +51,8 @@ def _bind(path):
     L.oracle_sw_expm.argtypes = [C.c_double, C.c_int]
     L.oracle_sw_expm.restype = C.c_double
     L.oracle_sw_layer_opt.argtypes = [C.c_double] * 4 + [dp]
-    L.oracle_sw_gpoint.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, C.c_int, dp, dp, dp, dp, dp]
+    L.oracle_sw_gpoint.argtypes = [C.c_int, C.c_double, C.c_double, C.c_double, dp, dp, dp, dp, dp, C.c_int, dp, dp,
+                                   dp, dp, dp]
     rc = L.oracle_init(str(TABLES).encode())
     if rc != 0:
         raise RuntimeError(f"oracle: table load failed ({rc})")
@@ -176,9 +177,11 @@ def rdtemp(t, czen, glat, glon, czmean, rswtt, rlwtt, ntsd, dt, start, box):
         raise ValueError("oracle_rdtemp: bad arguments")
 
 
-def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = None, diag: bool = False):
-    """Run the oracle on host float64 columns (gen.py layout).  Returns (outputs, diag|None)."""
-    L = lib()
+def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = None, diag: bool = False, L=None):
+    """Run the oracle on host float64 columns (gen.py layout; optional aerosol optics aer_sw / aer_lw
+    [3][bands][nlay+1][ncol] float32).  L: a separately loaded oracle (default: this one).  Returns
+    (outputs, diag|None)."""
+    L = L or lib()
     params = params or abi.make_params()
     t = np.ascontiguousarray(cols["t_lay"], dtype=np.float64)
     nlay, ncol = t.shape
@@ -187,9 +190,14 @@ def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = N
             if cols.get(k) is not None}
     ids = cols.get("col_id")
     ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint64)
+    aer = {k: np.ascontiguousarray(cols[k], dtype=np.float32) for k in abi.AER_FIELDS if cols.get(k) is not None}
+    for k, a in aer.items():
+        if a.shape != abi.aer_shape(k, ncol, nlay):
+            raise ValueError(f"{k}: shape {a.shape}, expected {abi.aer_shape(k, ncol, nlay)}")
     ci = abi.WrColumns(ncol, nlay, *[_ptr(arrs.get(k)) for k in abi.INPUT_FIELDS],
                        _ptr(ids, C.c_uint64) if ids is not None else None, None, None,
-                       *[_ptr(arrs.get(k)) if path else None for k in abi.PATH_FIELDS])
+                       *[_ptr(arrs.get(k)) if path else None for k in abi.PATH_FIELDS],
+                       *[_ptr(aer[k], C.c_float) if k in aer else None for k in abi.AER_FIELDS], 0)
     outs = {k: np.zeros(s) for k, s in abi.output_shapes(ncol, nlay).items()}
     co = abi.WrOutputs(*[_ptr(outs[k]) for k in abi.OUTPUT_FIELDS])
     dg = None

@@ -24,7 +24,7 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layouts_match_header():
     # x86-64 SysV layout of the header structs
-    assert C.sizeof(abi.WrColumns) == 8 + 19 * 8
+    assert C.sizeof(abi.WrColumns) == 8 + 22 * 8
     assert C.sizeof(abi.WrNmmbState) == 6 * 4 + 12 * 8
     assert C.sizeof(abi.WrNmmbOpts) == 3 * 4
     assert C.sizeof(abi.WrNmmbCols) == 14 * 8

@@ -105,6 +105,58 @@ def test_glane_lw_kernels(gpu_device, monkeypatch):
         plain.close()
 
 
+def aerosol(ncol, nlay, seed, ld=None):
+    """Aerosol optics [3][bands][nlay+1][ld] float32: optical depth, single-scattering albedo, asymmetry."""
+    rng = np.random.default_rng(seed)
+    ld = ld or ncol
+    out = {}
+    for k in abi.AER_FIELDS:
+        a = np.zeros((3, abi.AER_BANDS[k], nlay + 1, ld), np.float32)
+        a[0] = rng.uniform(0.0, 0.5, a[0].shape) * (rng.uniform(size=a[0].shape) < 0.7)
+        a[1] = rng.uniform(0.5, 1.0, a[1].shape)
+        a[2] = rng.uniform(0.5, 0.8, a[2].shape)
+        out[k] = a
+    return out
+
+
+@pytest.mark.parametrize("mcica", [1, 0])
+def test_aerosol_matches_oracle(gpu_device, mcica):
+    """Aerosol optics on both routes: FP64 bit-identical to the oracle, also when the aerosol arrays
+    are a column slice of a wider grid array (setaer's whole-grid layout); mixed within the bar."""
+    print(f"case aerosol mcica={mcica}", flush=True)
+    ncol, c0, ld = 700, 150, 1100
+    cols = gen.columns(ncol, seed=40 + mcica)
+    nlay = cols["t_lay"].shape[0]
+    wide = aerosol(ncol, nlay, seed=50 + mcica, ld=ld)
+    host = {k: np.ascontiguousarray(v[..., c0:c0 + ncol]) for k, v in wide.items()}
+    p = Radiation.default_params(mcica=mcica, seed=3)
+    ref, _ = pyoracle.run({**cols, **host}, params=p)
+    plain, _ = pyoracle.run(cols, params=p)
+    assert np.abs(ref["flw_dn"] - plain["flw_dn"]).max() > 1.0 and np.abs(ref["fsw_dn"] - plain["fsw_dn"]).max() > 1.0
+    r = Radiation(0, max_cols=1024)
+    try:
+        dcols = to_dev(cols, r.device)
+        dwide = {k: torch.from_numpy(v).to(r.device) for k, v in wide.items()}
+        for aer in ({k: torch.from_numpy(v).to(r.device) for k, v in host.items()},
+                    {k: v[..., c0:c0 + ncol] for k, v in dwide.items()}):
+            out, _ = r.run({**dcols, **aer}, params=p)
+            torch.cuda.synchronize()
+            for k in abi.OUTPUT_FIELDS:
+                a = out[k].cpu().numpy()
+                assert np.array_equal(a.view(np.uint64), ref[k].view(np.uint64)), \
+                    f"{k}: {np.count_nonzero(a != ref[k])} elements differ, max {np.abs(a - ref[k]).max():.3e}"
+        mx, _ = r.run({**dcols, **aer}, params=p, precision="mixed")
+        torch.cuda.synchronize()
+        for k in ("hr_lw", "hr_sw"):
+            d = (mx[k].cpu().numpy() - ref[k]) * 86400.0
+            assert np.sqrt(np.mean(d * d)) <= 0.05, k
+        for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
+            d = mx[k].cpu().numpy() - ref[k]
+            assert np.sqrt(np.mean(d * d)) <= 0.5, k
+    finally:
+        r.close()
+
+
 def test_day_list_over_several_segments(gpu_device):
     """One 7,000-column tile: the day list is compacted from four 2,048-column segments, in order."""
     cols = gen.columns(7000, seed=13)

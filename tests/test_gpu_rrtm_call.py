@@ -121,11 +121,29 @@ def test_post_matches_oracle(gpu_device):
     assert out_h["ncfrst"][3, 3] == 8 and (out_h["cldfra"][:, 5, 7] == 0).all()
 
 
-def test_hook_matches_oracle_chain(gpu_device):
-    """rad_driver.run -> rrtm_call.run against the oracle chain on the same inputs."""
-    print("case rrtm_call hook vs oracle chain", flush=True)
+def setaer_grid(seed):
+    """Aerosol optics over the whole grid in setaer's layout [3][bands][LM+1][JM][IM] float32."""
+    rng = np.random.default_rng(seed)
+    out = {}
+    for key, nb in (("rrtm_faersw", 14), ("rrtm_faerlw", 16)):
+        a = np.zeros((3, nb, LM + 1, JM, IM), np.float32)
+        a[0] = rng.uniform(0.0, 0.3, a[0].shape)
+        a[1] = rng.uniform(0.6, 1.0, a[1].shape)
+        a[2] = rng.uniform(0.5, 0.8, a[2].shape)
+        out[key] = a
+    return out
+
+
+@pytest.mark.parametrize("with_aerosol", [False, True])
+def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
+    """rad_driver.run -> rrtm_call.run against the oracle chain on the same inputs (with the model's
+    aerosol optics in ctx.tables, or without)."""
+    print(f"case rrtm_call hook vs oracle chain aerosol={with_aerosol}", flush=True)
     h = ferrier_state(3)
     ctx = make_ctx(gpu_device, h)
+    aer = setaer_grid(9) if with_aerosol else {}
+    for key, a in aer.items():
+        ctx.tables[key] = torch.from_numpy(a).to(gpu_device)
     rad_driver.run(ctx, rrtm_call=rrtm_call.run)
     torch.cuda.synchronize()
     st = ctx.st
@@ -142,6 +160,10 @@ def test_hook_matches_oracle_chain(gpu_device):
     rc = {k: cols[k] for k in abi.NMMB_COL_FIELDS}
     rc.update({"cosz": cz, "t_sfc": flat(h["TSKIN"]), "alb": flat(h["ALBEDO"]), "emis": np.ones(c1 - c0),
                "col_id": np.arange(c0, c1, dtype=np.uint64)})
+    for k, key in rrtm_call.AER_KEYS.items():
+        if key in aer:
+            a = aer[key]
+            rc[k] = np.ascontiguousarray(a.reshape(3, a.shape[1], LM + 1, JM * IM)[..., c0:c1])
     from woof_nam_amd.radiation import Radiation
     solcon = ctx.tables["rrtm_call_flags"]["solcon"]
     rad, _ = pyoracle.run(rc, params=Radiation.default_params(mcica=0, solcon=solcon))

@@ -39,6 +39,12 @@ MUTATIONS = [
     ("oracle_lw.c", "0.943", "0.934", "band-4 upper g-point factor"),
     ("oracle_sw.c", "id[2] = imin(nrow, id[0] + d1);", "id[2] = imin(nrow, id[0] + d1 - 1);",
      "SW eta-table temperature stride"),
+    ("oracle_lw.c", "r->aerlw[0][b][kin] * (1.0 - r->aerlw[1][b][kin])", "r->aerlw[0][b][kin] * r->aerlw[1][b][kin]",
+     "LW aerosol absorption tau (1 - ssa)"),
+    ("oracle_sw.c", "const double zasy0 = a_y * a_s * a_t;", "const double zasy0 = a_y * a_t;",
+     "SW aerosol asymmetry weight"),
+    ("oracle_sw.c", "const double zssa0 = tr[k - 1] + a_t * a_s;", "const double zssa0 = tr[k - 1] + a_t;",
+     "SW aerosol scattering depth"),
 ]
 
 

@@ -22,7 +22,18 @@ class WrColumns(C.Structure):
                 ("qc", _dp), ("qi", _dp), ("t_sfc", _dp), ("emis", _dp), ("alb", _dp), ("cosz", _dp),
                 ("col_id", C.POINTER(C.c_uint64)), ("seed_dev", C.POINTER(C.c_uint64)),
                 ("ncol_dev", C.POINTER(C.c_int32)),
-                ("cwp", _dp), ("cip", _dp), ("crp", _dp), ("rew", _dp), ("rei", _dp)]
+                ("cwp", _dp), ("cip", _dp), ("crp", _dp), ("rew", _dp), ("rei", _dp),
+                ("aer_sw", C.POINTER(C.c_float)), ("aer_lw", C.POINTER(C.c_float)), ("aer_ld", C.c_int64)]
+
+
+# aerosol optics inputs: [3][bands][nlay+1][ncol] float32 (optical depth, single-scattering albedo,
+# asymmetry per radiation layer, the added top layer first)
+AER_FIELDS = ("aer_sw", "aer_lw")
+AER_BANDS = {"aer_sw": NBANDS_SW, "aer_lw": NBANDS_LW}
+
+
+def aer_shape(field: str, ncol: int, nlay: int) -> tuple:
+    return (3, AER_BANDS[field], nlay + 1, ncol)
 
 
 class WrParams(C.Structure):
@@ -89,6 +100,8 @@ def nmmb_col_shapes(ncol: int, lm: int) -> dict:
     for k in NMMB_COL_FIELDS[1:]:
         s[k] = (lm, ncol)
     return s
+
+
 OUTPUT_FIELDS = ("hr_lw", "hr_sw", "flw_up", "flw_dn", "fsw_up", "fsw_dn", "fsw_dn_sfc_clr")
 DIAG_FIELDS = ("lw_jp", "lw_jt", "lw_jt1", "lw_laytrop", "sw_jp", "sw_jt", "sw_jt1", "sw_laytrop",
                "lw_mask", "sw_mask")

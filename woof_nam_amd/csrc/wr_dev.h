@@ -104,6 +104,8 @@ struct ColIn {
     const WR_G uint64_t *seed_dev;  // optional device copy of the McICA seed
     const WR_G int32_t *ncol_dev;   // optional device count of valid columns (a shorter replayed tile)
     const WR_G double *cwp, *cip, *crp, *rew, *rei;  // optional path form of the clouds (cwp set)
+    const WR_G float *aer_sw, *aer_lw;               // optional aerosol optics [3][bands][nlay+1][aer_ld]
+    long aer_ld;
     int ncol;  // stride of the [k][column] fields
     int nlay;  // model layers
 };
@@ -215,6 +217,27 @@ struct RadCol {
             ri = fld(in.rei, l - 1);
             if (in.crp) rp = fld(in.crp, l - 1);
         }
+    }
+    // aerosol of radiation layer l (top-first) in LW band b (0-based): the absorption optical depth
+    // tau (1 - ssa) that lwrad adds to every g-point's gas optical depth (phys_rrtm_lw_lwrad.cu:214-219)
+    __device__ __forceinline__ double aer_lw_tau(int b, int l) const
+    {
+        if (!in.aer_lw) return 0.0;
+        const size_t ld = (size_t)in.aer_ld, nl = (size_t)nrl;
+        const double ta = (double)in.aer_lw[((size_t)b * nl + l) * ld + i];
+        const double sa = (double)in.aer_lw[((size_t)(16 + b) * nl + l) * ld + i];
+        return ta * (1.0 - sa);
+    }
+    // aerosol optical depth, single-scattering albedo and asymmetry of radiation layer l in SW band
+    // b (0 = band 16), swrad's tauae / ssaae / asyae (phys_rrtm_sw_swrad.cu:224-229)
+    __device__ __forceinline__ void aer_sw3(int b, int l, double &ta, double &sa, double &ya) const
+    {
+        ta = 0.0, sa = 0.0, ya = 0.0;
+        if (!in.aer_sw) return;
+        const size_t ld = (size_t)in.aer_ld, nl = (size_t)nrl;
+        ta = (double)in.aer_sw[((size_t)b * nl + l) * ld + i];
+        sa = (double)in.aer_sw[((size_t)(14 + b) * nl + l) * ld + i];
+        ya = (double)in.aer_sw[((size_t)(28 + b) * nl + l) * ld + i];
     }
     __device__ __forceinline__ LayV lay(int l) const
     {

@@ -177,7 +177,8 @@ __global__ void __launch_bounds__(64, WR_GLW_WAVES) k_glw(const Tabs<R> *__restr
             const R blay = planck<R>(T, B, L.tavel);
             const R dplnkd = planck<R>(T, B, tz_bot) - blay;
             const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
-            const GasT<R> q = lw_gas<R>(T, sdif, tau + (R)0);
+            const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
+            const GasT<R> q = lw_gas<R>(T, sdif, tau + taer);
             const R plfrac = fr;
             const R bbdgas = plfrac * (blay + dplnkd * q.gasfac);
             const R gassrcd = bbdgas * q.atrgas;
@@ -226,7 +227,8 @@ __global__ void __launch_bounds__(64, WR_GLW_WAVES) k_glw(const Tabs<R> *__restr
             const R blay = planck<R>(T, B, L.tavel);
             const R dplnku = planck<R>(T, B, tzt) - blay;
             const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
-            const GasT<R> q = lw_gas<R>(T, sdif, tau + (R)0);
+            const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
+            const GasT<R> q = lw_gas<R>(T, sdif, tau + taer);
             const R plfrac = fr;
             const R bbugas = plfrac * (blay + dplnku * q.gasfac);
             const R gassrcu = bbugas * q.atrgas;

@@ -370,6 +370,7 @@ int check_args(const wr_ctx *c, const wr_columns *in, const wr_params *p, const 
         !in->cosz)
         return WR_ERR_ARG;
     if (in->cwp ? (!in->cip || !in->rew || !in->rei) : (!in->qc || !in->qi)) return WR_ERR_ARG;
+    if ((in->aer_sw || in->aer_lw) && in->aer_ld != 0 && in->aer_ld < in->ncol) return WR_ERR_ARG;
     if (!o->hr_lw || !o->hr_sw || !o->flw_up || !o->flw_dn || !o->fsw_up || !o->fsw_dn) return WR_ERR_ARG;
     if (p->iovr < 0 || p->iovr > 2) return WR_ERR_ARG;
     if (!(p->re_liq > 0.0) || !(p->re_ice > 0.0) || !(p->solcon > 0.0)) return WR_ERR_ARG;
@@ -384,6 +385,8 @@ ColIn to_colin(const wr_columns *in)
     ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
     ci.seed_dev = gptr(in->seed_dev), ci.ncol_dev = gptr(in->ncol_dev);
     ci.cwp = gptr(in->cwp), ci.cip = gptr(in->cip), ci.crp = gptr(in->crp), ci.rew = gptr(in->rew), ci.rei = gptr(in->rei);
+    ci.aer_sw = gptr(in->aer_sw), ci.aer_lw = gptr(in->aer_lw);
+    ci.aer_ld = in->aer_ld > 0 ? (long)in->aer_ld : (long)in->ncol;
     ci.ncol = in->ncol, ci.nlay = in->nlay;
     return ci;
 }

@@ -542,10 +542,11 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
+        const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
         R s = (R)0;
 #pragma unroll
         for (int jj = 0; jj < NGL; ++jj) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + (R)0);
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + taer);
             const R plfrac = fr[jj];
             const R bbdgas = plfrac * (blay + dplnkd * q.gasfac);
             const R gassrcd = bbdgas * q.atrgas;
@@ -618,10 +619,11 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             if (lead) acc.lw_up[t] = s0;
         }
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
+        const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
         R s = (R)0;
 #pragma unroll
         for (int jj = 0; jj < NGL; ++jj) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + (R)0);
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[jj] + taer);
             const R plfrac = fr[jj];
             const R bbugas = plfrac * (blay + dplnku * q.gasfac);
             const R gassrcu = bbugas * q.atrgas;
@@ -702,10 +704,11 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R dplnku = planck<R>(T, B, tz_top) - blay;
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
+        const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
         R s = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + taer);
             const R plfrac = fr[g];
             const R gassrcd = plfrac * (blay + dplnkd * q.gasfac) * q.atrgas;
             const R gassrcu = plfrac * (blay + dplnku * q.gasfac) * q.atrgas;
@@ -773,31 +776,35 @@ template <typename R, int B> struct SwLayerRT {
     R rb[SW_NG[B]], rd[SW_NG[B]], tb[SW_NG[B]], td[SW_NG[B]], ld[SW_NG[B]];
 };
 
-// Layer R/T of one g-point from its gas (tg) and Rayleigh (tr) optical depths, plus the cloud
+// Layer R/T of one g-point from its gas (tg) and Rayleigh (tr) optical depths with the aerosol's
+// optical depth, single-scattering albedo and asymmetry (ta, sa, ya), plus the cloud
 // (tau, tau*w, tau*w*g) when cld.  FP32 evaluates in FP64 where FP32 loses the answer:
 // near-conservative scattering (w0 > 1 - 1e-6: 1 - w0 and gamma1 - gamma2 cancel; an FP32 pre-test
 // with a 10x wider margin skips the FP64 test for most layers) and near the two-stream singularity
 // k mu = 1 (flagged by the FP32 evaluation); the optical properties are then summed in FP64.
 template <typename R>
-__device__ __forceinline__ void layer_rt_eval(const Tabs<R> *__restrict__ T, R tr, R tg, bool cld, R ctau, R cssa, R casy,
-                                              R cz, R snt, R &rb, R &rd, R &tb, R &td, R &ld)
+__device__ __forceinline__ void layer_rt_eval(const Tabs<R> *__restrict__ T, R tr, R tg, R ta, R sa, R ya, bool cld, R ctau,
+                                              R cssa, R casy, R cz, R snt, R &rb, R &rd, R &tb, R &td, R &ld)
 {
-    R ztau0 = fmax((R)FTINY, tr + tg + (R)0);
-    R zssa0 = tr + (R)0;
-    R zasy0 = (R)0;
+    // clear-sky optical parameters with the aerosol (phys_rrtm_sw_spcvrtc.cu:323-329)
+    R ztau0 = fmax((R)FTINY, tr + tg + ta);
+    R zssa0 = tr + ta * sa;
+    R zasy0 = ya * sa * ta;
     if (cld) {
         ztau0 = ztau0 + ctau;
         zssa0 = zssa0 + cssa;
         zasy0 = zasy0 + casy;
     }
     if constexpr (sizeof(R) < sizeof(double)) {
-        const double t0 = fmax(FTINY, (double)tr + (double)tg + (cld ? (double)ctau : 0.0));
-        const double s0 = (double)tr + (cld ? (double)cssa : 0.0);
+        const double ta0 = (double)ta, sa0 = (double)sa, ya0 = (double)ya;
+        const double t0 = fmax(FTINY, (double)tr + (double)tg + ta0) + (cld ? (double)ctau : 0.0);
+        const double s0 = (double)tr + ta0 * sa0 + (cld ? (double)cssa : 0.0);
+        const double g0 = ya0 * sa0 * ta0 + (cld ? (double)casy : 0.0);
         bool fp64 = zssa0 > (R)0.99999 * ztau0 && s0 > W0_FP64 * t0;
         if (!fp64) layer_rt<R>(ztau0, zssa0, zasy0, cz, snt, T->sw_exp_tbl, rb, rd, tb, td, ld, &fp64);
         if (fp64) {
             double b, d, e, f, h;
-            layer_rt<double>(t0, s0, cld ? (double)casy : 0.0, (double)cz, (double)snt, T->sw_exp_tbl64, b, d, e, f, h);
+            layer_rt<double>(t0, s0, g0, (double)cz, (double)snt, T->sw_exp_tbl64, b, d, e, f, h);
             rb = (R)b, rd = (R)d, tb = (R)e, td = (R)f, ld = (R)h;
         }
     } else {
@@ -828,11 +835,14 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
         sw_layer<R>(T, rc, par, kin, la, L);
         sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
     }
+    double ta, sa, ya;
+    rc.aer_sw3(B, kin, ta, sa, ya);
     R ctau = (R)0, cssa = (R)1, casy = (R)0;
     if (cf1 > 0.0 && L.ct > FTINY) sw_cloud_l<R>(T, B, L, rc, kin, par, ctau, cssa, casy);
 #pragma unroll
     for (int g = 0; g < NG; ++g)
-        layer_rt_eval<R>(T, tr[g], tg[g], (mb[g] >> k) & 1, ctau, cssa, casy, cz, snt, o.rb[g], o.rd[g], o.tb[g],
+        layer_rt_eval<R>(T, tr[g], tg[g], (R)ta, (R)sa, (R)ya, (mb[g] >> k) & 1, ctau, cssa, casy, cz, snt, o.rb[g],
+                         o.rd[g], o.tb[g],
                          o.td[g], o.ld[g]);
 }
 

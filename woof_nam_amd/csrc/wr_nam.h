@@ -247,6 +247,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
+        const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
         const R clfr = (R)lw_cldfrc(rc, kk);
         const bool top = (lstd >> (kk - 1)) & 1;
         const R c1 = rt.at(FCLR1D, kk - 1, t), d1 = rt.at(FCLD1D, kk - 1, t), m1 = rt.at(FCMB1D, kk - 1, t);
@@ -254,7 +255,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         R s = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + taer);
             const R gassrcd = fr[g] * (blay + dplnkd * q.gasfac) * q.atrgas;
             if (top) {
                 totradd[g] = clfr * radtotd[g];
@@ -330,6 +331,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
             acc.lw_up[t] = s0;
         }
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
+        const R taer = (R)rc.aer_lw_tau(B, kin);  // tautot = taug + tauaer
         const R clfr = (R)lw_cldfrc(rc, kk);
         const bool bot = (lstu >> (kk - 1)) & 1;
         const R c1 = rt.at(FCLR1U, kk + 1, t), d1 = rt.at(FCLD1U, kk + 1, t), m1 = rt.at(FCMB1U, kk + 1, t);
@@ -337,7 +339,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         R s = (R)0;
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
-            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + (R)0);
+            const GasT<R> q = lw_gas<R>(T, sdif, tau[g] + taer);
             const R gasu = fr[g] * (blay + dplnku * q.gasfac) * q.atrgas;
             if (bot) {
                 totradu[g] = clfr * radtotu[g];
@@ -381,9 +383,12 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
     sw_layer<R>(T, rc, par, kin, la, L);
     R tg[NG], tr[NG];
     sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+    double ta, sa, ya;
+    rc.aer_sw3(B, kin, ta, sa, ya);
 #pragma unroll
     for (int g = 0; g < NG; ++g)
-        layer_rt_eval<R>(T, tr[g], tg[g], false, (R)0, (R)1, (R)0, cz, snt, o.rb[g], o.rd[g], o.tb[g], o.td[g], o.ld[g]);
+        layer_rt_eval<R>(T, tr[g], tg[g], (R)ta, (R)sa, (R)ya, false, (R)0, (R)1, (R)0, cz, snt, o.rb[g], o.rd[g], o.tb[g],
+                         o.td[g], o.ld[g]);
     if (pass == 0) return;
     const double zc1d = rc.cfrac(kin) / cf1;
     if (!(zc1d > FTINY)) return;
@@ -393,7 +398,7 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
 #pragma unroll
     for (int g = 0; g < NG; ++g) {
         R rb, rd, tb, td, ld;
-        layer_rt_eval<R>(T, tr[g], tg[g], true, ctau, cssa, casy, cz, snt, rb, rd, tb, td, ld);
+        layer_rt_eval<R>(T, tr[g], tg[g], (R)ta, (R)sa, (R)ya, true, ctau, cssa, casy, cz, snt, rb, rd, tb, td, ld);
         o.rb[g] = zc0 * o.rb[g] + zc1 * rb;
         o.rd[g] = zc0 * o.rd[g] + zc1 * rd;
         o.tb[g] = zc0 * o.tb[g] + zc1 * tb;

@@ -55,11 +55,35 @@ def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
     if nd is not None:
         _check_field("ncol_dev", nd, (1,), torch.int32, device)
     u64 = C.POINTER(C.c_uint64)
+    f32 = C.POINTER(C.c_float)
+    aer, ld = {}, 0
+    for k in abi.AER_FIELDS:
+        a = cols.get(k)
+        if a is not None:
+            p, ldk = _aer_ptr(k, a, ncol, nlay, device)
+            if ld and ldk != ld:
+                raise ValueError("aer_sw and aer_lw must share their column stride")
+            aer[k], ld = C.cast(C.c_void_p(p), f32), ldk
     return abi.WrColumns(ncol, nlay, *[_dptr(cols.get(k)) for k in abi.INPUT_FIELDS],
                          C.cast(C.c_void_p(ids.data_ptr()), u64) if ids is not None else None,
                          C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None,
                          C.cast(C.c_void_p(nd.data_ptr()), C.POINTER(C.c_int32)) if nd is not None else None,
-                         *[_dptr(cols.get(k)) for k in abi.PATH_FIELDS])
+                         *[_dptr(cols.get(k)) for k in abi.PATH_FIELDS], aer.get("aer_sw"), aer.get("aer_lw"), ld)
+
+
+def _aer_ptr(name, a, ncol, nlay, device):
+    """(data pointer, column stride) of aerosol optics [3][bands][nlay+1][ncol] float32 on the device:
+    contiguous, or a column slice [..., c0:c0+ncol] of a wider [3][bands][nlay+1][ld] array (the
+    layout setaer writes over the whole grid)."""
+    shp = abi.aer_shape(name, ncol, nlay)
+    if not isinstance(a, torch.Tensor) or a.dtype != torch.float32 or a.device != device:
+        raise ValueError(f"{name} must be a float32 tensor on {device}")
+    if tuple(a.shape) != shp:
+        raise ValueError(f"{name} has shape {tuple(a.shape)}, expected {shp}")
+    ld = a.stride(2)
+    if a.stride(3) != 1 or ld < ncol or a.stride(1) != shp[2] * ld or a.stride(0) != shp[1] * shp[2] * ld:
+        raise ValueError(f"{name} must be contiguous or a column slice of a contiguous grid array")
+    return a.data_ptr(), ld
 
 
 class Radiation:

@@ -17,6 +17,8 @@ What it does on a call, all of it on the device in library kernels:
     effective radii, low / middle / high cloud cover;
   * solar geometry: the sun's position for the date (sol_update) and each column's cos zenith averaged
     over the sunlit instants of the SW interval (coszmn), on the library (woof_nam_amd.solar);
+  * aerosol optics from the model's aerosol step when it left them in ctx.tables (the reference's
+    rrtm_faersw / rrtm_faerlw, setaer's whole-grid layout), passed as column slices;
   * the radiation call on the path-form columns (woof_nam_amd.radiation, the NAM route mcica = 0;
     FP64 unless ctx.cfg.rrtm_precision = "mixed"), in tiles sized from free device memory;
   * wr_nmmb_post (phys_rrtm_driver_post.cu:71-150): RLWTT, RSWTT (K/s), RLWIN, RSWIN, RSWOUT,
@@ -209,6 +211,28 @@ def post(args, device, rows, cols, out, czmean, lw_step, sw_step, stream=None):
     _lib.check(rc, "wr_nmmb_post")
 
 
+# setaer's outputs in the reference's table keys (phys_rrtm_grrad_aerosols.py: faersw [3, 14, LM+1, JM, IM],
+# faerlw [3, 16, LM+1, JM, IM] float32 on the device)
+AER_KEYS = {"aer_sw": "rrtm_faersw", "aer_lw": "rrtm_faerlw"}
+
+
+def aerosols(ctx, lm, jm, im, c0, c1, device) -> dict:
+    """The aerosol optics of the radiated columns when the model's aerosol step has left them in
+    ctx.tables (setaer's layout over the whole grid): column-slice views, no copy.  Empty otherwise."""
+    out = {}
+    for k, key in AER_KEYS.items():
+        a = ctx.tables.get(key) if hasattr(ctx, "tables") else None
+        if a is None:
+            continue
+        if not isinstance(a, torch.Tensor):
+            a = torch.as_tensor(np.asarray(a))
+        shp = (3, abi.AER_BANDS[k], lm + 1, jm, im)
+        if tuple(a.shape) != shp or a.dtype != torch.float32:
+            raise ValueError(f"rrtm-call: {key} must be float32 {shp}, got {a.dtype} {tuple(a.shape)}")
+        out[k] = a.to(device).contiguous().reshape(3, shp[1], lm + 1, jm * im)[..., c0:c1]
+    return out
+
+
 def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     """The RRTM call of one radiation step (see the module docstring); writes args' outputs in place."""
     nts = int(args["NTIMESTEP"])
@@ -248,6 +272,7 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     cols["emis"] = (_arr(args, "EMISS", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous() if "EMISS" in args
                     else torch.ones(c1 - c0, **f64))
     cols["col_id"] = torch.arange(c0, c1, dtype=torch.int64, device=device)
+    cols.update(aerosols(ctx, lm, jm, im, c0, c1, device))
     params = Radiation.default_params(mcica=mcica, solcon=float(args["SOLCON"]), seed=int(_cfg(ctx, "rrtm_seed", 0)))
     rad = _context(ctx, device, c1 - c0, lm)
     if stamp is not None:
