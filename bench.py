@@ -152,7 +152,7 @@ def served_block(d, dev, ncol, seconds):
           file=sys.stderr, flush=True)
     srv = Server(0, tile=best["tile"], slots=best["slots"], budget_gb=200.0, precision="mixed",
                  filler={k: v[..., :best["tile"]] for k, v in d.items()})
-    r = poisson_run(srv, kinds, rate=40.0, duration=seconds, seed=0)
+    r = poisson_run(srv, kinds, rate=40.0, duration=seconds, seed=0, progress=30.0 if seconds > 30 else 0.0)
     srv.close()
     r["autotune"] = tune["all"]
     r["tile"], r["slots"] = best["tile"], best["slots"]
@@ -353,9 +353,6 @@ def main(argv=None):
         del mout
         print(f"# mixed: {tm * 1e3:.1f} ms/step, {ncol / tm:.4g} columns/s", file=sys.stderr, flush=True)
 
-    if a.served_seconds > 0:
-        res["served"] = served_block(d, dev, ncol, a.served_seconds)
-
     if a.precision == "mixed" and a.acc_cols > 0:
         # error budget against the FP64 path (bit-identical to the CPU oracle) on a column sample,
         # per layer and per band
@@ -384,8 +381,11 @@ def main(argv=None):
         res["cpu_baseline"], res["accuracy"] = cpu_baseline(cols, out, ncol)
     else:
         res["cpu_baseline"] = None
+    rad.close()                     # the served run's slots and requests get the device memory
+    if a.served_seconds > 0:
+        torch.cuda.empty_cache()
+        res["served"] = served_block(d, dev, ncol, a.served_seconds)
     print(json.dumps(res), flush=True)
-    rad.close()
     return 0
 
 

@@ -81,6 +81,8 @@ typedef struct {
     const float *aer_sw;
     const float *aer_lw;
     int64_t aer_ld;
+    const double *o3_top;    /* optional [ncol] ozone mass mixing ratio of the layer the radiation adds
+                                above the model top (climatological ozone); NULL: the top layer's   */
 } wr_columns;
 
 /* Call parameters (NAM values by default; see wr_default_params). */
@@ -214,11 +216,25 @@ typedef struct {
     const float *glat;       /* [jm][im] latitude, radians                                            */
 } wr_nmmb_state;
 
+/* Climatological ozone (getozn, phys_rrtm_getozn.cu:44-111, the NAM's ntoz = 0 branch): the monthly
+ * zonal-mean climatology gas_init reads, interpolated in latitude, between two months and in the Exner
+ * function of pressure. */
+typedef struct {
+    const float *o3r;        /* [12][loz][jmr] ozone mass mixing ratio (device; ppmv x 1.655e-6)    */
+    const float *pstr;       /* [loz] its pressure levels, hPa, top first (device)                 */
+    int32_t loz, jmr;        /* levels (<= 64) and latitudes                                       */
+    float blte, dlte;        /* first latitude and latitude spacing, degrees                       */
+    int32_t k1oz, k2oz;      /* the two months (1..12), gas_update's time factors                  */
+    float facoz;             /* weight of k2oz                                                     */
+} wr_ozone;
+
 typedef struct {
     int32_t llyr;            /* top of the boundary-layer cloud: 1-based top-first radiation layer,
                                 cld_init's rule (radiation_clouds_nmmb.f:327-344, ivflip = 0)         */
     int32_t iovr;            /* gethml cloud overlap: 0 random, 1 maximum-random                      */
     int32_t subgrid;         /* 1: subgrid cloud water where RH > 95 % and no condensate (the NAM's)   */
+    const wr_ozone *ozone;   /* climatological ozone for the column call (with o3_top), or NULL: the
+                                state's O3 (the added top layer takes the top model layer's)          */
 } wr_nmmb_opts;
 
 typedef struct {             /* device arrays the prep writes; ncol = (j1 - j0) * im                  */
@@ -230,6 +246,8 @@ typedef struct {             /* device arrays the prep writes; ncol = (j1 - j0) 
     int32_t *mtop, *mbot;    /* [3][ncol] top / base layer of low, middle, high (1-based top-first
                                 radiation layers, the added layer above the model top is layer 1)    */
     float *p_phy;            /* optional [lm][jm][im] state field P_PHY (radiated rows), or NULL      */
+    double *o3_top;          /* [ncol] ozone of the layer added above the model top (written with the
+                                climatological ozone; may be NULL otherwise)                           */
 } wr_nmmb_cols;
 
 typedef struct {             /* state outputs (float32 device arrays; int32 counters)                */
@@ -247,6 +265,15 @@ int wr_nmmb_prep(const wr_nmmb_state *st, const wr_nmmb_opts *opt, const wr_nmmb
  * czmean [ncol]: the call's mean cosine of the zenith angle; lw_step / sw_step: which parts ran. */
 int wr_nmmb_post(const wr_nmmb_state *st, const wr_nmmb_cols *cols, const wr_outputs *rad, const double *czmean,
                  int32_t lw_step, int32_t sw_step, const wr_nmmb_out *out, void *stream);
+
+/* NAM surface SW albedo of the radiated columns (setalb with ialbflg = 2: the NMMB radiation
+ * driver's IALB = 2 block, phys_rrtm_grrad_clouds.cu:345-396, which setalb copies,
+ * phys_rrtm_setalb.cu:241-247): the state's ALBEDO, except over open water (SM > 0.5) with the sun up
+ * (coszen > 0), where the Payne (1972) open-water table at the fixed atmospheric transmission 0.537
+ * gives it by zenith angle.  albedo, sm: [jm][im] float32 state; coszen [ncol] the call's cos zenith
+ * of the radiated rows; alb [ncol] out, the column call's input.  Only st's grid fields are read. */
+int wr_nmmb_albedo(const wr_nmmb_state *st, const float *albedo, const float *sm, const double *coszen, double *alb,
+                   void *stream);
 
 /* ---- solar geometry around the call ---------------------------------------------------------------
  * wr_sol_update: sol_update_nmmb (phys_rrtm_sol_update.cu:16-160): the sun's position for a date

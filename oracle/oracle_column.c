@@ -204,7 +204,7 @@ static void o_prep(const wr_columns *in, const wr_params *p, int i, Rad *r)
     r->plyr[0] = 0.5 * r->plvl[1];
     r->tlyr[0] = r->tlyr[1];
     r->qlyr[0] = r->qlyr[1];
-    r->olyr[0] = r->olyr[1];
+    r->olyr[0] = in->o3_top ? fmax(1.0e-10, in->o3_top[i]) : r->olyr[1];
     r->tskn = in->t_sfc[i];
     /* tlvl by log-pressure interpolation; tlvl(top) = tlyr(top), tlvl(surface) = skin T */
     double la[O_NL], lb[O_NL];

@@ -3,6 +3,7 @@
  * writes the radiation outputs back into the state.  Same operation sequence as the library's
  * wr_nmmb_prep / wr_nmmb_post (FP64 on the single-precision state, own exp/log/pow), so the two
  * agree bit for bit.  Each step cites the reference kernel it restates. */
+#include "oalbd.h"
 #include "oracle.h"
 
 /* physcons of progcld2 (phys_rrtm_progcld2.cu:49-55) and grrad setup (phys_rrtm_grrad_setup.cu:16) */
@@ -54,6 +55,34 @@ static double n_fpvs(double t)
 #define S3(a, k, j, i) (a)[((size_t)(k) * st->jm + (j)) * st->im + (i)]
 #define S2(a, j, i) (a)[(size_t)(j) * st->im + (i)]
 
+/* the Exner function (p / 1e5 Pa)^(con_rd / con_cp) as funcphys' fpkap table gives it: linear between
+ * nodes every 10 Pa (phys_rrtm_gas_init.cu:52-72), the nodes evaluated here */
+static double n_fpkap(double p)
+{
+    const double rocp = (double)(287.05f / 1004.6f);
+    const double xinc = 110000.0 / 11000.0;
+    const double xj = fmin(fmax(1.0 + p / xinc, 1.0), 11001.0);
+    const int jx = (int)fmin(xj, 11000.0);
+    const double t0 = om_pow((double)(jx - 1) * xinc / 1.0e5, rocp);
+    const double t1 = om_pow((double)jx * xinc / 1.0e5, rocp);
+    return t0 + (xj - (double)jx) * (t1 - t0);
+}
+
+/* getozn (phys_rrtm_getozn.cu:57-110) at one layer of Exner function wk: linear in the Exner function
+ * between the climatology's levels pk, the end values outside */
+static double n_ozone_at(const double *o3i, const double *pk, int loz, double wk)
+{
+    if (wk > pk[loz - 1]) return o3i[loz - 1];
+    if (wk <= pk[0]) return o3i[0];
+    double v = o3i[0];
+    for (int k = 0; k < loz - 1; ++k)
+        if (wk > pk[k] && wk <= pk[k + 1]) {
+            const double tem = (pk[k + 1] - wk) * (1.0 / (pk[k + 1] - pk[k]));
+            v = tem * o3i[k] + (1.0 - tem) * o3i[k + 1];
+        }
+    return v;
+}
+
 /* One column c of the prep (see include/woofrad.h, wr_nmmb_prep). */
 static void n_prep_col(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_nmmb_cols *w, int c)
 {
@@ -70,6 +99,39 @@ static void n_prep_col(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_
         w->p_lev[(size_t)(k + 1) * nc + c] = (double)p8;
         if (w->p_phy) S3(w->p_phy, k, j, i) = st->sgml2[k] * pd + st->psgml1[k];
     }
+    /* climatological ozone (getozn): this latitude's climatology column, between the two months */
+    const wr_ozone *oz = o->ozone;
+    int oz_on = oz != NULL;
+    double o3i[64], pk[64];
+    if (oz_on) {
+        const double deglat = (double)S2(st->glat, j, i) * 57.29578;
+        const double blte = oz->blte, dlte = oz->dlte, elte = blte + (double)(oz->jmr - 1) * dlte;
+        int j1, j2;
+        double tem1;
+        if (deglat > blte && deglat < elte) {
+            tem1 = (deglat - blte) / dlte + 1.0;
+            j1 = (int)tem1;
+            j2 = j1 + 1;
+            tem1 = tem1 - (double)j1;
+        } else if (deglat <= blte) {
+            j1 = j2 = 1, tem1 = 1.0;
+        } else if (deglat >= elte) {
+            j1 = j2 = oz->jmr, tem1 = 1.0;
+        } else {
+            j1 = j2 = 1, tem1 = 1.0, oz_on = 0;
+        }
+        const double tem2 = 1.0 - tem1, facoz = (double)oz->facoz;
+        const double rocp = (double)(287.05f / 1004.6f);
+        for (int k = 0; k < oz->loz; ++k) {
+            const size_t m1 = ((size_t)(oz->k1oz - 1) * oz->loz + k) * oz->jmr;
+            const size_t m2 = ((size_t)(oz->k2oz - 1) * oz->loz + k) * oz->jmr;
+            const double tem3 = tem2 * (double)oz->o3r[m1 + j1 - 1] + tem1 * (double)oz->o3r[m1 + j2 - 1];
+            const double tem4 = tem2 * (double)oz->o3r[m2 + j1 - 1] + tem1 * (double)oz->o3r[m2 + j2 - 1];
+            o3i[k] = tem4 * facoz + tem3 * (1.0 - facoz);
+            const double ps = (double)oz->pstr[k];
+            pk[k] = ps >= 0.1 ? n_fpkap(ps * 100.0) : om_pow(ps / 1000.0, rocp);
+        }
+    }
     /* radiation layers, top-first, layer 0 = the added layer above the model top: plvl, plyr in hPa
      * as the column call forms them from p_lev */
     double plvl[O_NL + 1], plyr[O_NL], ct[O_NL];
@@ -77,6 +139,9 @@ static void n_prep_col(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_
     for (int l = 1; l <= nrl; ++l) plvl[l] = 10.0 * (w->p_lev[(size_t)(l - 1) * nc + c] * 1.0e-3);
     if (plvl[1] <= 1.0e-6) plvl[1] = 2.0e-6;
     plyr[0] = 0.5 * plvl[1];
+    if (w->o3_top)
+        w->o3_top[c] = oz_on ? n_ozone_at(o3i, pk, oz->loz, om_pow(plyr[0] * 100.0 / 1.0e5, (double)(287.05f / 1004.6f)))
+                             : (double)S3(st->o3, 0, j, i);
     for (int l = 1; l < nrl; ++l)
         plyr[l] = 10.0 * (0.5 * (w->p_lev[(size_t)(l - 1) * nc + c] + w->p_lev[(size_t)l * nc + c]) * 1.0e-3);
     ct[0] = 0.0;
@@ -86,7 +151,12 @@ static void n_prep_col(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_
         const double t = (double)S3(st->t, k, j, i), q = (double)S3(st->q, k, j, i);
         w->t_lay[kc] = t;
         w->q_lay[kc] = q;
-        w->o3_lay[kc] = (double)S3(st->o3, k, j, i);
+        if (oz_on) { /* the model layer's Exner function as the driver call forms it (PRSLK) */
+            const float prsl = (st->sgml2[k] * pd + st->psgml1[k]) / 1000.0f;
+            w->o3_lay[kc] = n_ozone_at(o3i, pk, oz->loz, om_pow((double)prsl * 0.01, (double)(287.04f / 1004.6f)));
+        } else {
+            w->o3_lay[kc] = (double)S3(st->o3, k, j, i);
+        }
         double cw = (double)S3(st->cw, k, j, i);
         double fice = (double)S3(st->f_ice, k, j, i), frain = (double)S3(st->f_rain, k, j, i);
         /* driver call SUBGRID (phys_rrtm_driver_call.cu:248-275): RH > 95 % without condensate */
@@ -255,6 +325,9 @@ static void n_prep_col(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_
 int oracle_nmmb_prep(const wr_nmmb_state *st, const wr_nmmb_opts *o, const wr_nmmb_cols *w)
 {
     if (!st || !o || !w || st->lm + 1 > O_NL - 1 || st->j0 < 0 || st->j1 > st->jm || st->j0 >= st->j1) return 1;
+    if (o->ozone && (o->ozone->loz < 2 || o->ozone->loz > 32 || o->ozone->k1oz < 1 || o->ozone->k1oz > 12 ||
+                     o->ozone->k2oz < 1 || o->ozone->k2oz > 12 || !w->o3_top))
+        return 1;
     const int n = (st->j1 - st->j0) * st->im;
 #pragma omp parallel for schedule(static)
     for (int c = 0; c < n; ++c) n_prep_col(st, o, w, c);
@@ -303,3 +376,37 @@ int oracle_nmmb_post(const wr_nmmb_state *st, const wr_nmmb_cols *w, const wr_ou
 }
 #undef S3
 #undef S2
+
+/* NAM surface albedo (the NMMB radiation driver's IALB = 2 block, phys_rrtm_grrad_clouds.cu:345-396;
+ * setalb ialbflg 2 copies it): the state's albedo; over open water (sm > 0.5) with the sun up, the
+ * Payne (1972) table at transmission 0.537, linear in zenith angle between its nodes and in
+ * transmission between its 0.50 and 0.55 columns. */
+int oracle_nmmb_albedo(int im, int jm, int j0, int j1, const float *albedo, const float *sm, const double *coszen,
+                       double *alb)
+{
+    static const double za[20] = WR_ALBD_ZA, dza[19] = WR_ALBD_DZA, lo[20] = WR_ALBD_LO, hi[20] = WR_ALBD_HI;
+    if (im <= 0 || jm <= 0 || j0 < 0 || j1 > jm || j0 >= j1 || !albedo || !sm || !coszen || !alb) return WR_ERR_ARG;
+    const int nc = (j1 - j0) * im;
+    for (int c = 0; c < nc; ++c) {
+        const size_t p = (size_t)(j0 + c / im) * im + c % im;
+        double a = (double)albedo[p];
+        const double cz = coszen[c];
+        if (cz > 0.0 && (double)sm[p] > 0.5) {
+            const double zen = (180.0 / 3.14159265) * acos(fmax(cz, 0.0));
+            int jx;
+            if (zen >= 74.0)
+                jx = (int)(0.5 * (90.0 - zen) + 1.0);
+            else if (zen >= 50.0)
+                jx = (int)(0.25 * (74.0 - zen) + 9.0);
+            else
+                jx = (int)(0.1 * (50.0 - zen) + 15.0);
+            jx = imin(19, imax(1, jx));
+            const double dzen = -(zen - za[jx - 1]) / dza[jx - 1];
+            const double a1 = lo[jx - 1] + dzen * (lo[jx] - lo[jx - 1]);
+            const double a2 = hi[jx - 1] + dzen * (hi[jx] - hi[jx - 1]);
+            a = a1 + 20.0 * (a2 - a1) * (0.537 - WR_ALBD_TRN_LO);
+        }
+        alb[c] = a;
+    }
+    return WR_OK;
+}

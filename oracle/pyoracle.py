@@ -105,9 +105,11 @@ def _nmmb_state(st: dict, im, jm, lm, j0, j1, pt):
                                                        "f_ice", "f_rain", "glat")])
 
 
-def nmmb_prep(st: dict, dims, opts):
+def nmmb_prep(st: dict, dims, opts, ozone: dict | None = None):
     """oracle_nmmb_prep on host float32 state arrays; dims = (im, jm, lm, j0, j1, pt), opts = (llyr, iovr,
-    subgrid).  Returns the column dict (float64 [k][ncol]) with clds, mtop, mbot and p_phy."""
+    subgrid); ozone: the climatology (o3r float32 [12][loz][jmr] mass mixing ratio, pstr float32 [loz]
+    hPa, blte, dlte, k1oz, k2oz, facoz) or None.  Returns the column dict (float64 [k][ncol]) with clds,
+    mtop, mbot, p_phy and o3_top."""
     im, jm, lm, j0, j1, pt = dims
     ncol = (j1 - j0) * im
     cols = {k: np.zeros(s) for k, s in abi.nmmb_col_shapes(ncol, lm).items()}
@@ -115,11 +117,20 @@ def nmmb_prep(st: dict, dims, opts):
     cols["mtop"] = np.zeros((3, ncol), dtype=np.int32)
     cols["mbot"] = np.zeros((3, ncol), dtype=np.int32)
     cols["p_phy"] = np.full((lm, jm, im), -1.0, dtype=np.float32)
+    cols["o3_top"] = np.zeros(ncol)
     cst = abi.WrNmmbCols(*[_ptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _ptr(cols["clds"], C.c_float),
-                         _ptr(cols["mtop"], C.c_int32), _ptr(cols["mbot"], C.c_int32), _ptr(cols["p_phy"], C.c_float))
+                         _ptr(cols["mtop"], C.c_int32), _ptr(cols["mbot"], C.c_int32), _ptr(cols["p_phy"], C.c_float),
+                         _ptr(cols["o3_top"]))
+    oz = None
+    if ozone is not None:
+        o3r = np.ascontiguousarray(ozone["o3r"], dtype=np.float32)
+        pstr = np.ascontiguousarray(ozone["pstr"], dtype=np.float32)
+        oz = abi.WrOzone(_ptr(o3r, C.c_float), _ptr(pstr, C.c_float), pstr.size, o3r.shape[2], ozone["blte"],
+                         ozone["dlte"], ozone["k1oz"], ozone["k2oz"], ozone["facoz"])
+    o = abi.WrNmmbOpts(*opts, C.pointer(oz) if oz is not None else None)
     L = lib()
     L.oracle_nmmb_prep.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbOpts), C.POINTER(abi.WrNmmbCols)]
-    rc = L.oracle_nmmb_prep(C.byref(_nmmb_state(st, *dims)), C.byref(abi.WrNmmbOpts(*opts)), C.byref(cst))
+    rc = L.oracle_nmmb_prep(C.byref(_nmmb_state(st, *dims)), C.byref(o), C.byref(cst))
     if rc != 0:
         raise ValueError(f"oracle_nmmb_prep: bad arguments ({rc})")
     return cols
@@ -140,6 +151,22 @@ def nmmb_post(st: dict, dims, cols: dict, rad: dict, czmean, lw_step: bool, sw_s
                             int(sw_step), C.byref(oo))
     if rc != 0:
         raise ValueError(f"oracle_nmmb_post: bad arguments ({rc})")
+
+
+def nmmb_albedo(albedo, sm, coszen, rows):
+    """oracle_nmmb_albedo: the NAM surface albedo of rows j0..j1 ([jm][im] float32 albedo and sea mask,
+    coszen [ncol] float64)."""
+    jm, im = albedo.shape
+    a = np.ascontiguousarray(albedo, dtype=np.float32)
+    s = np.ascontiguousarray(sm, dtype=np.float32)
+    cz = np.ascontiguousarray(coszen, dtype=np.float64)
+    out = np.zeros(cz.size)
+    L = lib()
+    fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.oracle_nmmb_albedo.argtypes = [C.c_int] * 4 + [fp, fp, dp, dp]
+    if L.oracle_nmmb_albedo(im, jm, rows[0], rows[1], _ptr(a, C.c_float), _ptr(s, C.c_float), _ptr(cz), _ptr(out)) != 0:
+        raise ValueError("oracle_nmmb_albedo: bad arguments")
+    return out
 
 
 def sol_update(yr, mon, day, hr, mn, sec, deltsw, deltim, solc0):
@@ -191,13 +218,15 @@ def run(cols: dict, params: abi.WrParams | None = None, nthreads: int | None = N
     ids = cols.get("col_id")
     ids = None if ids is None else np.ascontiguousarray(ids, dtype=np.uint64)
     aer = {k: np.ascontiguousarray(cols[k], dtype=np.float32) for k in abi.AER_FIELDS if cols.get(k) is not None}
+    o3t = None if cols.get("o3_top") is None else np.ascontiguousarray(cols["o3_top"], dtype=np.float64)
     for k, a in aer.items():
         if a.shape != abi.aer_shape(k, ncol, nlay):
             raise ValueError(f"{k}: shape {a.shape}, expected {abi.aer_shape(k, ncol, nlay)}")
     ci = abi.WrColumns(ncol, nlay, *[_ptr(arrs.get(k)) for k in abi.INPUT_FIELDS],
                        _ptr(ids, C.c_uint64) if ids is not None else None, None, None,
                        *[_ptr(arrs.get(k)) if path else None for k in abi.PATH_FIELDS],
-                       *[_ptr(aer[k], C.c_float) if k in aer else None for k in abi.AER_FIELDS], 0)
+                       *[_ptr(aer[k], C.c_float) if k in aer else None for k in abi.AER_FIELDS], 0,
+                       _ptr(o3t) if o3t is not None else None)
     outs = {k: np.zeros(s) for k, s in abi.output_shapes(ncol, nlay).items()}
     co = abi.WrOutputs(*[_ptr(outs[k]) for k in abi.OUTPUT_FIELDS])
     dg = None

@@ -24,10 +24,11 @@ def test_every_declared_symbol_is_exported():
 
 def test_struct_layouts_match_header():
     # x86-64 SysV layout of the header structs
-    assert C.sizeof(abi.WrColumns) == 8 + 22 * 8
+    assert C.sizeof(abi.WrColumns) == 8 + 23 * 8
     assert C.sizeof(abi.WrNmmbState) == 6 * 4 + 12 * 8
-    assert C.sizeof(abi.WrNmmbOpts) == 3 * 4
-    assert C.sizeof(abi.WrNmmbCols) == 14 * 8
+    assert C.sizeof(abi.WrNmmbOpts) == 3 * 4 + 4 + 8
+    assert C.sizeof(abi.WrOzone) == 2 * 8 + 7 * 4 + 4
+    assert C.sizeof(abi.WrNmmbCols) == 15 * 8
     assert C.sizeof(abi.WrNmmbOut) == 18 * 8
     assert C.sizeof(abi.WrParams) == 4 + 4 + 8 + 3 * 8 + 10 * 8
     assert C.sizeof(abi.WrOutputs) == 7 * 8

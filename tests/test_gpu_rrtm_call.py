@@ -14,7 +14,7 @@ import pytest
 import torch
 
 from oracle import pyoracle
-from woof_nam_amd import abi, gen, rad_driver, rrtm_call, solar
+from woof_nam_amd import abi, gen, ozone, rad_driver, rrtm_call, solar
 
 pytestmark = pytest.mark.gpu
 
@@ -44,7 +44,7 @@ def ferrier_state(seed=0):
             "PDSG1": f32(np.diff(sg1) * pdtop), "DSG2": f32(np.diff(sg2)),
             "PSGML1": f32(0.5 * (sg1[1:] + sg1[:-1]) * pdtop + gen.PT), "SGML2": f32(0.5 * (sg2[1:] + sg2[:-1])),
             "TSKIN": f32(c["t_sfc"].reshape(JM, IM)), "ALBEDO": f32(c["alb"].reshape(JM, IM)),
-            "SM": f32(np.ones((JM, IM)))}
+            "SM": f32(np.zeros((JM, IM)))}   # land: the albedo is ALBEDO (the open-water table has its own test)
 
 
 def oracle_state(h):
@@ -75,16 +75,31 @@ def dev_args(h, device):
     return a
 
 
-def test_prep_matches_oracle(gpu_device):
-    print("case nmmb prep", flush=True)
+def ozone_case(month, day, hour):
+    """The climatology with gas_update's time factors of a date: (host dict for the oracle, WrOzone)."""
+    tf = ozone.TimeFactors().update(month, day, hour)
+    host = {**ozone.host_tables(), "k1oz": tf.k1oz, "k2oz": tf.k2oz, "facoz": float(tf.facoz)}
+    return host, tf
+
+
+@pytest.mark.parametrize("with_ozone", [False, True])
+def test_prep_matches_oracle(gpu_device, with_ozone):
+    print(f"case nmmb prep ozone={with_ozone}", flush=True)
     h = ferrier_state(1)
     llyr = rrtm_call.boundary_layer_top(rrtm_call.sigma_interfaces(dev_args(h, gpu_device), LM))
     args = dev_args(h, gpu_device)
     args["P_PHY"] = torch.full((LM, JM, IM), -1.0, dtype=torch.float32, device=gpu_device)
-    cols, _ = rrtm_call.prep(args, gpu_device, (1, JM - 1), llyr, iovr=1, subgrid=True)
+    oz_host, oz_dev = None, None
+    if with_ozone:
+        oz_host, tf = ozone_case(3, 7, 6)
+        clim = ozone.Climatology(gpu_device)
+        oz_dev = clim.struct(tf)
+    cols, _ = rrtm_call.prep(args, gpu_device, (1, JM - 1), llyr, iovr=1, subgrid=True, ozone=oz_dev)
     torch.cuda.synchronize()
-    ref = pyoracle.nmmb_prep(oracle_state(h), (IM, JM, LM, 1, JM - 1, np.float32(gen.PT)), (llyr, 1, 1))
-    for k in abi.NMMB_COL_FIELDS + ("clds", "mtop", "mbot"):
+    ref = pyoracle.nmmb_prep(oracle_state(h), (IM, JM, LM, 1, JM - 1, np.float32(gen.PT)), (llyr, 1, 1), ozone=oz_host)
+    if with_ozone:
+        assert not np.array_equal(ref["o3_lay"], np.ascontiguousarray(h["O3"][:, 1:JM - 1].reshape(LM, -1), dtype=np.float64))
+    for k in abi.NMMB_COL_FIELDS + ("clds", "mtop", "mbot", "o3_top"):
         a = cols[k].cpu().numpy()
         assert np.array_equal(a, ref[k]), f"{k}: {np.count_nonzero(a != ref[k])} elements differ"
     assert np.array_equal(args["P_PHY"].cpu().numpy(), ref["p_phy"])
@@ -151,13 +166,14 @@ def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
     d = (IM, JM, LM, 1, JM - 1, np.float32(gen.PT))
     args = dev_args(h, gpu_device)
     llyr = rrtm_call.boundary_layer_top(rrtm_call.sigma_interfaces(args, LM))
-    cols = pyoracle.nmmb_prep(oracle_state(h), d, (llyr, 1, 1))
+    t = solar.current(ctx.tables["JDAT"], 3 * 60.0)
+    cols = pyoracle.nmmb_prep(oracle_state(h), d, (llyr, 1, 1), ozone=ozone_case(t.month, t.day, t.hour)[0])
     c0, c1 = IM, (JM - 1) * IM
     glat = args["GLAT"].reshape(-1)[c0:c1]
     glon = args["GLON"].reshape(-1)[c0:c1]
     cz = solar.coszen(ctx.tables["JDAT"], 3 * 60.0, 3, 60.0, glat, glon)[0].cpu().numpy()
     flat = lambda a: np.ascontiguousarray(a.reshape(-1)[c0:c1], dtype=np.float64)
-    rc = {k: cols[k] for k in abi.NMMB_COL_FIELDS}
+    rc = {k: cols[k] for k in abi.NMMB_COL_FIELDS + ("o3_top",)}
     rc.update({"cosz": cz, "t_sfc": flat(h["TSKIN"]), "alb": flat(h["ALBEDO"]), "emis": np.ones(c1 - c0),
                "col_id": np.arange(c0, c1, dtype=np.uint64)})
     for k, key in rrtm_call.AER_KEYS.items():
@@ -180,6 +196,25 @@ def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
     # what the state shows: heating where there is cloud, ice-only and rainy layers radiated
     assert (st["CLDFRA"].cpu().numpy()[:, 1:-1] > 0).mean() > 0.05
     assert float(st["RSWTT"].abs().max()) > 0 and float(st["RLWTT"].abs().max()) > 0
+
+
+def test_albedo_matches_oracle(gpu_device):
+    """wr_nmmb_albedo against oracle_nmmb_albedo: ALBEDO over land and at night, the Payne open-water
+    table over sea in daylight.  The zenith angle comes from the device's and the host's acos, which
+    may differ in the last unit, so the albedo agrees to 1e-13."""
+    print("case nmmb albedo", flush=True)
+    rng = np.random.default_rng(8)
+    h = ferrier_state(5)
+    h["SM"] = (rng.uniform(size=(JM, IM)) < 0.5).astype(np.float32)
+    args = dev_args(h, gpu_device)
+    nc = (JM - 2) * IM
+    cz = rng.uniform(-0.2, 1.0, nc)
+    got = rrtm_call.albedo(args, gpu_device, (1, JM - 1), torch.from_numpy(cz).to(gpu_device))
+    torch.cuda.synchronize()
+    ref = pyoracle.nmmb_albedo(h["ALBEDO"], h["SM"], cz, (1, JM - 1))
+    assert np.abs(got.cpu().numpy() - ref).max() <= 1e-13
+    water_day = (h["SM"][1:JM - 1].reshape(-1) > 0.5) & (cz > 0)
+    assert water_day.sum() > 20 and np.any(ref[water_day] != h["ALBEDO"][1:JM - 1].reshape(-1)[water_day])
 
 
 def test_cadence_and_zeroing(gpu_device):

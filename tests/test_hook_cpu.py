@@ -128,3 +128,32 @@ def test_mcica_route_refused_by_the_hook():
     args = rad_driver.rrtm_args(ctx, ctx.tables["JDAT"], 3, 60.0, rad_driver.MP_FLAGS["gfs"], 1361.0)
     with pytest.raises(NotImplementedError, match="RSWINC"):
         rrtm_call.run(ctx, args)
+
+
+def test_restart_state_round_trip():
+    """A run resumed from the hook's saved state takes the same ozone time factors as the run that went
+    on: gas_update's months change only at mid-month, so the saved months matter between change days."""
+    import json
+    from types import SimpleNamespace
+    from woof_nam_amd import ozone
+    ctx = SimpleNamespace(tables={})
+    tf = ozone.TimeFactors()
+    for day in range(1, 13):                          # the run so far: January 1 to 12
+        tf.update(1, day, 6)
+    ctx.tables[rrtm_call.KEY_OZONE] = (None, tf)
+    ctx.tables[rrtm_call.KEY_LLYR] = 57
+    saved = json.loads(json.dumps(rrtm_call.state_dict(ctx)))
+    resumed = SimpleNamespace(tables={})
+    rrtm_call.load_state_dict(resumed, saved)
+    a, b = tf, resumed.tables[rrtm_call.KEY_OZONE][1]
+    for month, day in ((1, 13), (1, 15), (1, 16), (1, 30), (2, 14), (2, 15)):
+        a.update(month, day, 6)
+        b.update(month, day, 6)
+        assert (a.k1oz, a.k2oz, a.facoz) == (b.k1oz, b.k2oz, b.facoz)
+    assert resumed.tables[rrtm_call.KEY_LLYR] == 57
+    # without the saved state a resumed run would pick other months: with no call at hour 0 of the
+    # middle day, the running state keeps December / January where a first call takes January / February
+    fresh = ozone.TimeFactors().update(1, 30, 6)
+    assert (fresh.k1oz, fresh.k2oz) == (1, 2) and (b.k1oz, b.k2oz) == (12, 1)
+    with pytest.raises(ValueError):
+        rrtm_call.load_state_dict(resumed, {"format": 9})

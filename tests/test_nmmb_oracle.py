@@ -224,3 +224,115 @@ def test_post_outputs_and_accumulators():
     assert out["acfrst"][2, 2] == np.float32(0.5) + cols["clds"][3, c]
     assert out["ncfrst"][2, 2] == 3 and out["ncfrcv"][2, 2] == 3 and out["acfrcv"][2, 2] == np.float32(0.5)
     assert (out["cldfra"][:, 0, :] == -3.0).all() and out["ncfrst"][0, 0] == 2   # rows outside J_S..J_E
+
+
+def payne(cz):
+    """The Payne open-water albedo at transmission 0.537 by piecewise-linear interpolation of the table
+    (ocean_albedo.json) in the zenith angle, numpy's interp standing in for the driver's node search."""
+    import json
+    from pathlib import Path
+    d = json.loads((Path(__file__).resolve().parents[1] / "woof_nam_amd" / "data" / "ocean_albedo.json").read_text())
+    za = np.array(d["ZA"])[::-1]
+    tab = np.array(d["ALBD_zenith_by_transmission"])[::-1]
+    iq = int(20.0 * 0.537 + 1.0)
+    zen = (180.0 / 3.14159265) * np.arccos(np.maximum(cz, 0.0))
+    a1, a2 = np.interp(zen, za, tab[:, iq - 1]), np.interp(zen, za, tab[:, iq])
+    return a1 + 20.0 * (a2 - a1) * (0.537 - d["TRN"][iq - 1])
+
+
+def test_nam_albedo_payne_table_over_water_in_daylight():
+    rng = np.random.default_rng(21)
+    jm, im = 6, 50
+    alb = rng.uniform(0.05, 0.8, (jm, im)).astype(np.float32)
+    sm = (rng.uniform(size=(jm, im)) < 0.6).astype(np.float32)          # 1 = sea
+    j0, j1 = 1, jm - 1
+    nc = (j1 - j0) * im
+    cz = rng.uniform(-0.3, 1.0, nc)
+    cz[:20] = np.cos(np.radians(np.array([90, 88, 86, 84, 82, 80, 78, 76, 74, 70, 66, 62, 58, 54, 50, 40, 30, 20, 10,
+                                          0.0]) * 3.14159265 / np.pi))   # the table's own zenith nodes
+    got = pyoracle.nmmb_albedo(alb, sm, cz, (j0, j1))
+    a = alb[j0:j1].reshape(-1).astype(np.float64)
+    s = sm[j0:j1].reshape(-1)
+    water_day = (s > 0.5) & (cz > 0.0)
+    assert np.array_equal(got[~water_day], a[~water_day])                 # land, or the sun down: ALBEDO
+    assert np.allclose(got[water_day], payne(cz[water_day]), rtol=0, atol=1e-12)
+    assert water_day.sum() > 50
+    # the curve: 0.05-0.6, higher the lower the sun
+    z = np.linspace(0.0, 89.9, 400)
+    cz2 = np.cos(np.radians(z * 3.14159265 / np.pi))
+    one = pyoracle.nmmb_albedo(np.zeros((1, 400), np.float32), np.ones((1, 400), np.float32), cz2, (0, 1))
+    assert np.all(np.diff(one) >= -1e-15) and 0.045 < one.min() and one.max() < 0.6
+
+
+def _clim(fn, pstr=None):
+    """A synthetic ozone climatology o3r[m][k][j] = fn(m, k, j) on the real pressure levels."""
+    from woof_nam_amd import ozone
+    h = ozone.host_tables()
+    pstr = h["pstr"] if pstr is None else pstr
+    m, k, j = np.meshgrid(np.arange(12), np.arange(pstr.size), np.arange(18), indexing="ij")
+    return {"o3r": fn(m, k, j).astype(np.float32), "pstr": pstr, "blte": h["blte"], "dlte": h["dlte"]}
+
+
+def test_ozone_month_and_latitude_interpolation():
+    """A climatology independent of pressure and linear in latitude (a_m + beta * latitude_j): every
+    layer's ozone is (1 - facoz) a_k1 + facoz a_k2 + beta * latitude, inside the latitude range, and the
+    edge latitude's value outside it."""
+    a = np.linspace(1.0e-6, 2.2e-6, 12)
+    beta = 1.0e-9
+    lat_j = lambda j: -85.0 + 10.0 * j
+    clim = _clim(lambda m, k, j: a[m] + beta * lat_j(j))
+    for lat, k1, k2, f in ((37.3, 3, 4, 0.25), (-62.0, 12, 1, 0.8), (88.0, 6, 7, 0.5), (-89.5, 1, 2, 0.0)):
+        st = state(2, 1, seed=4, lat_deg=lat)
+        oz = {**clim, "k1oz": k1, "k2oz": k2, "facoz": f}
+        cols = pyoracle.nmmb_prep(st, dims(st), (10, 1, 0), ozone=oz)
+        deglat = float(st["glat"][0, 0]) * 57.29578
+        latc = min(max(deglat, -85.0), 85.0)
+        expect = (1.0 - np.float32(f)) * a[k1 - 1] + np.float32(f) * a[k2 - 1] + beta * latc
+        assert np.allclose(cols["o3_lay"], expect, rtol=1e-6, atol=0), lat
+        assert np.allclose(cols["o3_top"], expect, rtol=1e-6, atol=0), lat
+
+
+def test_ozone_is_linear_in_the_exner_function():
+    """A climatology proportional to the Exner function of its own levels, gamma (p / 1e5 Pa)^(R/cp):
+    the layers inside the climatology's pressure range get gamma times their own Exner function (the
+    driver call's (P_PHY / 1e5)^(R/cp)); the layers below its lowest level get the lowest level's value."""
+    from woof_nam_amd import ozone
+    h = ozone.host_tables()
+    rocp = float(np.float32(287.05) / np.float32(1004.6))
+    pk = (h["pstr"].astype(np.float64) * 100.0 / 1.0e5) ** rocp
+    gamma = 3.0e-6
+    clim = _clim(lambda m, k, j: gamma * pk[k] + 0.0 * m + 0.0 * j)
+    st = state(3, 2, seed=6)
+    cols = pyoracle.nmmb_prep(st, dims(st), (10, 1, 0), ozone={**clim, "k1oz": 5, "k2oz": 6, "facoz": 0.3})
+    pphy = (st["sgml2"][:, None, None] * st["pd"][None] + st["psgml1"][:, None, None]).astype(np.float32)
+    wk = ((pphy / np.float32(1000.0)).astype(np.float64) * 0.01) ** float(np.float32(287.04) / np.float32(1004.6))
+    wk = wk.reshape(LM, -1)
+    inside = (pphy.reshape(LM, -1) / 100.0 < h["pstr"][-1]) & (pphy.reshape(LM, -1) / 100.0 > h["pstr"][0])
+    below = pphy.reshape(LM, -1) / 100.0 > h["pstr"][-1] * 1.001
+    assert inside.sum() > 20 and below.sum() > 20
+    assert np.allclose(cols["o3_lay"][inside], gamma * wk[inside], rtol=2e-6)
+    assert np.allclose(cols["o3_lay"][below], np.float32(gamma * pk[-1]), rtol=1e-6)
+
+
+def test_ozone_time_factors():
+    """gas_update's time factors: facoz runs 0 -> 1 between two months' middle days, the months move
+    on at hour 0 of the middle day, and the interpolated value is continuous across that change."""
+    from woof_nam_amd import ozone
+    days = (31, 28, 31, 30, 31, 30, 31, 31, 30, 31, 30, 31)
+    t = ozone.TimeFactors()
+    prev = None
+    for month in range(1, 13):
+        for day in range(1, days[month - 1] + 1):
+            for hour in (0, 12):
+                t.update(month, day, hour)
+                assert 0.0 <= t.facoz < 1.0 and t.k2oz == t.k1oz % 12 + 1
+                mid = days[month - 1] // 2 + 1
+                assert (t.k1oz == month) == (day >= mid)
+                w = np.zeros(13)                               # the months' weights
+                w[t.k1oz] += 1.0 - t.facoz
+                w[t.k2oz] += t.facoz
+                if prev is not None:
+                    assert np.abs(w - prev).max() <= 1.0 / 28 + 1e-6    # at most one day's step
+                prev = w
+                if day == mid:
+                    assert t.facoz == 0.0 and t.k1oz == month     # a month's middle day is its own value

@@ -18,7 +18,7 @@ TABLES = Path(__file__).resolve().parent / "data" / "rrtmg_tables.bin"
 EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", "wr_run_mixed", "wr_run_mixed_noalloc", "wr_run_band", "wr_cal_cldfra",
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
-           "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate", "wr_nmmb_prep", "wr_nmmb_post",
+           "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate", "wr_nmmb_prep", "wr_nmmb_post", "wr_nmmb_albedo",
            "wr_sol_update", "wr_coszmn", "wr_rdtemp")
 
 _ERRORS = {
@@ -91,6 +91,8 @@ def load():
     L.wr_nmmb_post.argtypes = [C.POINTER(abi.WrNmmbState), C.POINTER(abi.WrNmmbCols), C.POINTER(abi.WrOutputs), vp,
                                C.c_int32, C.c_int32, C.POINTER(abi.WrNmmbOut), vp]
     L.wr_nmmb_post.restype = C.c_int
+    L.wr_nmmb_albedo.argtypes = [C.POINTER(abi.WrNmmbState), vp, vp, vp, vp, vp]
+    L.wr_nmmb_albedo.restype = C.c_int
     i32, f64 = C.c_int32, C.c_double
     L.wr_sol_update.argtypes = [i32] * 6 + [f64, f64, f64, C.POINTER(abi.WrSol)]
     L.wr_sol_update.restype = C.c_int

@@ -23,7 +23,8 @@ class WrColumns(C.Structure):
                 ("col_id", C.POINTER(C.c_uint64)), ("seed_dev", C.POINTER(C.c_uint64)),
                 ("ncol_dev", C.POINTER(C.c_int32)),
                 ("cwp", _dp), ("cip", _dp), ("crp", _dp), ("rew", _dp), ("rei", _dp),
-                ("aer_sw", C.POINTER(C.c_float)), ("aer_lw", C.POINTER(C.c_float)), ("aer_ld", C.c_int64)]
+                ("aer_sw", C.POINTER(C.c_float)), ("aer_lw", C.POINTER(C.c_float)), ("aer_ld", C.c_int64),
+                ("o3_top", _dp)]
 
 
 # aerosol optics inputs: [3][bands][nlay+1][ncol] float32 (optical depth, single-scattering albedo,
@@ -76,15 +77,22 @@ class WrNmmbState(C.Structure):
                                                           "cw", "f_ice", "f_rain", "glat")]
 
 
+class WrOzone(C.Structure):
+    _fields_ = [("o3r", C.POINTER(C.c_float)), ("pstr", C.POINTER(C.c_float)), ("loz", C.c_int32), ("jmr", C.c_int32),
+                ("blte", C.c_float), ("dlte", C.c_float), ("k1oz", C.c_int32), ("k2oz", C.c_int32),
+                ("facoz", C.c_float)]
+
+
 class WrNmmbOpts(C.Structure):
-    _fields_ = [("llyr", C.c_int32), ("iovr", C.c_int32), ("subgrid", C.c_int32)]
+    _fields_ = [("llyr", C.c_int32), ("iovr", C.c_int32), ("subgrid", C.c_int32), ("ozone", C.POINTER(WrOzone))]
 
 
 NMMB_COL_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "cwp", "cip", "crp", "rew", "rei")
 
 
 class WrNmmbCols(C.Structure):
-    _fields_ = [(n, _dp) for n in NMMB_COL_FIELDS] + [("clds", _fp), ("mtop", _ip), ("mbot", _ip), ("p_phy", _fp)]
+    _fields_ = [(n, _dp) for n in NMMB_COL_FIELDS] + [("clds", _fp), ("mtop", _ip), ("mbot", _ip), ("p_phy", _fp),
+                                                       ("o3_top", _dp)]
 
 
 NMMB_OUT_FIELDS = ("rlwtt", "rswtt", "cldfra", "rlwin", "rlwtoa", "sigt4", "rswin", "rswout", "rswinc", "rswtoa",

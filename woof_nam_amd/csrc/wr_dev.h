@@ -98,16 +98,20 @@ template <typename R> struct Tabs {
 };
 
 // ---- column inputs ---------------------------------------------------------------------------
+// Every member has a default (null, 0), so a ColIn made any way holds no indeterminate pointer that a
+// kernel would test and follow.
 struct ColIn {
-    const WR_G double *p_lev, *t_lay, *q_lay, *o3_lay, *cldf, *qc, *qi, *t_sfc, *emis, *alb, *cosz;
-    const WR_G uint64_t *col_id;
-    const WR_G uint64_t *seed_dev;  // optional device copy of the McICA seed
-    const WR_G int32_t *ncol_dev;   // optional device count of valid columns (a shorter replayed tile)
-    const WR_G double *cwp, *cip, *crp, *rew, *rei;  // optional path form of the clouds (cwp set)
-    const WR_G float *aer_sw, *aer_lw;               // optional aerosol optics [3][bands][nlay+1][aer_ld]
-    long aer_ld;
-    int ncol;  // stride of the [k][column] fields
-    int nlay;  // model layers
+    const WR_G double *p_lev = nullptr, *t_lay = nullptr, *q_lay = nullptr, *o3_lay = nullptr, *cldf = nullptr;
+    const WR_G double *qc = nullptr, *qi = nullptr, *t_sfc = nullptr, *emis = nullptr, *alb = nullptr, *cosz = nullptr;
+    const WR_G uint64_t *col_id = nullptr;
+    const WR_G uint64_t *seed_dev = nullptr;  // optional device copy of the McICA seed
+    const WR_G int32_t *ncol_dev = nullptr;   // optional device count of valid columns (a shorter replayed tile)
+    const WR_G double *cwp = nullptr, *cip = nullptr, *crp = nullptr, *rew = nullptr, *rei = nullptr;  // path form
+    const WR_G float *aer_sw = nullptr, *aer_lw = nullptr;  // optional aerosol optics [3][bands][nlay+1][aer_ld]
+    long aer_ld = 0;
+    const WR_G double *o3_top = nullptr;                    // optional ozone of the added top layer
+    int ncol = 0;  // stride of the [k][column] fields
+    int nlay = 0;  // model layers
 };
 
 struct RunPar {
@@ -154,7 +158,11 @@ struct RadCol {
     }
     __device__ __forceinline__ double tlyr(int l) const { return fld(in.t_lay, l == 0 ? 0 : l - 1); }
     __device__ __forceinline__ double qlyr(int l) const { return fmax(1.0e-7, fld(in.q_lay, l == 0 ? 0 : l - 1)); }
-    __device__ __forceinline__ double olyr(int l) const { return fmax(1.0e-10, fld(in.o3_lay, l == 0 ? 0 : l - 1)); }
+    __device__ __forceinline__ double olyr(int l) const
+    {
+        if (l == 0 && in.o3_top) return fmax(1.0e-10, in.o3_top[i]);
+        return fmax(1.0e-10, fld(in.o3_lay, l == 0 ? 0 : l - 1));
+    }
     __device__ __forceinline__ double tskn() const { return in.t_sfc[i]; }
     // level temperature by log-pressure interpolation; la(l) = log(plyr(l)), lb(l) = log(plvl(l))
     __device__ __forceinline__ double tlvl(int l, double la_l, double la_lm1, double lb_l) const

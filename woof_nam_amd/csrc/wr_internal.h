@@ -68,6 +68,23 @@ inline void tend(wr_ctx *c, hipStream_t s)
 }
 
 namespace wr {
+// The device view of a call's column inputs.  Every ColIn is made here: value-initialised, then every
+// field of wr_columns copied, so that no field a kernel tests (the optional path form, aerosol, top
+// ozone) is ever left indeterminate.
+inline ColIn to_colin(const wr_columns *in)
+{
+    ColIn ci{};
+    ci.p_lev = gptr(in->p_lev), ci.t_lay = gptr(in->t_lay), ci.q_lay = gptr(in->q_lay), ci.o3_lay = gptr(in->o3_lay);
+    ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
+    ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
+    ci.seed_dev = gptr(in->seed_dev), ci.ncol_dev = gptr(in->ncol_dev);
+    ci.cwp = gptr(in->cwp), ci.cip = gptr(in->cip), ci.crp = gptr(in->crp), ci.rew = gptr(in->rew), ci.rei = gptr(in->rei);
+    ci.aer_sw = gptr(in->aer_sw), ci.aer_lw = gptr(in->aer_lw);
+    ci.aer_ld = in->aer_ld > 0 ? (long)in->aer_ld : (long)in->ncol;
+    ci.o3_top = gptr(in->o3_top);
+    ci.ncol = in->ncol, ci.nlay = in->nlay;
+    return ci;
+}
 inline ColScal colscal(wr_ctx *c)
 {
     const size_t m = (size_t)c->max_cols;

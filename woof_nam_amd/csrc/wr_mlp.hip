@@ -346,12 +346,7 @@ int wr_layer_optics(wr_ctx *c, const wr_columns *in, const wr_params *p, float *
     if (in->ncol == 0) return WR_OK;
     if (!in->p_lev || !in->t_lay || !in->q_lay || !in->o3_lay || !in->cldf || !in->qc || !in->qi || !in->t_sfc)
         return WR_ERR_ARG;
-    ColIn ci;
-    ci.p_lev = gptr(in->p_lev), ci.t_lay = gptr(in->t_lay), ci.q_lay = gptr(in->q_lay), ci.o3_lay = gptr(in->o3_lay);
-    ci.cldf = gptr(in->cldf), ci.qc = gptr(in->qc), ci.qi = gptr(in->qi), ci.t_sfc = gptr(in->t_sfc);
-    ci.emis = gptr(in->emis), ci.alb = gptr(in->alb), ci.cosz = gptr(in->cosz), ci.col_id = gptr(in->col_id);
-    ci.seed_dev = gptr(in->seed_dev), ci.ncol_dev = gptr(in->ncol_dev);
-    ci.ncol = in->ncol, ci.nlay = in->nlay;
+    const ColIn ci = to_colin(in);
     RunPar par;
     par.mcica = p->mcica, par.iovr = p->iovr, par.seed = p->seed;
     par.solcon = p->solcon, par.re_liq = p->re_liq, par.re_ice = p->re_ice;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include "../../include/woofrad.h"
+#include "wr_albd.h"
 #include "wr_dev.h"
 
 namespace {
@@ -63,9 +64,50 @@ struct NCols {
     WR_G float *clds;
     WR_G int32_t *mtop, *mbot;
     WR_G float *p_phy;
+    WR_G double *o3_top;
 };
 
-__global__ void __launch_bounds__(64) k_nmmb_prep(NState st, wr_nmmb_opts o, NCols w)
+// climatological ozone (wr_ozone with global-address-space pointers); on = 0: the state's O3
+constexpr int OZ_LOZ_MAX = 32;
+struct NOz {
+    const WR_G float *o3r;   // [12][loz][jmr]
+    const WR_G float *pstr;  // [loz] hPa
+    int on, loz, jmr, k1oz, k2oz;
+    double blte, dlte, facoz;
+};
+
+// the Exner function (p / 1e5 Pa)^(R/cp) as funcphys' fpkap table gives it: linear between the
+// table's nodes every 10 Pa (phys_rrtm_gas_init.cu:52-72), the nodes evaluated here
+__device__ double fpkap(double p)
+{
+#pragma clang fp contract(off)
+    const double rocp = (double)(287.05f / 1004.6f);  // con_rd / con_cp, formed in single precision
+    const double xinc = 110000.0 / 11000.0;
+    const double xj = fmin(fmax(1.0 + p / xinc, 1.0), 11001.0);
+    const int jx = (int)fmin(xj, 11000.0);
+    const double t0 = pow_d((double)(jx - 1) * xinc / 1.0e5, rocp);
+    const double t1 = pow_d((double)jx * xinc / 1.0e5, rocp);
+    return t0 + (xj - (double)jx) * (t1 - t0);
+}
+
+// getozn (phys_rrtm_getozn.cu:57-110) at one layer: the climatology column o3i at the climatology's
+// Exner levels pk, interpolated linearly in the Exner function wk; the end values outside
+__device__ double ozone_at(const double *o3i, const double *pk, int loz, double wk)
+{
+#pragma clang fp contract(off)
+    if (wk > pk[loz - 1]) return o3i[loz - 1];
+    if (wk <= pk[0]) return o3i[0];
+    double v = o3i[0];
+    for (int k = 0; k < loz - 1; ++k) {
+        if (wk > pk[k] && wk <= pk[k + 1]) {
+            const double tem = (pk[k + 1] - wk) * (1.0 / (pk[k + 1] - pk[k]));
+            v = tem * o3i[k] + (1.0 - tem) * o3i[k + 1];
+        }
+    }
+    return v;
+}
+
+__global__ void __launch_bounds__(64) k_nmmb_prep(NState st, wr_nmmb_opts o, NCols w, NOz oz)
 {
 #pragma clang fp contract(off)
     const int nc = (st.j1 - st.j0) * st.im;
@@ -83,6 +125,37 @@ __global__ void __launch_bounds__(64) k_nmmb_prep(NState st, wr_nmmb_opts o, NCo
         w.p_lev[(size_t)(k + 1) * snc + c] = (double)p8;
         if (w.p_phy) w.p_phy[st.s3(k, j, i)] = st.sgml2[k] * pd + st.psgml1[k];
     }
+    // climatological ozone (getozn): this latitude's column of the climatology, between the two months
+    double o3i[OZ_LOZ_MAX], pk[OZ_LOZ_MAX];
+    bool oz_on = oz.on != 0;
+    if (oz_on) {
+        const double deglat = (double)st.glat[st.s2(j, i)] * 57.29578;
+        const double elte = oz.blte + (double)(oz.jmr - 1) * oz.dlte;
+        int j1, j2;
+        double tem1;
+        if (deglat > oz.blte && deglat < elte) {
+            tem1 = (deglat - oz.blte) / oz.dlte + 1.0;
+            j1 = (int)tem1;
+            j2 = j1 + 1;
+            tem1 = tem1 - (double)j1;
+        } else if (deglat <= oz.blte) {
+            j1 = j2 = 1, tem1 = 1.0;
+        } else if (deglat >= elte) {
+            j1 = j2 = oz.jmr, tem1 = 1.0;
+        } else {
+            j1 = j2 = 1, tem1 = 1.0, oz_on = false;  // latitude NaN: the state's O3
+        }
+        const double tem2 = 1.0 - tem1;
+        const double rocp = (double)(287.05f / 1004.6f);
+        for (int k = 0; k < oz.loz; ++k) {
+            const size_t m1 = ((size_t)(oz.k1oz - 1) * oz.loz + k) * oz.jmr, m2 = ((size_t)(oz.k2oz - 1) * oz.loz + k) * oz.jmr;
+            const double tem3 = tem2 * (double)oz.o3r[m1 + j1 - 1] + tem1 * (double)oz.o3r[m1 + j2 - 1];
+            const double tem4 = tem2 * (double)oz.o3r[m2 + j1 - 1] + tem1 * (double)oz.o3r[m2 + j2 - 1];
+            o3i[k] = tem4 * oz.facoz + tem3 * (1.0 - oz.facoz);
+            const double ps = (double)oz.pstr[k];
+            pk[k] = ps >= 0.1 ? fpkap(ps * 100.0) : pow_d(ps / 1000.0, rocp);
+        }
+    }
     // layer pressures (hPa) as the column call forms them from the interfaces; layer 0 is the added
     // layer above the model top.  ct keeps the layer cloud fractions for gethml.
     double ct[N_NL];
@@ -92,6 +165,9 @@ __global__ void __launch_bounds__(64) k_nmmb_prep(NState st, wr_nmmb_opts o, NCo
         if (pl1 <= 1.0e-6) pl1 = 2.0e-6;
         plyr_top = 0.5 * pl1;
     }
+    if (w.o3_top)  // the added top layer: getozn at its Exner function (grrad_setup's fpkapx), or none
+        w.o3_top[c] = oz_on ? ozone_at(o3i, pk, oz.loz, pow_d(plyr_top * 100.0 / 1.0e5, (double)(287.05f / 1004.6f)))
+                            : (double)st.o3[st.s3(0, j, i)];
     ct[0] = 0.0;
     const double recw1 = 620.35 / pow_d(200.0, 1.0 / 3.0);
     double pb = w.p_lev[c];
@@ -103,7 +179,13 @@ __global__ void __launch_bounds__(64) k_nmmb_prep(NState st, wr_nmmb_opts o, NCo
         const double t = (double)st.t[s], q = (double)st.q[s];
         w.t_lay[kc] = t;
         w.q_lay[kc] = q;
-        w.o3_lay[kc] = (double)st.o3[s];
+        if (oz_on) {  // the model layer's Exner function as the driver call forms it (PRSLK)
+            const float prsl = (st.sgml2[k] * st.pd[st.s2(j, i)] + st.psgml1[k]) / 1000.0f;
+            const double wk = pow_d((double)prsl * 0.01, (double)(287.04f / 1004.6f));
+            w.o3_lay[kc] = ozone_at(o3i, pk, oz.loz, wk);
+        } else {
+            w.o3_lay[kc] = (double)st.o3[s];
+        }
         double cw = (double)st.cw[s];
         double fice = (double)st.f_ice[s], frain = (double)st.f_rain[s];
         // driver call SUBGRID (phys_rrtm_driver_call.cu:248-275)
@@ -332,6 +414,47 @@ __global__ void __launch_bounds__(64) k_nmmb_post(NState st, NPostIn pin, NRad r
     o.ncfrcv[s2] = o.ncfrcv[s2] + 1;
 }
 
+// Payne (1972) open-water albedo at the NAM's atmospheric transmission 0.537 (the IALB = 2 block of the
+// NMMB radiation driver, phys_rrtm_grrad_clouds.cu:363-385): linear in the zenith angle between the
+// table's nodes, then in the transmission between its 0.50 and 0.55 columns
+__constant__ double c_albd_za[20] = WR_ALBD_ZA;
+__constant__ double c_albd_dza[19] = WR_ALBD_DZA;
+__constant__ double c_albd_lo[20] = WR_ALBD_LO;
+__constant__ double c_albd_hi[20] = WR_ALBD_HI;
+
+struct NAlb {  // kernel parameters are structs or plain values (see NPostIn)
+    const WR_G float *albedo, *sm;  // [jm][im]
+    const WR_G double *coszen;      // [ncol]
+    WR_G double *alb;               // [ncol]
+    int im, j0, nc;
+};
+
+__global__ void __launch_bounds__(256) k_nmmb_albedo(NAlb g)
+{
+#pragma clang fp contract(off)
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= g.nc) return;
+    const size_t p = (size_t)(g.j0 + c / g.im) * g.im + c % g.im;
+    double a = (double)g.albedo[p];
+    const double cz = g.coszen[c];
+    if (cz > 0.0 && (double)g.sm[p] > 0.5) {
+        const double zen = (180.0 / 3.14159265) * acos(fmax(cz, 0.0));
+        int jx;
+        if (zen >= 74.0)
+            jx = (int)(0.5 * (90.0 - zen) + 1.0);
+        else if (zen >= 50.0)
+            jx = (int)(0.25 * (74.0 - zen) + 9.0);
+        else
+            jx = (int)(0.1 * (50.0 - zen) + 15.0);
+        jx = imin(19, imax(1, jx));  // zenith 0 exactly would index past the table's last node
+        const double dzen = -(zen - c_albd_za[jx - 1]) / c_albd_dza[jx - 1];
+        const double a1 = c_albd_lo[jx - 1] + dzen * (c_albd_lo[jx] - c_albd_lo[jx - 1]);
+        const double a2 = c_albd_hi[jx - 1] + dzen * (c_albd_hi[jx] - c_albd_hi[jx - 1]);
+        a = a1 + 20.0 * (a2 - a1) * (0.537 - WR_ALBD_TRN_LO);
+    }
+    g.alb[c] = a;
+}
+
 int check_state(const wr_nmmb_state *st)
 {
     if (!st || st->im <= 0 || st->jm <= 0 || st->lm < 2 || st->lm + 1 > N_NL) return WR_ERR_ARG;
@@ -370,8 +493,34 @@ int wr_nmmb_prep(const wr_nmmb_state *st, const wr_nmmb_opts *opt, const wr_nmmb
     c.cldf = gptr(w->cldf), c.cwp = gptr(w->cwp), c.cip = gptr(w->cip), c.crp = gptr(w->crp), c.rew = gptr(w->rew);
     c.rei = gptr(w->rei), c.clds = gptr(w->clds), c.mtop = gptr(w->mtop), c.mbot = gptr(w->mbot);
     c.p_phy = gptr(w->p_phy);
+    c.o3_top = gptr(w->o3_top);
+    NOz oz{};
+    if (opt->ozone) {
+        const wr_ozone *z = opt->ozone;
+        if (!z->o3r || !z->pstr || z->loz < 2 || z->loz > OZ_LOZ_MAX || z->jmr < 1 || !(z->dlte > 0.0f) ||
+            z->k1oz < 1 || z->k1oz > 12 || z->k2oz < 1 || z->k2oz > 12 || !w->o3_top)
+            return WR_ERR_ARG;
+        oz.o3r = gptr(z->o3r), oz.pstr = gptr(z->pstr), oz.on = 1, oz.loz = z->loz, oz.jmr = z->jmr;
+        oz.k1oz = z->k1oz, oz.k2oz = z->k2oz, oz.blte = z->blte, oz.dlte = z->dlte, oz.facoz = z->facoz;
+    }
+    wr_nmmb_opts ob = *opt;
+    ob.ozone = nullptr;  // the kernel takes the climatology through oz
     const int nc = (st->j1 - st->j0) * st->im;
-    hipLaunchKernelGGL(k_nmmb_prep, dim3((nc + 63) / 64), dim3(64), 0, (hipStream_t)stream, to_state(st), *opt, c);
+    hipLaunchKernelGGL(k_nmmb_prep, dim3((nc + 63) / 64), dim3(64), 0, (hipStream_t)stream, to_state(st), ob, c, oz);
+    return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
+}
+
+int wr_nmmb_albedo(const wr_nmmb_state *st, const float *albedo, const float *sm, const double *coszen, double *alb,
+                   void *stream)
+{
+    if (!st || st->im <= 0 || st->jm <= 0 || st->j0 < 0 || st->j1 > st->jm || st->j0 >= st->j1) return WR_ERR_ARG;
+    if ((int64_t)(st->j1 - st->j0) * st->im > 0x7fffffff) return WR_ERR_ARG;
+    if (!albedo || !sm || !coszen || !alb) return WR_ERR_ARG;
+    const int nc = (st->j1 - st->j0) * st->im;
+    NAlb g;
+    g.albedo = gptr(albedo), g.sm = gptr(sm), g.coszen = gptr(coszen), g.alb = gptr(alb);
+    g.im = st->im, g.j0 = st->j0, g.nc = nc;
+    hipLaunchKernelGGL(k_nmmb_albedo, dim3((nc + 255) / 256), dim3(256), 0, (hipStream_t)stream, g);
     return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }
 

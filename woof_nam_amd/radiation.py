@@ -68,7 +68,15 @@ def _colstruct(cols, ncol, nlay, device) -> abi.WrColumns:
                          C.cast(C.c_void_p(ids.data_ptr()), u64) if ids is not None else None,
                          C.cast(C.c_void_p(sd.data_ptr()), u64) if sd is not None else None,
                          C.cast(C.c_void_p(nd.data_ptr()), C.POINTER(C.c_int32)) if nd is not None else None,
-                         *[_dptr(cols.get(k)) for k in abi.PATH_FIELDS], aer.get("aer_sw"), aer.get("aer_lw"), ld)
+                         *[_dptr(cols.get(k)) for k in abi.PATH_FIELDS], aer.get("aer_sw"), aer.get("aer_lw"), ld,
+                         _dptr(_o3_top(cols, ncol, device)))
+
+
+def _o3_top(cols, ncol, device):
+    a = cols.get("o3_top")
+    if a is not None:
+        _check_field("o3_top", a, (ncol,), torch.float64, device)
+    return a
 
 
 def _aer_ptr(name, a, ncol, nlay, device):

@@ -9,6 +9,8 @@ returns None.  The input species arrays are only read.
 
 What it does on a call, all of it on the device in library kernels:
   * cadence: SW on NTIMESTEP % NRADS == 0, LW on NTIMESTEP % NRADL == 0 (rrtm_call.py:253-254);
+  * climatological ozone, the NAM's (ntoz = 0): gas_init's monthly zonal means with gas_update's time
+    factors, interpolated by getozn inside wr_nmmb_prep (ctx.cfg.rrtm_ozone = "state" takes O3 instead);
   * wr_nmmb_prep on the interior rows J_S..J_E (phys_rad_driver.py:165-166): driver prep (P8W,
     P_PHY, phys_rrtm_driver_prep.cu:82), the driver call's condensate and subgrid cloud
     (phys_rrtm_driver_call.cu:200-275), grrad setup, and progcld2 with gethml
@@ -19,6 +21,8 @@ What it does on a call, all of it on the device in library kernels:
     over the sunlit instants of the SW interval (coszmn), on the library (woof_nam_amd.solar);
   * aerosol optics from the model's aerosol step when it left them in ctx.tables (the reference's
     rrtm_faersw / rrtm_faerlw, setaer's whole-grid layout), passed as column slices;
+  * the surface albedo (wr_nmmb_albedo: setalb with IALB = 2, the Payne open-water table over sea in
+    daylight, the state's ALBEDO elsewhere; emissivity 1, setemis with IEMS = 0, unless EMISS is given);
   * the radiation call on the path-form columns (woof_nam_amd.radiation, the NAM route mcica = 0;
     FP64 unless ctx.cfg.rrtm_precision = "mixed"), in tiles sized from free device memory;
   * wr_nmmb_post (phys_rrtm_driver_post.cu:71-150): RLWTT, RSWTT (K/s), RLWIN, RSWIN, RSWOUT,
@@ -32,7 +36,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import _lib, abi, solar
+from . import _lib, abi, ozone, solar
 from .radiation import Radiation
 
 KEY_CTX = "woof_nam_amd.radiation"
@@ -174,26 +178,43 @@ def nmmb_state(args, device, rows):
     return st, keep, (im, jm, lm)
 
 
-def prep(args, device, rows, llyr, iovr=1, subgrid=True, stream=None):
+def prep(args, device, rows, llyr, iovr=1, subgrid=True, stream=None, ozone=None):
     """wr_nmmb_prep: the path-form columns of the radiated rows (float64 device tensors) with gethml's
-    clds [5][ncol], mtop / mbot [3][ncol]; P_PHY is written when args holds it."""
+    clds [5][ncol], mtop / mbot [3][ncol] and the added top layer's ozone o3_top [ncol]; P_PHY is
+    written when args holds it.  ozone: an abi.WrOzone (climatological ozone) or None (the state's O3)."""
     st, keep, (im, jm, lm) = nmmb_state(args, device, rows)
     ncol = (rows[1] - rows[0]) * im
     cols = {k: torch.empty(s, dtype=torch.float64, device=device) for k, s in abi.nmmb_col_shapes(ncol, lm).items()}
     cols["clds"] = torch.empty((5, ncol), dtype=torch.float32, device=device)
     cols["mtop"] = torch.empty((3, ncol), dtype=torch.int32, device=device)
     cols["mbot"] = torch.empty((3, ncol), dtype=torch.int32, device=device)
+    cols["o3_top"] = torch.empty(ncol, dtype=torch.float64, device=device)
     pphy = args.get("P_PHY")
     if pphy is not None:
         pphy = _arr(args, "P_PHY", (lm, jm, im), torch.float32)
     cst = abi.WrNmmbCols(*[_dptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _fptr(cols["clds"]), _iptr(cols["mtop"]),
-                         _iptr(cols["mbot"]), _fptr(pphy))
+                         _iptr(cols["mbot"]), _fptr(pphy), _dptr(cols["o3_top"]))
     s = stream or torch.cuda.current_stream(device)
-    rc = _lib.load().wr_nmmb_prep(C.byref(st), C.byref(abi.WrNmmbOpts(int(llyr), int(iovr), int(bool(subgrid)))),
-                                  C.byref(cst), C.c_void_p(s.cuda_stream))
+    opts = abi.WrNmmbOpts(int(llyr), int(iovr), int(bool(subgrid)), C.pointer(ozone) if ozone is not None else None)
+    rc = _lib.load().wr_nmmb_prep(C.byref(st), C.byref(opts), C.byref(cst), C.c_void_p(s.cuda_stream))
     _lib.check(rc, "wr_nmmb_prep")
     cols["_keep"] = keep
     return cols, (im, jm, lm)
+
+
+def albedo(args, device, rows, coszen, stream=None):
+    """wr_nmmb_albedo: the NAM surface albedo of the radiated rows (ALBEDO; over open water in daylight the
+    Payne open-water table by zenith angle), float64 [ncol] on the device."""
+    st, keep, (im, jm, lm) = nmmb_state(args, device, rows)
+    a = _arr(args, "ALBEDO", (jm, im), torch.float32)
+    sm = _arr(args, "SM", (jm, im), torch.float32)
+    cz = coszen.to(device=device, dtype=torch.float64).contiguous()
+    out = torch.empty((rows[1] - rows[0]) * im, dtype=torch.float64, device=device)
+    s = stream or torch.cuda.current_stream(device)
+    rc = _lib.load().wr_nmmb_albedo(C.byref(st), C.c_void_p(a.data_ptr()), C.c_void_p(sm.data_ptr()),
+                                    C.c_void_p(cz.data_ptr()), C.c_void_p(out.data_ptr()), C.c_void_p(s.cuda_stream))
+    _lib.check(rc, "wr_nmmb_albedo")
+    return out
 
 
 def post(args, device, rows, cols, out, czmean, lw_step, sw_step, stream=None):
@@ -202,7 +223,7 @@ def post(args, device, rows, cols, out, czmean, lw_step, sw_step, stream=None):
     o = {k: _arr(args, k, (lm, jm, im), torch.float32) for k in OUT_3D}
     o.update({k: _arr(args, k, (jm, im), torch.float32) for k in OUT_2D})
     o.update({k: _arr(args, k, (jm, im), torch.int32) for k in ("NCFRST", "NCFRCV")})
-    cst = abi.WrNmmbCols(*[_dptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _fptr(cols["clds"]), None, None, None)
+    cst = abi.WrNmmbCols(*[_dptr(cols[k]) for k in abi.NMMB_COL_FIELDS], _fptr(cols["clds"]), None, None, None, None)
     ro = abi.WrOutputs(*[_dptr(out[k]) for k in abi.OUTPUT_FIELDS])
     oo = abi.WrNmmbOut(*[_fptr(o[k.upper()]) for k in abi.NMMB_OUT_FIELDS], _iptr(o["NCFRST"]), _iptr(o["NCFRCV"]))
     s = stream or torch.cuda.current_stream(device)
@@ -233,6 +254,54 @@ def aerosols(ctx, lm, jm, im, c0, c1, device) -> dict:
     return out
 
 
+KEY_OZONE = "woof_nam_amd.ozone"
+
+
+def ozone_struct(ctx, args, device, seconds):
+    """The climatological ozone of the step (the NAM's ntoz = 0: getozn over gas_init's climatology with
+    gas_update's time factors at JDAT + seconds), or None when ctx.cfg.rrtm_ozone = "state" (the
+    state's O3)."""
+    if str(_cfg(ctx, "rrtm_ozone", "climatology")) == "state":
+        return None
+    st = ctx.tables.get(KEY_OZONE)
+    if st is None or st[0] is None or st[0].device != device:
+        st = (ozone.Climatology(device), st[1] if st is not None else ozone.TimeFactors())
+        ctx.tables[KEY_OZONE] = st
+    clim, tf = st
+    t = solar.current(args["JDAT"], seconds)
+    tf.update(t.month, t.day, t.hour)
+    return clim.struct(tf)
+
+
+def state_dict(ctx) -> dict:
+    """The hook's own state between calls, for the model's restart sets (namgpu/restart.py:1-12 saves
+    every array reachable from the state; this is what the hook keeps outside it): gas_update's ozone
+    time factors (their saved months and mid-month days carry across the change days) and the
+    boundary-layer cloud top llyr.  The device context and tables are rebuilt on resume.  JSON-able."""
+    out = {"format": 1}
+    if KEY_LLYR in ctx.tables:
+        out["llyr"] = int(ctx.tables[KEY_LLYR])
+    oz = ctx.tables.get(KEY_OZONE)
+    if oz is not None:
+        tf = oz[1]
+        out["ozone_time"] = {"k1oz": tf.k1oz, "k2oz": tf.k2oz, "midm": tf.midm, "midp": tf.midp,
+                             "facoz": float(tf.facoz), "started": bool(tf.started)}
+    return out
+
+
+def load_state_dict(ctx, state: dict) -> None:
+    """Restore what state_dict saved (before the first call of the resumed run)."""
+    if int(state.get("format", 0)) != 1:
+        raise ValueError(f"rrtm-call: unknown state format {state.get('format')!r}")
+    if "llyr" in state:
+        ctx.tables[KEY_LLYR] = int(state["llyr"])
+    t = state.get("ozone_time")
+    if t is not None:
+        tf = ozone.TimeFactors(k1oz=int(t["k1oz"]), k2oz=int(t["k2oz"]), midm=int(t["midm"]), midp=int(t["midp"]),
+                               facoz=np.float32(t["facoz"]), started=bool(t["started"]))
+        ctx.tables[KEY_OZONE] = (None, tf)        # the climatology tables are loaded again on the next call
+
+
 def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     """The RRTM call of one radiation step (see the module docstring); writes args' outputs in place."""
     nts = int(args["NTIMESTEP"])
@@ -259,7 +328,9 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     j0, j1 = int(args.get("J_S", 2)) - 1, int(args.get("J_E", jm - 1))
     if not (0 <= j0 < j1 <= jm):
         raise ValueError(f"rrtm-call: rows J_S..J_E = {j0 + 1}..{j1} outside 1..{jm}")
-    cols, _ = prep(args, device, (j0, j1), _llyr(ctx, args, lm), iovr=1, subgrid=bool(_cfg(ctx, "rrtm_subgrid", True)))
+    oz = ozone_struct(ctx, args, device, nts * dt)
+    cols, _ = prep(args, device, (j0, j1), _llyr(ctx, args, lm), iovr=1, subgrid=bool(_cfg(ctx, "rrtm_subgrid", True)),
+                   ozone=oz)
     c0, c1 = j0 * im, j1 * im
     glat = _arr(args, "GLAT", (jm, im)).reshape(-1)[c0:c1]
     glon = _arr(args, "GLON", (jm, im)).reshape(-1)[c0:c1].to(device)
@@ -268,7 +339,7 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     f64 = dict(dtype=torch.float64, device=device)
     cols["cosz"] = cz if sw_step else torch.full_like(cz, -1.0)
     cols["t_sfc"] = _arr(args, "TSKIN", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous()
-    cols["alb"] = _arr(args, "ALBEDO", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous()
+    cols["alb"] = albedo(args, device, (j0, j1), cz)
     cols["emis"] = (_arr(args, "EMISS", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous() if "EMISS" in args
                     else torch.ones(c1 - c0, **f64))
     cols["col_id"] = torch.arange(c0, c1, dtype=torch.int64, device=device)

@@ -24,6 +24,7 @@ from __future__ import annotations
 import collections
 import json
 import os
+import sys
 import time
 from dataclasses import dataclass, field
 
@@ -134,6 +135,12 @@ class Server:
             filler = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.device) for k, v in cols.items()}
         self.slots = [_Slot(self.device, self.tile, nlay, precision, self.params, graphs, filler)
                       for _ in range(self.nslots)]
+        # the admission budget never exceeds what the device has left once the slots exist, less a
+        # reserve for the runtime (queue scratch, graph and internal allocations): outputs admitted
+        # beyond free memory starve the runtime, which then aborts the process
+        free, total = torch.cuda.mem_get_info(self.device)
+        self.reserve = max(8e9, 0.03 * total)
+        self.budget = min(self.budget, max(0.0, free - self.reserve))
         self.pending = collections.deque()     # submitted, not admitted
         self.tiles = collections.deque()       # (request, c0, n) of admitted requests
         self.inflight_bytes = 0
@@ -210,12 +217,13 @@ def latency_stats(reqs) -> dict:
 
 
 def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed: int = 0,
-                members: int = 8, keep_outputs: bool = False) -> dict:
+                members: int = 8, keep_outputs: bool = False, progress: float = 0.0) -> dict:
     """Drive the server with Poisson arrivals for `duration` seconds (real time).
 
     kinds: name -> (cols dict of device tensors, col_id tensor or None).  An arrival picks a kind
     uniformly; the kind "member" stands for one ensemble member of the parent with its own seed
-    (members seeds 1..members).  Returns throughput and latency figures.
+    (members seeds 1..members).  progress > 0: a line on stderr every `progress` seconds (arrived,
+    finished, columns done).  Returns throughput and latency figures.
     """
     rng = np.random.default_rng(seed)
     names = list(kinds)
@@ -229,8 +237,14 @@ def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed:
     server.t0 = time.perf_counter()
     reqs = []
     i = 0
+    t_report = progress
     while i < len(arrivals) or not server.idle():
         now = server.now()
+        if progress > 0 and now >= t_report:
+            done = [r for r in reqs if r.done is not None]
+            print(f"# served {now:.0f} s: {i}/{len(arrivals)} arrived, {len(done)} finished, "
+                  f"{sum(r.ncol for r in done) / 1e6:.1f} M columns", file=sys.stderr, flush=True)
+            t_report += progress
         while i < len(arrivals) and arrivals[i][0] <= now:
             ta, kind, member = arrivals[i]
             cols, ids = kinds[kind]
