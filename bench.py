@@ -7,7 +7,9 @@ resident in HBM.  Prints ONE JSON line (rank 0).  Also reported beside the throu
   * roofline: the dominant kernel's algorithmic HBM bytes / its mean launch duration (HIP events
     on the launching stream, recorded by the library around every launch);
   * cpu_baseline: the CPU FP64 oracle on the 4,096-column CPU-reference workload, on this host's
-    cores (median of 5 after 1 warm-up), with nproc and the CPU model.
+    cores (median of 5 after 1 warm-up), with nproc and the CPU model;
+  * aerosol: the climatological aerosol optics (wr_aer_optics) on the same columns, ms per call and
+    its HBM roofline (the headline call runs without aerosol, as the configuration states none).
 """
 from __future__ import annotations
 
@@ -180,6 +182,47 @@ def served_block(d, dev, ncol, seconds):
     return r
 
 
+def aerosol_block(d, dev, nx, ny, steps):
+    """wr_aer_optics (the OPAC aerosol climatology of January) on the grid's columns: ms per call from
+    HIP events on the launching stream, and its HBM roofline.  Algorithmic bytes per column: p_lev (61),
+    T and q (60 each) in FP64 and longitude, latitude and surface type in FP32 read; optical depth,
+    single-scattering albedo and asymmetry of 30 bands x 61 layers written in FP32."""
+    import torch
+
+    from woof_nam_amd import aerosol
+    ncol = int(d["t_lay"].shape[1])
+    clim = aerosol.Climatology(dev)
+    clim.set_month(1)
+    f32 = dict(dtype=torch.float32, device=dev)
+    lat = torch.deg2rad(torch.linspace(15.0, 70.0, ny, **f32))[:, None].expand(ny, nx).reshape(-1).contiguous()
+    lon = torch.deg2rad(torch.linspace(-150.0, -50.0, nx, **f32))[None, :].expand(ny, nx).reshape(-1).contiguous()
+    sl = (torch.arange(ncol, device=dev) % 3).to(torch.float32)     # sea, land and sea ice in turn
+    cols = {k: d[k] for k in ("p_lev", "t_lay", "q_lay")}
+    out = aerosol.optics(clim, cols, lat, lon, sl)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream(dev)
+    ms = []
+    for _ in range(max(steps, 3)):
+        b, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        del out
+        b.record(stream)
+        out = aerosol.optics(clim, cols, lat, lon, sl)
+        e.record(stream)
+        torch.cuda.synchronize()
+        ms.append(b.elapsed_time(e))
+    aod = float(out["aer_sw"][0, 9].sum(0).mean())
+    del out
+    torch.cuda.empty_cache()
+    t = float(np.median(ms))
+    per_col = (NL + 1 + 2 * NL) * 8 + 3 * 4 + 3 * 30 * (NL + 1) * 4
+    ach = per_col * ncol / (t * 1e-3) / 1e9
+    return {"ms_per_call": t, "columns_per_s": ncol / (t * 1e-3), "mean_aod_band_0p44_0p63um": aod,
+            "roofline": {"bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": ach / HBM_PEAK_GBS,
+                         "algorithmic_bytes_per_column": per_col},
+            "what": "climatological aerosol optics (setaer + aer_property + radclimaer) for every column, "
+                    "January OPAC tables, median of the calls"}
+
+
 def cpu_info() -> dict:
     """Host cores and CPU model (nproc and /proc/cpuinfo), and the threads this run may use: the
     OMP_NUM_THREADS share when the host sets one, else every core."""
@@ -249,6 +292,7 @@ def main(argv=None):
     ap.add_argument("--served-seconds", type=float, default=3.0,
                     help="seconds of Poisson arrivals (40 req/s) for the served block (0 = skip)")
     ap.add_argument("--no-serial", action="store_true", help="skip the pass with the band kernels on one stream")
+    ap.add_argument("--no-aerosol", action="store_true", help="skip the climatological-aerosol block")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB between two device buffers (reference for HBM byte counters)")
     a = ap.parse_args(argv)
@@ -357,6 +401,10 @@ def main(argv=None):
             rf["valu"] = {"achieved": ach, "peak": peak, "unit": "TFLOP/s", "frac": ach / peak,
                           "flops_per_launch": fl, "from": "rocprofv3 SQ_INSTS_VALU_* counts per column x 64 lanes (an upper bound: masked lanes count)"}
         res["serial"] = {"ms_per_call": serial["call_ms"], "kernel_ms": serial["ms"]}
+
+    if not a.no_aerosol:
+        res["aerosol"] = aerosol_block(d, dev, a.nx, a.ny, a.steps)
+        print(f"# aerosol optics: {res['aerosol']['ms_per_call']:.2f} ms/call", file=sys.stderr, flush=True)
 
     if a.precision == "fp64" and not a.no_mixed:
         # the same workload on the mixed-precision path: throughput, speedup, errors against FP64

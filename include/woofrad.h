@@ -275,6 +275,47 @@ int wr_nmmb_post(const wr_nmmb_state *st, const wr_nmmb_cols *cols, const wr_out
 int wr_nmmb_albedo(const wr_nmmb_state *st, const float *albedo, const float *sm, const double *coszen, double *alb,
                    void *stream);
 
+/* ---- climatological aerosol (the NAM's iaer = 11: the OPAC climatology) --------------------------
+ * wr_aer_optics replaces the setaer call of the radiation (phys_rrtm_grrad_aerosols.py:91-124 ->
+ * phys_rrtm_setaer.py:142-196 with its kernels phys_rrtm_setaer_prep, phys_rrtm_setaer.cu:86-161,
+ * and phys_rrtm_aer_property_column, phys_rrtm_aer_property.cu:104-434), in the NAM's setting
+ * (laswflg, lalwflg, no volcanic aerosol, layers from the top, ivflip = 0).  Per column:
+ *   - the layer thickness and level heights (km) from the hypsometric equation on the column call's
+ *     levels, the added top layer from the top interface to half its pressure, counted twice;
+ *   - the layer relative humidity of grrad setup (phys_rrtm_grrad_setup.cu:156-161, the GFS
+ *     saturation vapour pressure table fpvs), the added top layer taking the one below;
+ *   - the 5-degree climatology bilinearly mapped on the column: species mixing ratios, the number
+ *     densities of the mixing layer and the mineral layer, the nearest profile type with its
+ *     mineral-overlay rules; the vertical domains (mixing layer, mineral transport, free
+ *     troposphere, stratosphere, above) from the surface up by reference sigma, and each layer's
+ *     effective thickness under the domain's exponential profile;
+ *   - per band (14 SW, 16 LW) and layer: optical depth, single-scattering albedo and asymmetry from
+ *     the band-mean species optics (humidity-dependent species interpolated in relative humidity),
+ *     then the optical depth smoothed across domain boundaries (ratios above 30 or below 1/30).
+ * cols: p_lev, t_lay, q_lay of the column call (ncol, nlay <= 63; the layer pressure is the interface
+ * mean, as the column call forms it); xlon, xlat [ncol] radians; slmsk [ncol] the driver call's
+ * surface type (0 sea, 1 land, 2 sea ice); aer_sw [3][14][nlay+1][aer_ld], aer_lw [3][16][nlay+1]
+ * [aer_ld] out (optical depth, single-scattering albedo, asymmetry: the wr_columns.aer_sw / aer_lw
+ * inputs).  FP64 arithmetic on the float32 tables; device pointers. */
+typedef struct {
+    const float *haer, *prsref, *sigref;             /* [7][5] profile type x domain: scale height (km),
+                                                        reference pressure (mb), 0.001 prsref          */
+    const float *rhlev;                              /* [8] relative-humidity levels                    */
+    const float *extrhi, *scarhi, *ssarhi, *asyrhi;  /* [30][6] band x humidity-independent species     */
+    const float *extrhd, *scarhd, *ssarhd, *asyrhd;  /* [30][4][8] band x humidity-dependent species x
+                                                        humidity level                                  */
+    const float *extstra;                            /* [30] stratospheric background extinction        */
+    const float *cmixg;                              /* [37][72][5] latitude (90 N first) x longitude
+                                                        (from 0 E) x component: mixing ratio            */
+    const float *denng;                              /* [37][72][2] number density: mixing layer, mineral */
+    const int32_t *idxcg;                            /* [37][72][5] species of each component (1..10, 0 none) */
+    const int32_t *kprfg;                            /* [37][72] profile type (1..7)                    */
+} wr_aer_tables;
+/* Table rows of the band dimension: 0..13 the SW bands in the column call's order, 14..29 the LW bands. */
+
+int wr_aer_optics(const wr_aer_tables *tab, const wr_columns *cols, const float *xlon, const float *xlat,
+                  const float *slmsk, float *aer_sw, float *aer_lw, int64_t aer_ld, void *stream);
+
 /* ---- solar geometry around the call ---------------------------------------------------------------
  * wr_sol_update: sol_update_nmmb (phys_rrtm_sol_update.cu:16-160): the sun's position for a date
  * (UTC) from its orbit (Kepler's equation by Newton iteration) and the solar constant solc0 / r^2 at

@@ -282,3 +282,76 @@ def cal_cldfra(cldfra, qc, qi, i_lo, i_hi, j_lo, j_hi, f_qc=True, f_qi=True):
     L.oracle_cal_cldfra.argtypes = [C.c_int] * 9 + [fp, fp, fp]
     L.oracle_cal_cldfra(im, jm, lm, i_lo, i_hi, j_lo, j_hi, int(f_qc), int(f_qi), _ptr(cldfra, C.c_float),
                         _ptr(qc, C.c_float), _ptr(qi, C.c_float))
+
+
+def _aer_tables(tab: dict):
+    """wr_aer_tables over host arrays; the arrays it points into are returned with it."""
+    keep, vals = [], []
+    for k in abi.AER_TABLE_FLOAT:
+        a = np.ascontiguousarray(tab[k], dtype=np.float32)
+        keep.append(a)
+        vals.append(_ptr(a, C.c_float))
+    for k in abi.AER_TABLE_INT:
+        a = np.ascontiguousarray(tab[k], dtype=np.int32)
+        keep.append(a)
+        vals.append(_ptr(a, C.c_int32))
+    return abi.WrAerTables(*vals), keep
+
+
+def _aer_cols(p_lev, t_lay, q_lay, xlon, xlat, slmsk):
+    f64 = [np.ascontiguousarray(a, dtype=np.float64) for a in (p_lev, t_lay, q_lay)]
+    f32 = [np.ascontiguousarray(a, dtype=np.float32) for a in (xlon, xlat, slmsk)]
+    nlay, ncol = f64[1].shape
+    if f64[0].shape != (nlay + 1, ncol) or f64[2].shape != (nlay, ncol) or any(a.shape != (ncol,) for a in f32):
+        raise ValueError("aerosol oracle: p_lev [nlay+1][ncol], t_lay/q_lay [nlay][ncol], xlon/xlat/slmsk [ncol]")
+    return f64, f32, nlay, ncol
+
+
+def aer_optics(tab: dict, p_lev, t_lay, q_lay, xlon, xlat, slmsk, L=None):
+    """oracle_aer_optics: (aer_sw float32 [3][14][nlay+1][ncol], aer_lw float32 [3][16][nlay+1][ncol])."""
+    (p, t, q), (lo, la, sm), nlay, ncol = _aer_cols(p_lev, t_lay, q_lay, xlon, xlat, slmsk)
+    tb, keep = _aer_tables(tab)
+    sw = np.zeros((3, abi.NBANDS_SW, nlay + 1, ncol), np.float32)
+    lw = np.zeros((3, abi.NBANDS_LW, nlay + 1, ncol), np.float32)
+    L = L or lib()
+    fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
+    L.oracle_aer_optics.argtypes = [C.POINTER(abi.WrAerTables), C.c_int, C.c_int, dp, dp, dp, fp, fp, fp, fp, fp,
+                                    C.c_int64]
+    if L.oracle_aer_optics(C.byref(tb), ncol, nlay, _ptr(p), _ptr(t), _ptr(q), _ptr(lo, C.c_float),
+                           _ptr(la, C.c_float), _ptr(sm, C.c_float), _ptr(sw, C.c_float), _ptr(lw, C.c_float),
+                           ncol) != 0:
+        raise ValueError("oracle_aer_optics: bad arguments")
+    del keep
+    return sw, lw
+
+
+def aer_column(tab: dict, p_lev, t_lay, q_lay, xlon, xlat, slmsk, c: int, L=None) -> dict:
+    """oracle_aer_column: one column's intermediate fields (dz, hz, rh, delz, rdrh, idom, ih1, cmix, denn,
+    kp)."""
+    (p, t, q), _, nlay, ncol = _aer_cols(p_lev, t_lay, q_lay, xlon, xlat, slmsk)
+    tb, keep = _aer_tables(tab)
+    nrl = nlay + 1
+    o = {k: np.zeros(nrl) for k in ("dz", "rh", "delz", "rdrh")}
+    o["hz"] = np.zeros(nrl + 1)
+    o["idom"], o["ih1"] = np.zeros(nrl, np.int32), np.zeros(nrl, np.int32)
+    o["cmix"], o["denn"] = np.zeros(11), np.zeros(3)
+    L = L or lib()
+    dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+    L.oracle_aer_column.argtypes = [C.POINTER(abi.WrAerTables), C.c_int, C.c_int, dp, dp, dp, C.c_double, C.c_double,
+                                    C.c_double, C.c_int] + [dp] * 5 + [ip, ip, dp, dp]
+    kp = L.oracle_aer_column(C.byref(tb), ncol, nlay, _ptr(p), _ptr(t), _ptr(q), float(np.float32(xlon[c])),
+                             float(np.float32(xlat[c])), float(np.float32(slmsk[c])), int(c),
+                             *[_ptr(o[k]) for k in ("dz", "hz", "rh", "delz", "rdrh")],
+                             _ptr(o["idom"], C.c_int), _ptr(o["ih1"], C.c_int), _ptr(o["cmix"]), _ptr(o["denn"]))
+    if kp < 0:
+        raise ValueError("oracle_aer_column: bad arguments")
+    o["kp"] = kp
+    del keep
+    return o
+
+
+def aer_fpvs(t: float, L=None) -> float:
+    L = L or lib()
+    L.oracle_aer_fpvs.argtypes = [C.c_double]
+    L.oracle_aer_fpvs.restype = C.c_double
+    return L.oracle_aer_fpvs(float(t))

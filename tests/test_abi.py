@@ -29,6 +29,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.WrNmmbOpts) == 3 * 4 + 4 + 8
     assert C.sizeof(abi.WrOzone) == 2 * 8 + 7 * 4 + 4
     assert C.sizeof(abi.WrNmmbCols) == 15 * 8
+    assert C.sizeof(abi.WrAerTables) == 17 * 8
     assert C.sizeof(abi.WrNmmbOut) == 18 * 8
     assert C.sizeof(abi.WrParams) == 4 + 4 + 8 + 3 * 8 + 10 * 8
     assert C.sizeof(abi.WrOutputs) == 7 * 8
@@ -52,3 +53,20 @@ def test_null_arguments_rejected_without_gpu():
     assert L.wr_run_fp64(None, None, None, None, None, None) == abi.WR_ERR_ARG
     assert L.wr_timing(None, 1) == abi.WR_ERR_ARG
     assert L.wr_mcica_masks(-1, 61, None, None, 0, 1, None, None, None) == abi.WR_ERR_ARG
+
+
+def test_aer_optics_rejects_bad_arguments_without_gpu():
+    L = _lib.load()
+    t, cols = abi.WrAerTables(), abi.WrColumns()
+    assert L.wr_aer_optics(None, None, None, None, None, None, None, 0, None) == abi.WR_ERR_ARG
+    # missing tables, then 64 radiation layers (63 model layers is the most)
+    assert L.wr_aer_optics(C.byref(t), C.byref(cols), 8, 8, 8, 8, 8, 1, None) == abi.WR_ERR_ARG
+    f = C.cast(C.c_void_p(8), C.POINTER(C.c_float))
+    t = abi.WrAerTables(*([f] * len(abi.AER_TABLE_FLOAT)), *([C.cast(C.c_void_p(8), C.POINTER(C.c_int32))] * 2))
+    d = C.cast(C.c_void_p(8), C.POINTER(C.c_double))
+    cols.ncol, cols.nlay, cols.p_lev, cols.t_lay, cols.q_lay = 1, 64, d, d, d
+    assert L.wr_aer_optics(C.byref(t), C.byref(cols), 8, 8, 8, 8, 8, 1, None) == abi.WR_ERR_ARG
+    cols.nlay = 60
+    assert L.wr_aer_optics(C.byref(t), C.byref(cols), 8, 8, 8, 8, 8, 0, None) == abi.WR_ERR_ARG   # aer_ld < ncol
+    cols.ncol = 0
+    assert L.wr_aer_optics(C.byref(t), C.byref(cols), 8, 8, 8, 8, 8, 0, None) == abi.WR_OK       # nothing to do

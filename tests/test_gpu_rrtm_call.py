@@ -14,7 +14,7 @@ import pytest
 import torch
 
 from oracle import pyoracle
-from woof_nam_amd import abi, gen, ozone, rad_driver, rrtm_call, solar
+from woof_nam_amd import abi, aerosol, gen, ozone, rad_driver, rrtm_call, solar
 
 pytestmark = pytest.mark.gpu
 
@@ -149,14 +149,17 @@ def setaer_grid(seed):
     return out
 
 
-@pytest.mark.parametrize("with_aerosol", [False, True])
-def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
-    """rad_driver.run -> rrtm_call.run against the oracle chain on the same inputs (with the model's
-    aerosol optics in ctx.tables, or without)."""
-    print(f"case rrtm_call hook vs oracle chain aerosol={with_aerosol}", flush=True)
+@pytest.mark.parametrize("aerosol_mode", ["none", "tables", "climatology"])
+def test_hook_matches_oracle_chain(gpu_device, aerosol_mode):
+    """rad_driver.run -> rrtm_call.run against the oracle chain on the same inputs: without aerosol, with
+    setaer's optics left in ctx.tables, and with the OPAC climatology (the default)."""
+    print(f"case rrtm_call hook vs oracle chain aerosol={aerosol_mode}", flush=True)
     h = ferrier_state(3)
+    if aerosol_mode == "climatology":
+        h["SICE"] = (np.random.default_rng(11).uniform(size=(JM, IM)) < 0.2).astype(np.float32)
     ctx = make_ctx(gpu_device, h)
-    aer = setaer_grid(9) if with_aerosol else {}
+    ctx.cfg.rrtm_aerosol = aerosol_mode
+    aer = setaer_grid(9) if aerosol_mode == "tables" else {}
     for key, a in aer.items():
         ctx.tables[key] = torch.from_numpy(a).to(gpu_device)
     rad_driver.run(ctx, rrtm_call=rrtm_call.run)
@@ -166,12 +169,12 @@ def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
     d = (IM, JM, LM, 1, JM - 1, np.float32(gen.PT))
     args = dev_args(h, gpu_device)
     llyr = rrtm_call.boundary_layer_top(rrtm_call.sigma_interfaces(args, LM))
-    t = solar.current(ctx.tables["JDAT"], 3 * 60.0)
+    t = solar.current(ctx.tables["JDAT"], 0.0)        # JDAT is the time of the step
     cols = pyoracle.nmmb_prep(oracle_state(h), d, (llyr, 1, 1), ozone=ozone_case(t.month, t.day, t.hour)[0])
     c0, c1 = IM, (JM - 1) * IM
     glat = args["GLAT"].reshape(-1)[c0:c1]
     glon = args["GLON"].reshape(-1)[c0:c1]
-    cz = solar.coszen(ctx.tables["JDAT"], 3 * 60.0, 3, 60.0, glat, glon)[0].cpu().numpy()
+    cz = solar.coszen(ctx.tables["JDAT"], 0.0, 3, 60.0, glat, glon)[0].cpu().numpy()
     flat = lambda a: np.ascontiguousarray(a.reshape(-1)[c0:c1], dtype=np.float64)
     rc = {k: cols[k] for k in abi.NMMB_COL_FIELDS + ("o3_top",)}
     rc.update({"cosz": cz, "t_sfc": flat(h["TSKIN"]), "alb": flat(h["ALBEDO"]), "emis": np.ones(c1 - c0),
@@ -180,6 +183,12 @@ def test_hook_matches_oracle_chain(gpu_device, with_aerosol):
         if key in aer:
             a = aer[key]
             rc[k] = np.ascontiguousarray(a.reshape(3, a.shape[1], LM + 1, JM * IM)[..., c0:c1])
+    if aerosol_mode == "climatology":
+        slmsk = np.where(h["SICE"] > 0.5, 2.0, 1.0 - h["SM"]).astype(np.float32).reshape(-1)[c0:c1]
+        rc["aer_sw"], rc["aer_lw"] = pyoracle.aer_optics(
+            aerosol.host_tables(int(ctx.tables["JDAT"][1])), cols["p_lev"], cols["t_lay"], cols["q_lay"],
+            h["GLON"].reshape(-1)[c0:c1], h["GLAT"].reshape(-1)[c0:c1], slmsk)
+        assert (rc["aer_sw"][0].sum((0, 1)) > 0).all()
     from woof_nam_amd.radiation import Radiation
     solcon = ctx.tables["rrtm_call_flags"]["solcon"]
     rad, _ = pyoracle.run(rc, params=Radiation.default_params(mcica=0, solcon=solcon))

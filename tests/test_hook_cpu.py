@@ -157,3 +157,30 @@ def test_restart_state_round_trip():
     assert (fresh.k1oz, fresh.k2oz) == (1, 2) and (b.k1oz, b.k2oz) == (12, 1)
     with pytest.raises(ValueError):
         rrtm_call.load_state_dict(resumed, {"format": 9})
+
+
+def test_clock_gives_the_step_time():
+    """JDAT handed to the hook is the time at the start of the step (phys_rad_driver.py:76-94): start +
+    NTIMESTEP DT with a clock, the context's own JDAT as given without one."""
+    import datetime as dt
+    ctx = make_ctx()
+    calls = []
+    ctx.ntimestep_rad = 3
+    rad_driver.run(ctx, rrtm_call=lambda c, a: calls.append(a))
+    assert calls[-1]["JDAT"] == [2025, 3, 20, 0, 12, 0, 0, 0]
+    ctx.start = dt.datetime(2025, 3, 31, 23, 0, 0)
+    ctx.ntimestep_rad = 90
+    rad_driver.run(ctx, rrtm_call=lambda c, a: calls.append(a))
+    assert calls[-1]["JDAT"] == [2025, 4, 1, 0, 0, 30, 0, 0] == ctx.tables["JDAT"]
+
+
+def test_aerosol_mode_checked():
+    ctx = make_ctx()
+    ctx.cfg.rrtm_aerosol = "volcanic"
+    with pytest.raises(ValueError, match="rrtm_aerosol"):
+        rrtm_call.aerosols(ctx, {}, {}, LM, JM, IM, 0, IM, CPU)
+    ctx.cfg.rrtm_aerosol = "tables"
+    with pytest.raises(KeyError, match="rrtm_faersw"):
+        rrtm_call.aerosols(ctx, {}, {}, LM, JM, IM, 0, IM, CPU)
+    ctx.cfg.rrtm_aerosol = "none"
+    assert rrtm_call.aerosols(ctx, {}, {}, LM, JM, IM, 0, IM, CPU) == {}

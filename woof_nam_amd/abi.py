@@ -37,6 +37,16 @@ def aer_shape(field: str, ncol: int, nlay: int) -> tuple:
     return (3, AER_BANDS[field], nlay + 1, ncol)
 
 
+# wr_aer_tables: the climatological aerosol's tables (device pointers, float32 then int32 members)
+AER_TABLE_FLOAT = ("haer", "prsref", "sigref", "rhlev", "extrhi", "scarhi", "ssarhi", "asyrhi", "extrhd", "scarhd",
+                   "ssarhd", "asyrhd", "extstra", "cmixg", "denng")
+AER_TABLE_INT = ("idxcg", "kprfg")
+
+
+class WrAerTables(C.Structure):
+    _fields_ = [(n, C.POINTER(C.c_float)) for n in AER_TABLE_FLOAT] + [(n, _ip) for n in AER_TABLE_INT]
+
+
 class WrParams(C.Structure):
     _fields_ = [("mcica", C.c_int32), ("iovr", C.c_int32), ("seed", C.c_uint64),
                 ("solcon", C.c_double), ("re_liq", C.c_double), ("re_ice", C.c_double),

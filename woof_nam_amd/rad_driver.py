@@ -7,8 +7,10 @@ for the rrtm/rrtm scheme with the default cloud branch:
   3. on radiation steps (NTIMESTEP == 1 or NTIMESTEP % NRADS == 0): build the RRTM argument dict
      (phys_rad_driver.py:108-168) and call the hook (default woof_nam_amd.rrtm_call.run).
 ctx: .cfg (nrads, nradl, nphs, shortwave, longwave, microphysics), .st (state arrays by name),
-.grid (im, jm, lm, pt, pdsg1, psgml1), .ntimestep_rad, .dt, .tables, and the date array JDAT in
-ctx.tables["JDAT"].
+.grid (im, jm, lm, pt, pdsg1, psgml1), .ntimestep_rad, .dt, .tables, and the clock: ctx.start (a
+datetime, the forecast start) from which each step's JDAT is formed, or else the date array JDAT of
+the current step in ctx.tables["JDAT"] (phys_rad_driver.py:76-94).  JDAT handed to the hook is the
+time at the start of the step.
 """
 from __future__ import annotations
 
@@ -28,8 +30,25 @@ ZEROED = ("CLDFRA", "CFRACH", "CFRACL", "CFRACM", "CZMEAN", "SIGT4")
 
 
 def default_radupdate(ctx, *, jdat, dtsw, dtx, lssw):
-    """The solar constant at the date of the step (sol_update_nmmb, JDAT + NTIMESTEP DT)."""
-    return {"SOLCON": solar.solcon(jdat, float(ctx.ntimestep_rad) * dtx, dtsw, dtx)}
+    """The solar constant at the date of the step (sol_update_nmmb at JDAT)."""
+    return {"SOLCON": solar.solcon(jdat, 0.0, dtsw, dtx)}
+
+
+def clock_jdat(ctx) -> list:
+    """JDAT (year, month, day, tz, hour, minute, second, ms) at the start of the step: ctx.start +
+    NTIMESTEP DT when the context has a start time (kept in ctx.tables["JDAT"]), else ctx.tables["JDAT"]
+    as given (phys_rad_driver.py:76-94)."""
+    import datetime as _dt
+    start = getattr(ctx, "start", None)
+    if start is None:
+        jd = ctx.tables.get("JDAT")
+        if jd is None:
+            raise RuntimeError("rad-driver: no clock (ctx.start) and no ctx.tables['JDAT']")
+        return list(jd)
+    t = start + _dt.timedelta(seconds=float(ctx.ntimestep_rad) * float(ctx.dt))
+    jd = [t.year, t.month, t.day, 0, t.hour, t.minute, t.second, 0]
+    ctx.tables["JDAT"] = jd
+    return jd
 
 
 def rrtm_args(ctx, jdat, nts, dt, flags, solcon):
@@ -57,7 +76,7 @@ def rrtm_args(ctx, jdat, nts, dt, flags, solcon):
         a[k] = st[k]
     a["QCW"] = st["QC"] if "QC" in st else zero3
     a["QCI"] = st["QI"] if (flags["F_QI"] and "QI" in st) else zero3
-    for k in ("EMISS", "P_PHY"):
+    for k in ("EMISS", "P_PHY", "SICE"):
         if k in st:
             a[k] = st[k]
     return a
@@ -79,7 +98,7 @@ def run(ctx, *, radupdate=None, rrtm_call=None, mype: int = 0) -> None:
     nts = int(ctx.ntimestep_rad)
     dt = float(ctx.dt)
     nrads = int(cfg.nrads)
-    jdat = ctx.tables["JDAT"]
+    jdat = clock_jdat(ctx)
     for k in ZEROED:                                     # 1. every step
         st[k].zero_()
     solcon = float(radupdate(ctx, jdat=jdat, dtsw=nrads * dt, dtx=dt, lssw=(nts % nrads == 0))["SOLCON"])

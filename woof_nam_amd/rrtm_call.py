@@ -19,8 +19,10 @@ What it does on a call, all of it on the device in library kernels:
     effective radii, low / middle / high cloud cover;
   * solar geometry: the sun's position for the date (sol_update) and each column's cos zenith averaged
     over the sunlit instants of the SW interval (coszmn), on the library (woof_nam_amd.solar);
-  * aerosol optics from the model's aerosol step when it left them in ctx.tables (the reference's
-    rrtm_faersw / rrtm_faerlw, setaer's whole-grid layout), passed as column slices;
+  * climatological aerosol, the NAM's (iaer = 11): wr_aer_optics maps the OPAC climatology of the
+    month onto the columns (setaer, aer_property, radclimaer) and gives each layer's optical depth,
+    single-scattering albedo and asymmetry per band (ctx.cfg.rrtm_aerosol = "tables" takes setaer's
+    outputs from ctx.tables instead, "none" runs without aerosol);
   * the surface albedo (wr_nmmb_albedo: setalb with IALB = 2, the Payne open-water table over sea in
     daylight, the state's ALBEDO elsewhere; emissivity 1, setemis with IEMS = 0, unless EMISS is given);
   * the radiation call on the path-form columns (woof_nam_amd.radiation, the NAM route mcica = 0;
@@ -36,7 +38,7 @@ import ctypes as C
 import numpy as np
 import torch
 
-from . import _lib, abi, ozone, solar
+from . import _lib, abi, aerosol, ozone, solar
 from .radiation import Radiation
 
 KEY_CTX = "woof_nam_amd.radiation"
@@ -235,16 +237,15 @@ def post(args, device, rows, cols, out, czmean, lw_step, sw_step, stream=None):
 # setaer's outputs in the reference's table keys (phys_rrtm_grrad_aerosols.py: faersw [3, 14, LM+1, JM, IM],
 # faerlw [3, 16, LM+1, JM, IM] float32 on the device)
 AER_KEYS = {"aer_sw": "rrtm_faersw", "aer_lw": "rrtm_faerlw"}
+KEY_AER = "woof_nam_amd.aerosol"
 
 
-def aerosols(ctx, lm, jm, im, c0, c1, device) -> dict:
-    """The aerosol optics of the radiated columns when the model's aerosol step has left them in
-    ctx.tables (setaer's layout over the whole grid): column-slice views, no copy.  Empty otherwise."""
+def _aer_tables(ctx, lm, jm, im, c0, c1, device) -> dict:
     out = {}
     for k, key in AER_KEYS.items():
         a = ctx.tables.get(key) if hasattr(ctx, "tables") else None
         if a is None:
-            continue
+            raise KeyError(f"rrtm-call: rrtm_aerosol = 'tables' but ctx.tables[{key!r}] is missing")
         if not isinstance(a, torch.Tensor):
             a = torch.as_tensor(np.asarray(a))
         shp = (3, abi.AER_BANDS[k], lm + 1, jm, im)
@@ -252,6 +253,39 @@ def aerosols(ctx, lm, jm, im, c0, c1, device) -> dict:
             raise ValueError(f"rrtm-call: {key} must be float32 {shp}, got {a.dtype} {tuple(a.shape)}")
         out[k] = a.to(device).contiguous().reshape(3, shp[1], lm + 1, jm * im)[..., c0:c1]
     return out
+
+
+def aerosols(ctx, args, cols, lm, jm, im, c0, c1, device) -> dict:
+    """The aerosol optics of the radiated columns (the column call's aer_sw / aer_lw), by
+    ctx.cfg.rrtm_aerosol:
+      "climatology" (default; the NAM's iaer = 11): wr_aer_optics over the OPAC climatology of the month of
+          JDAT, as the reference's call runs setaer on every radiation call (rrtm_call.py:354,
+          phys_rrtm_grrad_aerosols.py:91-124) after aer_update loaded the month (phys_rrtm_radupdate.py:
+          119-121); the surface type is the driver call's SLMSK: 2 on sea ice (SICE > 0.5), else 1 - SM;
+      "tables": setaer's outputs already in ctx.tables (rrtm_faersw / rrtm_faerlw over the whole grid),
+          taken as column slices;
+      "none": no aerosol (iaer = 0)."""
+    mode = str(_cfg(ctx, "rrtm_aerosol", "climatology"))
+    if mode == "none":
+        return {}
+    if mode == "tables":
+        return _aer_tables(ctx, lm, jm, im, c0, c1, device)
+    if mode != "climatology":
+        raise ValueError(f"rrtm-call: rrtm_aerosol must be 'climatology', 'tables' or 'none', got {mode!r}")
+    clim = ctx.tables.get(KEY_AER)
+    if clim is None or clim.device != device:
+        clim = aerosol.Climatology(device)
+        ctx.tables[KEY_AER] = clim
+    clim.set_month(int(args["JDAT"][1]))
+    sm = _arr(args, "SM", (jm, im), torch.float32).reshape(-1)[c0:c1].to(device)
+    if "SICE" in args:
+        sice = _arr(args, "SICE", (jm, im), torch.float32).reshape(-1)[c0:c1].to(device)
+        slmsk = torch.where(sice > 0.5, torch.full_like(sm, 2.0), 1.0 - sm)
+    else:
+        slmsk = 1.0 - sm
+    glat = _arr(args, "GLAT", (jm, im), torch.float32).reshape(-1)[c0:c1]
+    glon = _arr(args, "GLON", (jm, im), torch.float32).reshape(-1)[c0:c1]
+    return aerosol.optics(clim, cols, glat, glon, slmsk)
 
 
 KEY_OZONE = "woof_nam_amd.ozone"
@@ -328,13 +362,13 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     j0, j1 = int(args.get("J_S", 2)) - 1, int(args.get("J_E", jm - 1))
     if not (0 <= j0 < j1 <= jm):
         raise ValueError(f"rrtm-call: rows J_S..J_E = {j0 + 1}..{j1} outside 1..{jm}")
-    oz = ozone_struct(ctx, args, device, nts * dt)
+    oz = ozone_struct(ctx, args, device, 0.0)
     cols, _ = prep(args, device, (j0, j1), _llyr(ctx, args, lm), iovr=1, subgrid=bool(_cfg(ctx, "rrtm_subgrid", True)),
                    ozone=oz)
     c0, c1 = j0 * im, j1 * im
     glat = _arr(args, "GLAT", (jm, im)).reshape(-1)[c0:c1]
     glon = _arr(args, "GLON", (jm, im)).reshape(-1)[c0:c1].to(device)
-    cz = solar.coszen(args["JDAT"], nts * dt, nrads, dt, glat.to(device), glon)[0].contiguous()
+    cz = solar.coszen(args["JDAT"], 0.0, nrads, dt, glat.to(device), glon)[0].contiguous()
     sw_step = bool(sw and lsswr)
     f64 = dict(dtype=torch.float64, device=device)
     cols["cosz"] = cz if sw_step else torch.full_like(cz, -1.0)
@@ -343,7 +377,7 @@ def run(ctx, args: dict, *, stamp=None, rows=None, sw: bool = True) -> None:
     cols["emis"] = (_arr(args, "EMISS", (jm, im)).reshape(-1)[c0:c1].to(**f64).contiguous() if "EMISS" in args
                     else torch.ones(c1 - c0, **f64))
     cols["col_id"] = torch.arange(c0, c1, dtype=torch.int64, device=device)
-    cols.update(aerosols(ctx, lm, jm, im, c0, c1, device))
+    cols.update(aerosols(ctx, args, cols, lm, jm, im, c0, c1, device))
     params = Radiation.default_params(mcica=mcica, solcon=float(args["SOLCON"]), seed=int(_cfg(ctx, "rrtm_seed", 0)))
     rad = _context(ctx, device, c1 - c0, lm)
     if stamp is not None:
