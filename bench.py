@@ -61,6 +61,8 @@ TRAFFIC_MIXED = {
 
 
 def kernel_traffic(name: str, precision: str, cols_per_launch: float, days_per_launch: float):
+    if precision == "surrogate":      # measured for the table paths only
+        return None
     table = TRAFFIC_FP64 if precision == "fp64" else TRAFFIC_MIXED
     if name not in table:
         return None
@@ -106,6 +108,8 @@ FLOPS_MIXED = {
 
 
 def kernel_flops(name: str, precision: str, ncol: int, nday: int):
+    if precision == "surrogate":
+        return None
     table = FLOPS_FP64 if precision == "fp64" else FLOPS_MIXED
     if name not in table:
         return None
@@ -144,13 +148,21 @@ def serial_pass(d, precision, max_cols):
     return {"call_ms": call_ms, "ms": ms}
 
 
-def served_block(d, dev, ncol, seconds):
+def served_block(d, dev, ncol, seconds, precision="mixed"):
     """Config-3 serving: Poisson arrivals at 40 req/s of the parent grid, 3 km / 6 km nests and
-    ensemble members (one member per request, seeds 1..8), mixed precision, 200 GB HBM budget."""
+    ensemble members (one member per request, seeds 1..8), 200 GB HBM budget; mixed precision, or the
+    surrogate path with the trained network (woof_nam_amd/data/surrogate_rrtmg.npz) in the served
+    graphs."""
     import torch
 
     from woof_nam_amd import gen
     from woof_nam_amd.serve import Server, autotune, poisson_run
+    net = None
+    shapes = ((65536, 4), (131072, 2), (131072, 4), (262144, 2), (262144, 4), (524288, 2))
+    if precision == "surrogate":
+        from woof_nam_amd import surrogate as S
+        net = S.MLP.load(os.path.join(ROOT, "woof_nam_amd", "data", "surrogate_rrtmg.npz"), dev)
+        shapes = ((65536, 4), (131072, 2), (131072, 4), (262144, 2))   # 25 GB of network work space per 262,144-column slot
 
     def dev_grid(nx, ny, seed, id0):
         c = gen.grid(nx, ny, seed=seed)
@@ -163,23 +175,49 @@ def served_block(d, dev, ncol, seconds):
     t0 = time.perf_counter()
     # 524,288-column tiles only with 2 slots: 4 such slots beside this run's whole-grid context and the
     # requests' outputs could pass the 288 GB of HBM
-    tune = autotune(0, ncol=786432, precision="mixed", nreq=12,
-                    shapes=((65536, 4), (131072, 2), (131072, 4), (262144, 2), (262144, 4), (524288, 2)))
+    tune = autotune(0, ncol=786432, precision=precision, nreq=12, shapes=shapes, surrogate=net)
     best = tune["best"]
     print(f"# autotune {time.perf_counter() - t0:.1f} s: tile {best['tile']} slots {best['slots']}",
           file=sys.stderr, flush=True)
-    srv = Server(0, tile=best["tile"], slots=best["slots"], budget_gb=200.0, precision="mixed",
-                 filler={k: v[..., :best["tile"]] for k, v in d.items()})
+    srv = Server(0, tile=best["tile"], slots=best["slots"], budget_gb=200.0, precision=precision,
+                 filler={k: v[..., :best["tile"]] for k, v in d.items()}, surrogate=net)
     r = poisson_run(srv, kinds, rate=40.0, duration=seconds, seed=0, progress=30.0 if seconds > 30 else 0.0)
     srv.close()
     r["autotune"] = tune["all"]
-    r["tile"], r["slots"] = best["tile"], best["slots"]
+    r["tile"], r["slots"], r["precision"] = best["tile"], best["slots"], precision
     r["arrivals"] = f"Poisson 40 req/s for {seconds} s; kinds uniform over parent 796,590 / nest 110,889 / " \
                     "nest 40,000 / one ensemble member of the parent (own seed); offered load exceeds capacity, " \
                     "so p99 includes queueing"
     print(f"# served: {r['requests']} requests, {r['columns_per_s']:.4g} columns/s, p50 {r['p50_ms']:.0f} ms, "
           f"p99 {r['p99_ms']:.0f} ms", file=sys.stderr, flush=True)
     return r
+
+
+MFMA_PEAK_TFLOPS_BF16 = 2500.0   # dense BF16 MFMA peak of MI355X
+
+
+def surrogate_block(rad, d, out, tot, cnt, ncol, max_cols, net):
+    """The surrogate path's network against the MFMA roofline (the three layers of one tile per
+    launch: 2 x (8 x 128 + 128 x 128 + 128 x 504) FLOPs per radiation layer, inputs padded to 8), its
+    share of the call, and the call's errors against the FP64 path on the same columns."""
+    import torch
+    lay = (NL + 1) * min(ncol, max_cols)
+    fl = 2.0 * lay * (8 * 128 + 128 * 128 + 128 * 504)
+    mlp_ms = tot.get("mlp_gemm", 0.0) / max(1, cnt.get("mlp_gemm", 1))
+    feat_ms = tot.get("k_features", 0.0) / max(1, cnt.get("k_features", 1))
+    ach = fl / (mlp_ms * 1e-3) / 1e12 if mlp_ms > 0 else None
+    ref, _ = rad.run(d, precision="fp64")
+    torch.cuda.synchronize()
+    rms = {k: float(torch.sqrt(torch.mean((out[k] - ref[k]) ** 2))) for k in ref}
+    del ref
+    return {"mlp_ms_per_tile": mlp_ms, "features_ms_per_tile": feat_ms,
+            "roofline": {"bound": "mfma", "achieved": ach, "peak": MFMA_PEAK_TFLOPS_BF16, "unit": "TFLOP/s",
+                         "frac": ach / MFMA_PEAK_TFLOPS_BF16 if ach else None, "flops_per_launch": fl},
+            "network_bands": {"lw": [b + 1 for b in range(16) if net.bands >> b & 1],
+                              "sw": [16 + b for b in range(14) if net.bands >> (16 + b) & 1]},
+            "accuracy_vs_fp64": {"hr_lw_rms_kday": rms["hr_lw"] * KDAY, "hr_sw_rms_kday": rms["hr_sw"] * KDAY,
+                                 "flux_lw_rms_wm2": max(rms["flw_up"], rms["flw_dn"]),
+                                 "flux_sw_rms_wm2": max(rms["fsw_up"], rms["fsw_dn"])}}
 
 
 def aerosol_block(d, dev, nx, ny, steps):
@@ -282,7 +320,8 @@ def main(argv=None):
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--precision", choices=("fp64", "mixed", "surrogate"), default="fp64",
+                    help="surrogate: the mixed path with the trained network's gas optics (data/surrogate_rrtmg.npz)")
     ap.add_argument("--max-cols", type=int, default=0, help="tile columns (0 = the whole grid in one tile)")
     ap.add_argument("--nx", type=int, default=954)
     ap.add_argument("--ny", type=int, default=835)
@@ -293,6 +332,8 @@ def main(argv=None):
                     help="seconds of Poisson arrivals (40 req/s) for the served block (0 = skip)")
     ap.add_argument("--no-serial", action="store_true", help="skip the pass with the band kernels on one stream")
     ap.add_argument("--no-aerosol", action="store_true", help="skip the climatological-aerosol block")
+    ap.add_argument("--served-precision", choices=("mixed", "surrogate"), default="mixed",
+                    help="the served block's path (surrogate: the trained network inside the served graphs)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="first copy 1 GiB between two device buffers (reference for HBM byte counters)")
     a = ap.parse_args(argv)
@@ -323,6 +364,11 @@ def main(argv=None):
     d = {k: torch.from_numpy(v).to(dev) for k, v in cols.items()}
     a.max_cols = a.max_cols or ncol
     rad = Radiation(0, max_cols=a.max_cols)
+    if a.precision == "surrogate":
+        from woof_nam_amd import surrogate as S
+        net = S.MLP.load(os.path.join(ROOT, "woof_nam_amd", "data", "surrogate_rrtmg.npz"), dev)
+        rad.attach_surrogate(net)
+        a.no_serial = True          # the serial pass's context carries no network
     out, _ = rad.run(d, precision=a.precision)
     torch.cuda.synchronize()
     for _ in range(a.warmup):
@@ -365,7 +411,8 @@ def main(argv=None):
     res = {
         "metric": METRIC, "value": value, "unit": "columns/s", "n_gpus": 1, "steps": a.steps,
         "warmup": a.warmup, "ms_per_step": ms_per_step, "higher_is_better": True, "scaling": "weak",
-        "vs_baseline": None, "dtype": "f64" if a.precision == "fp64" else "f32+f64",
+        "vs_baseline": None,
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "surrogate": "bf16+f32+f64"}[a.precision],
         "data": "synthetic (seeded generator: NAM sigma levels, ISA T + 300 km GRF, RH GRF, clouds, 12 UTC sun)",
         "config": {"workload": f"NAM 12 km grid {a.nx}x{a.ny} = {ncol} columns x {NL} layers, LW+SW RRTMG, "
                                f"McICA 140/112 g, {a.precision}",
@@ -380,7 +427,7 @@ def main(argv=None):
                                      "the time a kernel shares the CUs with the others",
                      "call_achieved_gbs": (6312.0 if a.precision == "fp64" else 3600.0) * ncol / (ms_per_step * 1e-3) / 1e9},
     }
-    table = FLOPS_FP64 if a.precision == "fp64" else FLOPS_MIXED
+    table = {"fp64": FLOPS_FP64, "mixed": FLOPS_MIXED}.get(a.precision, {})
     if len(table) == 30:
         # the whole call's VALU work against the vector peak (all 30 band kernels, one call)
         call_fl = sum(v * (ncol if k.startswith("k_lw") else nday) for k, v in table.items())
@@ -405,6 +452,9 @@ def main(argv=None):
     if not a.no_aerosol:
         res["aerosol"] = aerosol_block(d, dev, a.nx, a.ny, a.steps)
         print(f"# aerosol optics: {res['aerosol']['ms_per_call']:.2f} ms/call", file=sys.stderr, flush=True)
+
+    if a.precision == "surrogate":
+        res["surrogate"] = surrogate_block(rad, d, out, tot, cnt, ncol, a.max_cols, net)
 
     if a.precision == "fp64" and not a.no_mixed:
         # the same workload on the mixed-precision path: throughput, speedup, errors against FP64
@@ -456,7 +506,7 @@ def main(argv=None):
     rad.close()                     # the served run's slots and requests get the device memory
     if a.served_seconds > 0:
         torch.cuda.empty_cache()
-        res["served"] = served_block(d, dev, ncol, a.served_seconds)
+        res["served"] = served_block(d, dev, ncol, a.served_seconds, a.served_precision)
     print(json.dumps(res), flush=True)
     return 0
 

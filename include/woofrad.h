@@ -369,9 +369,12 @@ typedef struct {
     const float *b1, *b2, *b3;                  /* [128], [128], [504]                          */
     const float *fmean, *fstd, *fmin, *fmax;    /* [5]: normalisation and training envelope     */
     const float *tmean, *tstd;                  /* [504]                                        */
+    uint32_t bands;                             /* bands that use the network: bit b for LW band
+                                                   b + 1 (b < 16), SW band b (16 <= b < 30);
+                                                   0: every band.  The others keep the tables.  */
 } wr_surrogate;
 
-/* Attach a surrogate to a context and allocate its tile work space (about 2.6 KB per layer-column
+/* Attach a surrogate to a context and allocate its tile work space (about 1.04 KB per layer-column
  * of max_cols x 64 layers).  wr_run_surrogate: the mixed-precision call with the LW and SW gas
  * optics (taugb/taumol) replaced by the network for every column whose layers all lie inside the
  * training envelope; other columns use the tables.  col_flag (optional, device, [ncol]) receives 1
@@ -394,6 +397,11 @@ int wr_mlp_mse(const float *Y, const float *T, int64_t n, float scale, float *dY
 int wr_mlp_adam(float *w, void *wb, const float *grad, float *m1, float *m2, int64_t n, float lr,
                 float b1, float b2, float eps, int32_t step, void *stream);
 int wr_mlp_to_bf16(const float *a, void *b, int64_t n, void *stream);
+/* The network's inference in one kernel (what wr_run_surrogate runs per tile): X [M][8] normalised
+ * features (float32, 16-byte aligned rows) -> Y bfloat16, output-major, output j of row m at
+ * Y[j*ldy + m] (ldy >= M), the hidden layers kept on chip.  Equal bit for bit to the three
+ * wr_mlp_gemm layers (bias + SiLU, bias + SiLU, bias) with a BF16 output.  Weights 16-byte aligned. */
+int wr_mlp_forward(const wr_surrogate *w, int64_t M, const float *X, void *Y, int64_t ldy, void *stream);
 
 /* Per-launch kernel timing for measurement (no counterpart in the reference: the reference
  * publishes call-level times only).  wr_timing(ctx, 1) creates an event pool once and brackets

@@ -30,6 +30,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.WrOzone) == 2 * 8 + 7 * 4 + 4
     assert C.sizeof(abi.WrNmmbCols) == 15 * 8
     assert C.sizeof(abi.WrAerTables) == 17 * 8
+    assert C.sizeof(abi.WrSurrogate) == 12 * 8 + 8
     assert C.sizeof(abi.WrNmmbOut) == 18 * 8
     assert C.sizeof(abi.WrParams) == 4 + 4 + 8 + 3 * 8 + 10 * 8
     assert C.sizeof(abi.WrOutputs) == 7 * 8

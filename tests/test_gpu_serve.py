@@ -1,4 +1,5 @@
-"""Served radiation: continuous batching, HBM-budget admission, graph replays.
+"""Served radiation: continuous batching, HBM-budget admission, graph replays (FP64, mixed, and the
+surrogate path with the trained network's inference inside the served graphs).
 
 Each request's result must be the same alone and in a mix of other requests (bit for bit: every
 column depends only on its inputs, column id and seed), and the same as a direct call.
@@ -7,9 +8,14 @@ import numpy as np
 import pytest
 import torch
 
+from pathlib import Path
+
 from woof_nam_amd import abi, gen
+from woof_nam_amd import surrogate as S
 from woof_nam_amd.radiation import Radiation
 from woof_nam_amd.serve import Request, Server, poisson_run
+
+TRAINED = Path(S.__file__).resolve().parent / "data" / "surrogate_rrtmg.npz"
 
 pytestmark = pytest.mark.gpu
 
@@ -23,12 +29,13 @@ def same(a, b):
     return all(torch.equal(a[k], b[k]) for k in abi.OUTPUT_FIELDS)
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
+@pytest.mark.parametrize("precision", ["mixed", "fp64", "surrogate"])
 def test_alone_equals_mix_equals_direct(gpu_device, precision):
     print(f"case serve alone-vs-mix {precision}", flush=True)
     a_cols = dev_cols(1500, 1, gpu_device)
     ids = torch.arange(10_000, 11_500, dtype=torch.int64, device=gpu_device)
-    srv = Server(0, tile=1024, slots=3, precision=precision)
+    net = S.MLP.load(TRAINED, gpu_device) if precision == "surrogate" else None
+    srv = Server(0, tile=1024, slots=3, precision=precision, surrogate=net)
     a1 = Request(rid=0, kind="a", cols=a_cols, seed=5, col_id=ids)
     srv.submit(a1)
     srv.drain()
@@ -43,6 +50,8 @@ def test_alone_equals_mix_equals_direct(gpu_device, precision):
     assert same(a1.out, mix[1].out)
     assert not torch.equal(a1.out["hr_sw"], mix[3].out["hr_sw"]) or not torch.equal(a1.out["hr_lw"], mix[3].out["hr_lw"])
     rad = Radiation(0, max_cols=4096)
+    if net is not None:
+        rad.attach_surrogate(net)
     direct, _ = rad.run({**a_cols, "col_id": ids}, params=Radiation.default_params(seed=5), precision=precision)
     torch.cuda.synchronize()
     assert same(a1.out, direct)

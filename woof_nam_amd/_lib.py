@@ -19,7 +19,7 @@ EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", 
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
            "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate", "wr_nmmb_prep", "wr_nmmb_post", "wr_nmmb_albedo",
-           "wr_sol_update", "wr_coszmn", "wr_rdtemp", "wr_aer_optics")
+           "wr_sol_update", "wr_coszmn", "wr_rdtemp", "wr_aer_optics", "wr_mlp_forward")
 
 _ERRORS = {
     abi.WR_ERR_ARG: ValueError,
@@ -96,6 +96,8 @@ def load():
     L.wr_aer_optics.argtypes = [C.POINTER(abi.WrAerTables), C.POINTER(abi.WrColumns), vp, vp, vp, vp, vp, C.c_int64,
                                 vp]
     L.wr_aer_optics.restype = C.c_int
+    L.wr_mlp_forward.argtypes = [C.POINTER(abi.WrSurrogate), C.c_int64, vp, vp, C.c_int64, vp]
+    L.wr_mlp_forward.restype = C.c_int
     i32, f64 = C.c_int32, C.c_double
     L.wr_sol_update.argtypes = [i32] * 6 + [f64, f64, f64, C.POINTER(abi.WrSol)]
     L.wr_sol_update.restype = C.c_int

@@ -66,7 +66,7 @@ class WrDiag(C.Structure):
 
 class WrSurrogate(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("w1", "w2", "w3", "b1", "b2", "b3", "fmean", "fstd", "fmin", "fmax",
-                                           "tmean", "tstd")]
+                                           "tmean", "tstd")] + [("bands", C.c_uint32)]
 
 
 INPUT_FIELDS = ("p_lev", "t_lay", "q_lay", "o3_lay", "cldf", "qc", "qi", "t_sfc", "emis", "alb", "cosz")

@@ -135,7 +135,7 @@ __global__ void __launch_bounds__(64, WR_GLW_WAVES) k_glw(const Tabs<R> *__restr
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = col_ok ? cs.lcf1[tt] : 0;
-    const int table = SURR ? (col_ok ? sg.flag[tt] : 1) : 1;
+    const int table = SURR ? (col_ok ? sg.table(tt, B) : 1) : 1;
     const double pwv = col_ok ? cs.pwv[tt] : 0.0;
     const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
     const double sfe = col_ok ? in.emis[i] : 1.0;

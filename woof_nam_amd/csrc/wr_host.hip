@@ -668,8 +668,7 @@ int wr_surrogate_attach(wr_ctx *c, const wr_surrogate *w)
     if (!c->sx) {
         const size_t S = (size_t)c->nl_max * c->max_cols;
         if (hipMalloc((void **)&c->sx, S * 8 * sizeof(float)) != hipSuccess ||
-            hipMalloc((void **)&c->sy, S * 504 * sizeof(float)) != hipSuccess ||
-            hipMalloc(&c->sh1, S * 128 * 2) != hipSuccess || hipMalloc(&c->sh2, S * 128 * 2) != hipSuccess ||
+            hipMalloc(&c->sy, (S + 64) * 504 * 2) != hipSuccess ||
             hipMalloc((void **)&c->sflag, (size_t)c->max_cols * sizeof(int)) != hipSuccess)
             return WR_ERR_NOMEM;
     }
@@ -810,8 +809,6 @@ void wr_destroy(wr_ctx *c)
     (void)hipFree(c->lwrec);
     (void)hipFree(c->sx);
     (void)hipFree(c->sy);
-    (void)hipFree(c->sh1);
-    (void)hipFree(c->sh2);
     (void)hipFree(c->sflag);
     for (int q = 0; q < WR_NAUX; ++q) {
         (void)hipFree(c->pool[q]);

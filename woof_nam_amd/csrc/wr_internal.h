@@ -31,8 +31,8 @@ struct wr_ctx {
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
     int has_sur;
     wr_surrogate sur;
-    float *sx, *sy;                  // features [S][8], outputs [504][S]
-    void *sh1, *sh2;                 // hidden activations, bfloat16 [S][128]
+    float *sx;                       // features [S][8]
+    void *sy;                        // network outputs, bfloat16 [504][S]
     int *sflag;                      // [max_cols] per-column table-optics flag
     // per-stream cache pools of the band kernels (pool[0]: the caller's stream and aux stream 0;
     // pool[q]: aux stream q), pool_bytes each, all allocated by wr_create

@@ -319,18 +319,23 @@ template <typename R> __device__ __forceinline__ void lw_tot(const Tabs<R> *__re
     }
 }
 
-// Learned gas optics of one tile (surrogate path): normalised network outputs Y[j * S + k * n + t]
+// Learned gas optics of one tile (surrogate path): normalised network outputs Y[j * S + k * n + t] (BF16;
+// S the padded row stride)
 // (j < 504 outputs, surface-first layer k, tile column t), de-normalised as y * tstd[j] + tmean[j];
-// flag[t] != 0 sends column t back to the table optics (inputs outside the training envelope).
+// flag[t] != 0 sends column t back to the table optics (inputs outside the training envelope);
+// bands: bit b set when band b uses the network (LW bands at bits 0..15, SW bands at 16..29).
 struct Surr {
-    const WR_G float *Y, *tmean, *tstd;
+    const WR_G __bf16 *Y;
+    const WR_G float *tmean, *tstd;
     const WR_G int *flag;
     long long S;
+    uint32_t bands;
+    __device__ __forceinline__ int table(int t, int bit) const { return ((bands >> bit) & 1u) ? flag[t] : 1; }
 };
 
 __device__ __forceinline__ float surr_out(const Surr &sg, int j, size_t s)
 {
-    return sg.Y[(size_t)j * sg.S + s] * sg.tstd[j] + sg.tmean[j];
+    return (float)sg.Y[(size_t)j * sg.S + s] * sg.tstd[j] + sg.tmean[j];
 }
 
 template <typename R, int B, int NGL = LW_NG[B]>
@@ -490,7 +495,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = cs.lcf1[t];
-    const int table = SURR ? sg.flag[t] : 1;  // 0: learned gas optics for this column
+    const int table = SURR ? sg.table(t, B) : 1;  // 0: learned gas optics for this column
     const double pwv = cs.pwv[t];
     const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
     const double sfe = in.emis[i];
@@ -667,7 +672,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
     const RadCol rc{in, i, in.nlay + 1};
     const int nrl = rc.nrl;
     const int lcf1 = cs.lcf1[t];
-    const int table = SURR ? sg.flag[t] : 1;
+    const int table = SURR ? sg.table(t, B) : 1;
     const double pwv = cs.pwv[t];
     const R sdif = (R)((B == 0 || B == 3 || B == 9) ? 1.66 : fmin(1.80, fmax(1.50, LW_A0[B] + LW_A1[B] * exp_d(LW_A2[B] * pwv))));
     const double sfe = in.emis[i];
@@ -866,7 +871,7 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     const R asfc = (R)((SW_IDXSFC[B] != 0) ? alb : 0.5 * (alb + alb));
     const double cf1 = cs.zcf1[t];
     const int lt = cs.sw_laytrop[t];
-    const int table = SURR ? sg.flag[t] : 1;
+    const int table = SURR ? sg.table(t, 16 + B) : 1;
     const uint64_t col = in.col_id ? in.col_id[i] : (uint64_t)i;
     const uint64_t seed = in.seed_dev ? *in.seed_dev : par.seed;
     uint64_t mb[NG];

@@ -2,9 +2,12 @@
 // FP64 path) and the BF16 MFMA kernels of the MLP (forward, backward, Adam).
 //
 // GEMM: C[M x N] = A[M x K] * B[K x N] with arbitrary element strides on A, B and C, operands
-// staged through LDS as BF16, v_mfma_f32_16x16x16_bf16 tiles, FP32 accumulation.  A 256-thread
-// workgroup computes a 64 x 64 tile of C (4 waves, 2 x 2 MFMA tiles of 16 x 16 each); K advances
-// 16 per step.  Split-K writes deterministic partial sums, reduced by k_splitk_reduce.
+// staged through LDS as BF16, gfx950's v_mfma_f32_16x16x32_bf16 tiles, FP32 accumulation.  A
+// 256-thread workgroup computes a 64 x 64 tile of C (4 waves, 2 x 2 MFMA tiles of 16 x 16 each); K
+// advances 32 per step.  Each thread stages a run of 8 elements of A and of B along whichever of
+// their dimensions is contiguous in memory (one 16- or 32-byte load when aligned and whole), so the
+// staging reads are coalesced for row-major activations, transposed activations and weights alike.
+// Split-K writes deterministic partial sums, reduced by k_splitk_reduce.
 // Epilogues: bias, bias + SiLU (keeps the pre-activation), multiply by SiLU'(Z) (backward), plain.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
@@ -14,12 +17,12 @@
 namespace {
 using namespace wr;
 
-typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
-template <typename T> __device__ __forceinline__ float ld(const T *p, size_t i) { return (float)p[i]; }
-
 enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_SILU = 2, EPI_DSILU = 3 };
+constexpr int GK = 32;      // K per step (one MFMA's depth)
+constexpr int GLD = GK + 8; // LDS row stride in BF16 elements: 80 bytes, rows 16-byte aligned
 
 struct GemmArgs {
     int M, N, K;
@@ -33,7 +36,7 @@ struct GemmArgs {
     const float *bias;       // [N]
     float *Zout;             // EPI_BIAS_SILU: pre-activation, same strides as C
     const float *Zin;        // EPI_DSILU: pre-activation the derivative is taken at, same strides as C
-    int k_per_split;         // K range of one split (grid.z)
+    int k_per_split;         // K range of one split (grid.z), a multiple of GK
     long long split_stride;  // element offset between split partial outputs
 };
 
@@ -44,11 +47,56 @@ __device__ __forceinline__ float dsilu(float z)
     return s * (1.0f + z * (1.0f - s));
 }
 
+// cnt (0..8) elements p[j * stride] as BF16, zero beyond; one vector load when the run is whole,
+// unit-stride and 16-byte aligned
+__device__ __forceinline__ bf16x8 run8(const float *p, long long stride, int cnt)
+{
+    bf16x8 v;
+    if (cnt == 8 && stride == 1 && ((uintptr_t)p & 15) == 0) {
+        const float4 a = *(const float4 *)p, b = *(const float4 *)(p + 4);
+        v[0] = (__bf16)a.x, v[1] = (__bf16)a.y, v[2] = (__bf16)a.z, v[3] = (__bf16)a.w;
+        v[4] = (__bf16)b.x, v[5] = (__bf16)b.y, v[6] = (__bf16)b.z, v[7] = (__bf16)b.w;
+        return v;
+    }
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (__bf16)(j < cnt ? p[j * stride] : 0.0f);
+    return v;
+}
+__device__ __forceinline__ bf16x8 run8(const __bf16 *p, long long stride, int cnt)
+{
+    if (cnt == 8 && stride == 1 && ((uintptr_t)p & 15) == 0) return *(const bf16x8 *)p;
+    bf16x8 v;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = j < cnt ? p[j * stride] : (__bf16)0.0f;
+    return v;
+}
+
+// Stage a 64 x GK block of an operand into S[row][k] (rows: M of A, N of B).  rows_contig: the row
+// index is the unit-stride one (a run of 8 rows at one k); otherwise a run of 8 k at one row.
+template <typename T>
+__device__ __forceinline__ void stage(const T *X, long long ld_row, long long ld_k, int nrow, int r0, int kk,
+                                      int k_end, bool rows_contig, __bf16 (*S)[GLD], int tid)
+{
+    if (rows_contig) {
+        const int c = tid >> 3, r = (tid & 7) * 8;
+        const int row = r0 + r, k = kk + c;
+        const int cnt = k < k_end ? max(0, min(8, nrow - row)) : 0;
+        const bf16x8 v = run8(X + (cnt > 0 ? (size_t)row * ld_row + (size_t)k * ld_k : 0), ld_row, cnt);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) S[r + j][c] = v[j];
+    } else {
+        const int r = tid >> 2, c = (tid & 3) * 8;
+        const int row = r0 + r, k = kk + c;
+        const int cnt = row < nrow ? max(0, min(8, k_end - k)) : 0;
+        *(bf16x8 *)&S[r][c] = run8(X + (cnt > 0 ? (size_t)row * ld_row + (size_t)k * ld_k : 0), ld_k, cnt);
+    }
+}
+
 template <typename TA, typename TB, int EPI>
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
 {
-    __shared__ __bf16 As[64][16 + 4];
-    __shared__ __bf16 Bs[64][16 + 4];  // transposed: Bs[n][k]
+    __shared__ __attribute__((aligned(16))) __bf16 As[64][GLD];
+    __shared__ __attribute__((aligned(16))) __bf16 Bs[64][GLD];  // transposed: Bs[n][k]
     const int tid = threadIdx.x;
     const int wave = tid >> 6, lane = tid & 63;
     const int m0 = blockIdx.x * 64, n0 = blockIdx.y * 64;  // M on x: up to 2^31 rows
@@ -56,6 +104,7 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
     const int k_end = min(g.K, k_begin + g.k_per_split);
     const TA *A = (const TA *)g.A;
     const TB *B = (const TB *)g.B;
+    const bool a_rows = g.lda_k != 1 && g.lda_m == 1, b_rows = g.ldb_k != 1 && g.ldb_n == 1;
     const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;  // wave's 32 x 32 sub-tile
     f32x4 acc[2][2];
 #pragma unroll
@@ -63,34 +112,19 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
 
-    for (int kk = k_begin; kk < k_end; kk += 16) {
-        // stage 64 x 16 of A and 16 x 64 of B (4 elements per thread each), zero outside the matrix
-#pragma unroll
-        for (int e = 0; e < 4; ++e) {
-            const int idx = tid * 4 + e;  // 0..1023
-            const int r = idx >> 4, c = idx & 15;
-            const int m = m0 + r, k = kk + c;
-            float va = 0.f;
-            if (m < g.M && k < k_end) va = ld(A, (size_t)m * g.lda_m + (size_t)k * g.lda_k);
-            As[r][c] = (__bf16)va;
-            const int n = n0 + r;
-            float vb = 0.f;
-            if (n < g.N && k < k_end) vb = ld(B, (size_t)k * g.ldb_k + (size_t)n * g.ldb_n);
-            Bs[r][c] = (__bf16)vb;
-        }
+    for (int kk = k_begin; kk < k_end; kk += GK) {
+        stage(A, g.lda_m, g.lda_k, g.M, m0, kk, k_end, a_rows, As, tid);
+        stage(B, g.ldb_n, g.ldb_k, g.N, n0, kk, k_end, b_rows, Bs, tid);
         __syncthreads();
-        const int row = lane & 15, kq = (lane >> 4) * 4;
+        // lane holds row (lane % 16) and k (lane / 16) * 8 + 0..7 of each operand tile
+        const int row = lane & 15, kq = (lane >> 4) * 8;
 #pragma unroll
         for (int i = 0; i < 2; ++i) {
-            bf16x4 a;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) a[q] = As[wm + i * 16 + row][kq + q];
+            const bf16x8 a = *(const bf16x8 *)&As[wm + i * 16 + row][kq];
 #pragma unroll
             for (int j = 0; j < 2; ++j) {
-                bf16x4 b;
-#pragma unroll
-                for (int q = 0; q < 4; ++q) b[q] = Bs[wn + j * 16 + row][kq + q];
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+                const bf16x8 b = *(const bf16x8 *)&Bs[wn + j * 16 + row][kq];
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i][j], 0, 0, 0);
             }
         }
         __syncthreads();
@@ -117,6 +151,136 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g)
                 if (g.C) C[o] = v;
                 if (g.Cb) g.Cb[o] = (__bf16)v;
             }
+}
+
+// ---- fused inference: the three layers of the network for 64 rows per workgroup ----------------
+// H1 = SiLU(X W1^T + b1) and H2 = SiLU(H1 W2^T + b2) stay in LDS (BF16); Y = H2 W3^T + b3 is written
+// output-major in BF16, Y[n * ldy + m], through an LDS transpose so that each store is a contiguous
+// run of rows.  Same MFMA operands, K order, bias, SiLU and BF16 roundings as the three-GEMM chain,
+// so the outputs equal the chain's bit for bit; the weights ([n][k] rows, BF16) are read straight
+// into MFMA operands (L2-resident).
+constexpr int FH = 128, FO = 504, FLD = FH + 8;
+
+struct MlpW {
+    const __bf16 *w1, *w2, *w3;  // [128][8], [128][128], [504][128]
+    const float *b1, *b2, *b3;   // [128], [128], [504]
+};
+
+__device__ __forceinline__ bf16x8 wfrag(const __bf16 *w, int ld, int n, int nmax, int k)
+{
+    bf16x8 v;
+    if (n < nmax) return *(const bf16x8 *)(w + (size_t)n * ld + k);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) v[j] = (__bf16)0.0f;
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_mlp_fused(MlpW w, const float *__restrict__ X, long long M,
+                                                   __bf16 *__restrict__ Y, long long ldy, int vec)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 Hs[2][64][FLD];  // H1, H2
+    __shared__ __attribute__((aligned(16))) __bf16 Ts[64][64 + 8];  // an output chunk, [n][m]
+    const int tid = threadIdx.x, wave = tid >> 6, lane = tid & 63;
+    const long long m0 = (long long)blockIdx.x * 64;
+    const int row = lane & 15, kq = (lane >> 4) * 8, mq = (lane >> 4) * 4;
+    const bf16x8 zero8 = {};
+
+    // layer 1 (K = 8, zero-padded to 32): wave w owns hidden units [32 w, 32 w + 32)
+    {
+        f32x4 acc[4][2];
+        bf16x8 b[2];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) b[j] = kq == 0 ? wfrag(w.w1, 8, wave * 32 + j * 16 + row, FH, 0) : zero8;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long long m = m0 + i * 16 + row;
+            bf16x8 a = zero8;
+            if (kq == 0 && m < M) {
+                const float4 x0 = *(const float4 *)(X + (size_t)m * 8), x1 = *(const float4 *)(X + (size_t)m * 8 + 4);
+                a[0] = (__bf16)x0.x, a[1] = (__bf16)x0.y, a[2] = (__bf16)x0.z, a[3] = (__bf16)x0.w;
+                a[4] = (__bf16)x1.x, a[5] = (__bf16)x1.y, a[6] = (__bf16)x1.z, a[7] = (__bf16)x1.w;
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], (f32x4){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = wave * 32 + j * 16 + row;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Hs[0][i * 16 + mq + q][n] = (__bf16)silu(acc[i][j][q] + w.b1[n]);
+            }
+    }
+    __syncthreads();
+    // layer 2 (K = 128)
+    {
+        f32x4 acc[4][2];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < FH; ks += 32) {
+            bf16x8 b[2];
+#pragma unroll
+            for (int j = 0; j < 2; ++j) b[j] = wfrag(w.w2, FH, wave * 32 + j * 16 + row, FH, ks + kq);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bf16x8 a = *(const bf16x8 *)&Hs[0][i * 16 + row][ks + kq];
+#pragma unroll
+                for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[j], acc[i][j], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 2; ++j) {
+                const int n = wave * 32 + j * 16 + row;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) Hs[1][i * 16 + mq + q][n] = (__bf16)silu(acc[i][j][q] + w.b2[n]);
+            }
+    }
+    __syncthreads();
+    // layer 3 (K = 128, 504 outputs in chunks of 64; wave w owns outputs [16 w, 16 w + 16) of a chunk)
+    for (int c0 = 0; c0 < FO; c0 += 64) {
+        const int nb = c0 + wave * 16;
+        f32x4 acc[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) acc[i] = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < FH; ks += 32) {
+            const bf16x8 b = wfrag(w.w3, FH, nb + row, FO, ks + kq);
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const bf16x8 a = *(const bf16x8 *)&Hs[1][i * 16 + row][ks + kq];
+                acc[i] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, acc[i], 0, 0, 0);
+            }
+        }
+        const int n = nb + row;
+        const float bias = n < FO ? w.b3[n] : 0.0f;
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) Ts[wave * 16 + row][i * 16 + mq + q] = (__bf16)(acc[i][q] + bias);
+        __syncthreads();
+        {   // 64 outputs x 64 rows: a thread stores 16 consecutive rows of one output
+            const int nl = tid >> 2, part = (tid & 3) * 16;
+            const int no = c0 + nl;
+            const long long m = m0 + part;
+            if (no < FO && m < M) {
+                __bf16 *dst = Y + (size_t)no * ldy + m;
+                if (vec && m + 16 <= M) {
+                    *(bf16x8 *)dst = *(const bf16x8 *)&Ts[nl][part];
+                    *(bf16x8 *)(dst + 8) = *(const bf16x8 *)&Ts[nl][part + 8];
+                } else {
+                    for (int e = 0; e < 16 && m + e < M; ++e) dst[e] = Ts[nl][part + e];
+                }
+            }
+        }
+        __syncthreads();
+    }
 }
 
 // sum of split partials: out[i] = sum_s part[s * stride + i]  (fixed order)
@@ -283,14 +447,32 @@ int wr_mlp_gemm(int32_t M, int32_t N, int32_t K, const void *A, int32_t a_bf16, 
     g.B = B, g.ldb_k = ldb_k, g.ldb_n = ldb_n;
     g.C = C, g.Cb = (__bf16 *)Cb, g.ldc_m = ldc_m, g.ldc_n = ldc_n;
     g.bias = bias, g.Zout = Zout, g.Zin = Zin;
-    const int kps = ((K + splits - 1) / splits + 15) / 16 * 16;
-    g.k_per_split = kps > 0 ? kps : 16;
+    const int kps = ((K + splits - 1) / splits + GK - 1) / GK * GK;
+    g.k_per_split = kps > 0 ? kps : GK;
     g.split_stride = split_stride;
     hipStream_t s = (hipStream_t)stream;
     if (a_bf16 && b_bf16) return gemm_dispatch<__bf16, __bf16>(epilogue, g, splits, s);
     if (a_bf16) return gemm_dispatch<__bf16, float>(epilogue, g, splits, s);
     if (b_bf16) return gemm_dispatch<float, __bf16>(epilogue, g, splits, s);
     return gemm_dispatch<float, float>(epilogue, g, splits, s);
+}
+
+int wr_mlp_forward(const wr_surrogate *w, int64_t M, const float *X, void *Y, int64_t ldy, void *stream)
+{
+    if (!w || !w->w1 || !w->w2 || !w->w3 || !w->b1 || !w->b2 || !w->b3 || !X || !Y || M < 0 || ldy < M ||
+        (M + 63) / 64 > 0x7fffffffLL)
+        return WR_ERR_ARG;
+    // the weight rows and feature rows are read as 16-byte vectors
+    const uintptr_t al = (uintptr_t)w->w1 | (uintptr_t)w->w2 | (uintptr_t)w->w3 | (uintptr_t)X;
+    if (al & 15) return WR_ERR_ARG;
+    const int vec = (ldy & 7) == 0 && ((uintptr_t)Y & 15) == 0;  // output runs stored as 16-byte vectors
+    if (M == 0) return WR_OK;
+    MlpW mw;
+    mw.w1 = (const __bf16 *)w->w1, mw.w2 = (const __bf16 *)w->w2, mw.w3 = (const __bf16 *)w->w3;
+    mw.b1 = w->b1, mw.b2 = w->b2, mw.b3 = w->b3;
+    hipLaunchKernelGGL(k_mlp_fused, dim3((unsigned)((M + 63) / 64)), dim3(256), 0, (hipStream_t)stream, mw, X,
+                       (long long)M, (__bf16 *)Y, (long long)ldy, vec);
+    return hipGetLastError() == hipSuccess ? WR_OK : WR_ERR_HIP;
 }
 
 int wr_mlp_splitk_reduce(const float *part, int32_t splits, int64_t stride, float *out, int64_t n, void *stream)

@@ -2,10 +2,7 @@
 // read by the mixed-precision band kernels (columns outside the training envelope use the tables).
 #include "wr_launch.h"
 
-extern "C" int wr_mlp_gemm(int32_t M, int32_t N, int32_t K, const void *A, int32_t a_bf16, int64_t lda_m,
-                           int64_t lda_k, const void *B, int32_t b_bf16, int64_t ldb_k, int64_t ldb_n, float *C,
-                           void *Cb, int64_t ldc_m, int64_t ldc_n, const float *bias, float *Zout, const float *Zin,
-                           int32_t epilogue, int32_t splits, int64_t split_stride, void *stream);
+extern "C" int wr_mlp_forward(const wr_surrogate *w, int64_t M, const float *X, void *Y, int64_t ldy, void *stream);
 
 namespace {
 using namespace wr;
@@ -57,15 +54,14 @@ int wr_launch_surr(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs 
     tend(c, s);
     if (hipGetLastError() != hipSuccess) return WR_ERR_HIP;
     const wr_surrogate &w = c->sur;
+    // the three layers in one kernel, hidden activations in LDS; outputs stored output-major, Y[j * ldy + s],
+    // for coalesced reads in the band kernels, in BF16: the normalised outputs are O(1), so BF16's 2^-8
+    // step sits far below the network's own error and the band kernels read half the bytes.  The row
+    // stride is padded to 64 outputs (128 bytes), so every 64-row run of an output is one whole cache
+    // line, stored as vectors.
+    const long long ldy = (S + 63) / 64 * 64;
     tbeg(c, KID_MLP, s);
-    int rc = wr_mlp_gemm((int)S, 128, 8, c->sx, 0, 8, 1, w.w1, 1, 1, 8, nullptr, c->sh1, 128, 1, w.b1, nullptr, nullptr, 2,
-                         1, 0, s);
-    if (rc == WR_OK)
-        rc = wr_mlp_gemm((int)S, 128, 128, c->sh1, 1, 128, 1, w.w2, 1, 1, 128, nullptr, c->sh2, 128, 1, w.b2, nullptr,
-                         nullptr, 2, 1, 0, s);
-    if (rc == WR_OK)  // outputs stored output-major, Y[j * S + s], for coalesced reads in the band kernels
-        rc = wr_mlp_gemm((int)S, 504, 128, c->sh2, 1, 128, 1, w.w3, 1, 1, 128, c->sy, nullptr, 1, S, w.b3, nullptr,
-                         nullptr, 1, 1, 0, s);
+    const int rc = wr_mlp_forward(&w, S, c->sx, c->sy, ldy, s);
     tend(c, s);
     if (rc != WR_OK) return rc;
     if (flag_out) {
@@ -73,6 +69,7 @@ int wr_launch_surr(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs 
         if (hipGetLastError() != hipSuccess) return WR_ERR_HIP;
     }
     Surr sg;
-    sg.Y = gptr(c->sy), sg.tmean = gptr(w.tmean), sg.tstd = gptr(w.tstd), sg.flag = gptr(c->sflag), sg.S = S;
+    sg.Y = gptr((const __bf16 *)c->sy), sg.tmean = gptr(w.tmean), sg.tstd = gptr(w.tstd), sg.flag = gptr(c->sflag), sg.S = ldy;
+    sg.bands = w.bands ? w.bands : 0x3fffffffu;
     return launch_tile<float, true>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, -1, s, sg);
 }

@@ -137,7 +137,8 @@ class Radiation:
         return out
 
     def attach_surrogate(self, mlp) -> None:
-        """Use a trained surrogate.MLP for precision="surrogate" calls (its tensors must stay alive)."""
+        """Use a trained surrogate.MLP for precision="surrogate" calls (its tensors must stay alive); its
+        ``bands`` mask (0: all) says which bands take the network's optics, the others the tables."""
         if mlp.norm is None:
             raise ValueError("surrogate has no normalisation: train it (surrogate.fit) or load it first")
         n = mlp.norm
@@ -145,7 +146,7 @@ class Radiation:
         w = abi.WrSurrogate(*[C.c_void_p(t.data_ptr()) for t in (
             mlp.wb["W1"], mlp.wb["W2"], mlp.wb["W3"], mlp.w["b1"], mlp.w["b2"], mlp.w["b3"],
             self._sur_keep["fmean"], self._sur_keep["fstd"], self._sur_keep["fmin"], self._sur_keep["fmax"],
-            self._sur_keep["tmean"], self._sur_keep["tstd"])])
+            self._sur_keep["tmean"], self._sur_keep["tstd"])], int(getattr(mlp, "bands", 0)))
         self._sur_mlp = mlp
         _lib.check(_lib.load().wr_surrogate_attach(self._h, C.byref(w)), "wr_surrogate_attach")
 

@@ -10,7 +10,8 @@ seeds) arrive at any time.  The server
   while others run join the tile queue at once: continuous batching at tile granularity);
 * runs each tile as one replay of a HIP graph captured once per slot (the kernels of a tile:
   column scalars, day list, 16 LW and 14 SW band kernels spread over the context's band streams,
-  post) over the slot's staging buffers; a tile is gathered into the staging buffers, the graph
+  post; with precision "surrogate" also the layer features and the network's three MFMA layers,
+  with the table fallback of columns outside the training envelope) over the slot's staging buffers; a tile is gathered into the staging buffers, the graph
   replayed and the outputs scattered back, all on the slot's stream, with the request's seed and
   the tile's column count read by the kernels from device memory (a short last tile costs only
   its own columns);
@@ -63,8 +64,10 @@ class Request:
 
 
 class _Slot:
-    def __init__(self, device, tile, nlay, precision, params, graphs, filler):
+    def __init__(self, device, tile, nlay, precision, params, graphs, filler, surrogate=None):
         self.rad = Radiation(device, max_cols=tile)
+        if surrogate is not None:        # the network's inference is captured into the slot's graph
+            self.rad.attach_surrogate(surrogate)
         self.stream = torch.cuda.Stream(device)           # default priority
         self.tile = tile
         self.precision = precision
@@ -124,7 +127,12 @@ class Server:
     """Continuous-batching radiation server on one GPU (see the module docstring)."""
 
     def __init__(self, device: int = 0, tile: int = 65536, slots: int = 4, budget_gb: float = 200.0,
-                 precision: str = "mixed", graphs: bool = True, nlay: int = 60, params=None, filler=None):
+                 precision: str = "mixed", graphs: bool = True, nlay: int = 60, params=None, filler=None,
+                 surrogate=None):
+        """precision: "fp64", "mixed" or "surrogate"; the last needs ``surrogate``, a trained
+        surrogate.MLP that every slot attaches (its tensors must outlive the server)."""
+        if precision == "surrogate" and surrogate is None:
+            raise ValueError("serve: precision 'surrogate' needs the trained network (surrogate=...)")
         self.device = torch.device("cuda", device)
         self.tile, self.nslots, self.budget = int(tile), int(slots), float(budget_gb) * 1e9
         self.precision, self.nlay = precision, nlay
@@ -133,7 +141,8 @@ class Server:
             from . import gen
             cols = gen.columns(self.tile, seed=12345)
             filler = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.device) for k, v in cols.items()}
-        self.slots = [_Slot(self.device, self.tile, nlay, precision, self.params, graphs, filler)
+        self.slots = [_Slot(self.device, self.tile, nlay, precision, self.params, graphs, filler,
+                            surrogate if precision == "surrogate" else None)
                       for _ in range(self.nslots)]
         # the admission budget never exceeds what the device has left once the slots exist, less a
         # reserve for the runtime (queue scratch, graph and internal allocations): outputs admitted
@@ -270,7 +279,8 @@ def poisson_run(server: Server, kinds: dict, rate: float, duration: float, seed:
 
 
 def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), ncol: int = 262144,
-             precision: str = "mixed", cache: str | None = None, nreq: int = 4, shapes=None) -> dict:
+             precision: str = "mixed", cache: str | None = None, nreq: int = 4, shapes=None,
+             surrogate=None) -> dict:
     """Pick the tile size and slot count with the highest columns/s on a fixed batch of requests.
 
     The candidates are every (tile, slots) of `tiles` x `slots`, or the explicit list `shapes` of
@@ -294,7 +304,7 @@ def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), nco
     cuts = [0] + [int(x) for x in np.round(np.cumsum(w) / w.sum() * ncol)]
     results = []
     for tile, ns in cand:
-        srv = Server(device, tile=tile, slots=ns, precision=precision,
+        srv = Server(device, tile=tile, slots=ns, precision=precision, surrogate=surrogate,
                      filler={k: v[..., :tile] if v.shape[-1] >= tile else v[..., :1].expand(*v.shape[:-1], tile)
                              for k, v in dev.items()})
         best = None

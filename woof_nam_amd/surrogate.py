@@ -23,6 +23,19 @@ from .radiation import Radiation, _colstruct, input_shapes
 
 K_IN, K_PAD, HIDDEN, N_OUT = 5, 8, 128, 504
 SLICES = {"lw_lntau": (0, 140), "lw_frac": (140, 280), "sw_lntau": (280, 392), "sw_ssa": (392, 504)}
+# layer optical depths outside [1e-5, 100] change no flux the radiation resolves (a transmittance of
+# 1 - 1e-5 or of e^-100): the training targets are clamped to that range, so the network spends its
+# precision where the heating rates depend on it
+LNTAU_MIN, LNTAU_MAX = math.log(1.0e-5), math.log(100.0)
+ALL_BANDS = (1 << 30) - 1
+
+
+def clamp_targets(targ):
+    """ln tau targets (LW and SW) clamped to [ln 1e-5, ln 100], in place."""
+    for name in ("lw_lntau", "sw_lntau"):
+        a, b = SLICES[name]
+        targ[:, a:b].clamp_(LNTAU_MIN, LNTAU_MAX)
+    return targ
 EPI_STORE, EPI_BIAS, EPI_BIAS_SILU, EPI_DSILU = 0, 1, 2, 3
 
 
@@ -76,6 +89,7 @@ class MLP:
                 self.w[name].copy_(torch.from_numpy(w.astype(np.float32)))
         self.sync_bf16()
         self.norm = None      # feature/target normalisation and envelope (set by fit)
+        self.bands = 0        # bands that use the network (bit b: LW band b + 1, b >= 16: SW band b); 0: all
         self._ws = {}
 
     def sync_bf16(self):
@@ -108,6 +122,18 @@ class MLP:
              epi=EPI_BIAS_SILU)
         gemm(B, N_OUT, HIDDEN, H2, HIDDEN, 1, self.wb["W3"], 1, HIDDEN, Y, None, N_OUT, 1, self.w["b3"],
              epi=EPI_BIAS)
+        return Y
+
+    def forward_fused(self, X, ldy=None):
+        """The inference kernel (wr_mlp_forward, the three layers in one kernel): X [B][8] normalised
+        features -> Y bfloat16 [504][ldy] output-major (rows >= B of each output untouched)."""
+        B = int(X.shape[0])
+        ldy = ldy or B
+        Y = torch.empty((N_OUT, ldy), dtype=torch.bfloat16, device=self.device)
+        w = abi.WrSurrogate(*[C.c_void_p(t.data_ptr()) for t in (self.wb["W1"], self.wb["W2"], self.wb["W3"],
+                                                                   self.w["b1"], self.w["b2"], self.w["b3"])])
+        X = X.contiguous()
+        _lib.check(_lib.load().wr_mlp_forward(C.byref(w), B, _p(X), _p(Y), ldy, _stream()), "wr_mlp_forward")
         return Y
 
     def _wgrad(self, dZ, H, n_out, n_in, out):
@@ -158,6 +184,7 @@ class MLP:
         d = {n: self.w[n].detach().cpu().numpy() for n, _ in self.SHAPES}
         for k, v in (self.norm or {}).items():
             d["norm_" + k] = v.detach().cpu().numpy() if isinstance(v, torch.Tensor) else np.asarray(v)
+        d["bands"] = np.uint32(self.bands)
         np.savez(path, **d)
 
     @classmethod
@@ -168,6 +195,7 @@ class MLP:
             m.w[n].copy_(torch.from_numpy(z[n]))
         m.sync_bf16()
         m.norm = {k[5:]: torch.from_numpy(z[k]).to(m.device) for k in z.files if k.startswith("norm_")}
+        m.bands = int(z["bands"]) if "bands" in z.files else 0
         return m
 
 
@@ -199,7 +227,9 @@ def apply_feat_norm(feat, norm):
 
 
 def fit(mlp: MLP, feat, targ, steps=2000, batch=8192, lr=2e-3, lr_end=1e-4, seed=0, log_every=200, log=print):
-    """Train on (feat, targ) (raw, as layer_optics returns them).  Returns the loss history."""
+    """Train on (feat, targ) (raw, as layer_optics returns them; the ln tau targets are clamped in
+    place).  Returns the loss history."""
+    clamp_targets(targ)
     norm = normalisation(feat, targ)
     mlp.norm = norm
     X = apply_feat_norm(feat, norm)
