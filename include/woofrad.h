@@ -403,6 +403,28 @@ int wr_mlp_to_bf16(const float *a, void *b, int64_t n, void *stream);
  * wr_mlp_gemm layers (bias + SiLU, bias + SiLU, bias) with a BF16 output.  Weights 16-byte aligned. */
 int wr_mlp_forward(const wr_surrogate *w, int64_t M, const float *X, void *Y, int64_t ldy, void *stream);
 
+/* ---- served tiles (no counterpart in the reference: the serving loop of this build) -------------
+ * A slot owns, for one context of max_cols (= tile) columns: a stream of its own (default
+ * priority), staging buffers for one tile's inputs and outputs (the qc / qi form of wr_columns,
+ * FP64), the tile's column ids, McICA seed and valid-count words, a completion event and, unless
+ * graphs is 0, a HIP graph of a whole-tile call captured at creation.  precision: 0 FP64, 1 mixed,
+ * 2 surrogate (the context must have one attached).  filler: columns (at least 1, nlay layers)
+ * repeated over the staging tile at creation, so a replay never computes on uninitialised memory.
+ * wr_slot_launch: columns [c0, c0 + n) of a request (req: device arrays [k][req->ncol], col_id
+ * optional) through the slot: one gather kernel (inputs, ids, seed, count), the graph replay (or the
+ * call), one scatter kernel into res ([k][req->ncol]; fsw_dn_sfc_clr optional), the event --
+ * asynchronous, nothing allocated.  wr_slot_query: 1 when the last launch has finished, 0 while it
+ * runs, negative on an error; wr_slot_sync waits for it.  One slot runs one tile at a time; the
+ * context must outlive the slot. */
+typedef struct wr_slot wr_slot;
+int wr_slot_create(wr_slot **slot, wr_ctx *ctx, int32_t precision, const wr_params *par, int32_t graphs,
+                   const wr_columns *filler);
+int wr_slot_launch(wr_slot *slot, const wr_columns *req, const wr_outputs *res, int32_t c0, int32_t n,
+                   uint64_t seed);
+int wr_slot_query(wr_slot *slot);
+int wr_slot_sync(wr_slot *slot);
+void wr_slot_destroy(wr_slot *slot);
+
 /* Per-launch kernel timing for measurement (no counterpart in the reference: the reference
  * publishes call-level times only).  wr_timing(ctx, 1) creates an event pool once and brackets
  * every later kernel launch of the context with HIP events on the launching stream (up to 8192

@@ -19,7 +19,8 @@ EXPORTS = ("wr_default_params", "wr_create", "wr_context_bytes", "wr_run_fp64", 
            "wr_mcica_masks", "wr_timing", "wr_timing_read", "wr_kernel_name", "wr_strerror", "wr_destroy",
            "wr_layer_optics", "wr_mlp_gemm", "wr_mlp_splitk_reduce", "wr_mlp_colsum", "wr_mlp_mse", "wr_mlp_adam",
            "wr_mlp_to_bf16", "wr_surrogate_attach", "wr_run_surrogate", "wr_nmmb_prep", "wr_nmmb_post", "wr_nmmb_albedo",
-           "wr_sol_update", "wr_coszmn", "wr_rdtemp", "wr_aer_optics", "wr_mlp_forward")
+           "wr_sol_update", "wr_coszmn", "wr_rdtemp", "wr_aer_optics", "wr_mlp_forward",
+           "wr_slot_create", "wr_slot_launch", "wr_slot_query", "wr_slot_sync", "wr_slot_destroy")
 
 _ERRORS = {
     abi.WR_ERR_ARG: ValueError,
@@ -98,6 +99,17 @@ def load():
     L.wr_aer_optics.restype = C.c_int
     L.wr_mlp_forward.argtypes = [C.POINTER(abi.WrSurrogate), C.c_int64, vp, vp, C.c_int64, vp]
     L.wr_mlp_forward.restype = C.c_int
+    L.wr_slot_create.argtypes = [C.POINTER(vp), vp, C.c_int32, C.POINTER(abi.WrParams), C.c_int32,
+                                 C.POINTER(abi.WrColumns)]
+    L.wr_slot_create.restype = C.c_int
+    L.wr_slot_launch.argtypes = [vp, C.POINTER(abi.WrColumns), C.POINTER(abi.WrOutputs), C.c_int32, C.c_int32,
+                                 C.c_uint64]
+    L.wr_slot_launch.restype = C.c_int
+    for f in ("wr_slot_query", "wr_slot_sync"):
+        getattr(L, f).argtypes = [vp]
+        getattr(L, f).restype = C.c_int
+    L.wr_slot_destroy.argtypes = [vp]
+    L.wr_slot_destroy.restype = None
     i32, f64 = C.c_int32, C.c_double
     L.wr_sol_update.argtypes = [i32] * 6 + [f64, f64, f64, C.POINTER(abi.WrSol)]
     L.wr_sol_update.restype = C.c_int

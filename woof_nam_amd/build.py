@@ -34,7 +34,8 @@ LIB = PKG / "libwoofrad.so"
 OBJ = PKG / "_build"
 ARCH = os.environ.get("WR_OFFLOAD_ARCH", "gfx950")
 
-UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_mixed.hip", "wr_mlp.hip", "wr_surr.hip", "wr_nmmb.hip", "wr_solar.hip", "wr_aer.hip"]
+UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_mixed.hip", "wr_mlp.hip", "wr_surr.hip", "wr_nmmb.hip", "wr_solar.hip", "wr_aer.hip",
+         "wr_serve.hip"]
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",

@@ -10,11 +10,13 @@ seeds) arrive at any time.  The server
   while others run join the tile queue at once: continuous batching at tile granularity);
 * runs each tile as one replay of a HIP graph captured once per slot (the kernels of a tile:
   column scalars, day list, 16 LW and 14 SW band kernels spread over the context's band streams,
-  post; with precision "surrogate" also the layer features and the network's three MFMA layers,
-  with the table fallback of columns outside the training envelope) over the slot's staging buffers; a tile is gathered into the staging buffers, the graph
-  replayed and the outputs scattered back, all on the slot's stream, with the request's seed and
-  the tile's column count read by the kernels from device memory (a short last tile costs only
-  its own columns);
+  post; with precision "surrogate" also the layer features and the network's inference kernel,
+  with the table fallback of columns outside the training envelope) over the slot's staging buffers.
+  The slots are the library's (wr_slot_*): a tile is one library call that gathers it into the
+  staging buffers with the request's seed and the tile's column count (one kernel), replays the
+  graph, scatters the outputs back (one kernel) and records the completion event, all on the
+  slot's stream (a short last tile costs only its own columns); no PyTorch operation is on the
+  per-tile path;
 * records per-request latency (arrival to last tile done) for p50/p99.
 
 Every column's result depends only on its own inputs, its column id and its request's seed, so a
@@ -32,8 +34,10 @@ from dataclasses import dataclass, field
 import numpy as np
 import torch
 
-from . import abi
-from .radiation import Radiation, input_shapes
+import ctypes as C
+
+from . import _lib, abi
+from .radiation import Radiation, _colstruct, _dptr, input_shapes
 
 BYTES_PER_COLUMN = (6 * 60 + 61 + 4 + 2 * 60 + 4 * 62) * 8   # FP64 inputs + outputs at 60 layers
 
@@ -63,63 +67,54 @@ class Request:
         return None if self.done is None else self.done - self.arrival
 
 
+PRECISION_CODE = {"fp64": 0, "mixed": 1, "surrogate": 2}
+
+
 class _Slot:
+    """A library slot (wr_slot_*): the context, its stream, staging buffers, the captured graph of a
+    whole-tile call and the completion event all live in the library; a tile costs one call here."""
+
     def __init__(self, device, tile, nlay, precision, params, graphs, filler, surrogate=None):
         self.rad = Radiation(device, max_cols=tile)
         if surrogate is not None:        # the network's inference is captured into the slot's graph
             self.rad.attach_surrogate(surrogate)
-        self.stream = torch.cuda.Stream(device)           # default priority
         self.tile = tile
-        self.precision = precision
-        self.params = params
-        shapes = input_shapes(tile, nlay)
-        self.inp = {k: torch.empty(s, dtype=torch.float64, device=device) for k, s in shapes.items()}
-        for k in self.inp:                               # a valid column set in every padding column
-            self.inp[k].copy_(filler[k])
-        self.inp["col_id"] = torch.arange(tile, dtype=torch.int64, device=device)
-        self.inp["seed_dev"] = torch.zeros(1, dtype=torch.int64, device=device)
-        self.inp["ncol_dev"] = torch.full((1,), tile, dtype=torch.int32, device=device)   # valid columns
-        self.out = {k: torch.empty(s, dtype=torch.float64, device=device)
-                    for k, s in abi.output_shapes(tile, nlay).items()}
-        self.event = torch.cuda.Event()
+        fill = {k: filler[k][..., :min(tile, int(filler[k].shape[-1]))].contiguous() for k in abi.INPUT_FIELDS}
+        nf = int(fill["t_lay"].shape[1])
+        ci = _colstruct(fill, nf, nlay, self.rad.device)
+        h = C.c_void_p()
+        _lib.check(_lib.load().wr_slot_create(C.byref(h), self.rad._h, PRECISION_CODE[precision], C.byref(params),
+                                              int(bool(graphs)), C.byref(ci)), "wr_slot_create")
+        self._h = h                      # the filler was copied and the graph captured inside the call
         self.work = None
-        self.graph = None
-        torch.cuda.synchronize(device)
-        if graphs:
-            with torch.cuda.stream(self.stream):
-                self.rad.run(self.inp, self.params, self.precision, out=self.out)   # warm-up before capture
-            self.stream.synchronize()
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g, stream=self.stream, capture_error_mode="relaxed"):
-                self.rad.run(self.inp, self.params, self.precision, out=self.out)
-            self.stream.synchronize()
-            self.graph = g
 
     def launch(self, req: Request, c0: int, n: int):
-        with torch.cuda.stream(self.stream):
-            sl = slice(c0, c0 + n)
-            for k in abi.INPUT_FIELDS:
-                self.inp[k][..., :n].copy_(req.cols[k][..., sl])
+        abi_args = req.meta.get("_abi")
+        if abi_args is None:             # the request's pointers, built once per request
+            cols = dict(req.cols)
             if req.col_id is not None:
-                self.inp["col_id"][:n].copy_(req.col_id[sl])
-            else:
-                self.inp["col_id"][:n].copy_(torch.arange(c0, c0 + n, dtype=torch.int64, device=self.inp["col_id"].device))
-            self.inp["seed_dev"].fill_(int(np.int64(np.uint64(req.seed))))
-            self.inp["ncol_dev"].fill_(n)                  # the graph skips the tile's unused columns
-            if self.graph is not None:
-                self.graph.replay()
-            else:
-                self.rad.run(self.inp, self.params, self.precision, out=self.out)
-            for k in abi.OUTPUT_FIELDS:
-                req.out[k][..., sl].copy_(self.out[k][..., :n])
-            self.event.record(self.stream)
+                cols["col_id"] = req.col_id
+            ci = _colstruct(cols, req.ncol, int(req.cols["t_lay"].shape[0]), self.rad.device)
+            co = abi.WrOutputs(*[_dptr(req.out[k]) for k in abi.OUTPUT_FIELDS])
+            abi_args = req.meta["_abi"] = (ci, co)
+        ci, co = abi_args
+        rc = _lib.load().wr_slot_launch(self._h, C.byref(ci), C.byref(co), int(c0), int(n),
+                                        int(req.seed) & 0xFFFFFFFFFFFFFFFF)
+        _lib.check(rc, "wr_slot_launch")
         self.work = req
 
     def ready(self) -> bool:
-        return self.work is None or self.event.query()
+        if self.work is None:
+            return True
+        r = _lib.load().wr_slot_query(self._h)
+        if r < 0:
+            _lib.check(-r, "wr_slot_query")
+        return r == 1
 
     def close(self):
-        self.graph = None
+        if self._h:
+            _lib.load().wr_slot_destroy(self._h)
+            self._h = None
         self.rad.close()
 
 
@@ -137,9 +132,9 @@ class Server:
         self.tile, self.nslots, self.budget = int(tile), int(slots), float(budget_gb) * 1e9
         self.precision, self.nlay = precision, nlay
         self.params = params or Radiation.default_params()
-        if filler is None:
+        if filler is None:               # a valid column set, repeated over each slot's staging tile
             from . import gen
-            cols = gen.columns(self.tile, seed=12345)
+            cols = gen.columns(min(self.tile, 8192), seed=12345)
             filler = {k: torch.from_numpy(np.ascontiguousarray(v)).to(self.device) for k, v in cols.items()}
         self.slots = [_Slot(self.device, self.tile, nlay, precision, self.params, graphs, filler,
                             surrogate if precision == "surrogate" else None)
@@ -189,7 +184,7 @@ class Server:
     def poll(self) -> int:
         """Retire finished tiles, admit what fits, start tiles on free slots.  Returns tiles started."""
         for s in self.slots:
-            if s.work is not None and s.event.query():
+            if s.work is not None and s.ready():
                 req = s.work
                 s.work = None
                 req.tiles_left -= 1
@@ -304,9 +299,7 @@ def autotune(device: int = 0, tiles=(32768, 65536, 131072), slots=(1, 2, 4), nco
     cuts = [0] + [int(x) for x in np.round(np.cumsum(w) / w.sum() * ncol)]
     results = []
     for tile, ns in cand:
-        srv = Server(device, tile=tile, slots=ns, precision=precision, surrogate=surrogate,
-                     filler={k: v[..., :tile] if v.shape[-1] >= tile else v[..., :1].expand(*v.shape[:-1], tile)
-                             for k, v in dev.items()})
+        srv = Server(device, tile=tile, slots=ns, precision=precision, surrogate=surrogate, filler=dev)
         best = None
         for rep in range(2):
             reqs = [Request(rid=j, kind="tune", cols={k: v[..., cuts[j]:cuts[j + 1]].contiguous() for k, v in dev.items()},
