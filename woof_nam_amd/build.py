@@ -34,8 +34,8 @@ LIB = PKG / "libwoofrad.so"
 OBJ = PKG / "_build"
 ARCH = os.environ.get("WR_OFFLOAD_ARCH", "gfx950")
 
-UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_mixed.hip", "wr_mlp.hip", "wr_surr.hip", "wr_nmmb.hip", "wr_solar.hip", "wr_aer.hip",
-         "wr_serve.hip"]
+UNITS = ["wr_host.hip", "wr_fp64.hip", "wr_fp64_ext.hip", "wr_mixed.hip", "wr_mixed_ext.hip", "wr_mlp.hip", "wr_surr.hip",
+         "wr_nmmb.hip", "wr_solar.hip", "wr_aer.hip", "wr_serve.hip"]
 CFLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
@@ -58,6 +58,7 @@ def hipcc() -> str:
 _FP32_UNIT = ["-ffp-contract=fast", "-fno-hip-fp32-correctly-rounded-divide-sqrt"]
 UNIT_FLAGS = {
     "wr_mixed.hip": _FP32_UNIT,
+    "wr_mixed_ext.hip": _FP32_UNIT,
     "wr_surr.hip": _FP32_UNIT,
 }
 

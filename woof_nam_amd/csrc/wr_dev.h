@@ -114,6 +114,23 @@ struct ColIn {
     int nlay = 0;  // model layers
 };
 
+// True when a call carries any of the optional inputs (path-form clouds, aerosol optics, the added
+// top layer's ozone).
+inline bool has_ext(const ColIn &in) { return in.cwp || in.aer_sw || in.aer_lw || in.o3_top; }
+
+// The column inputs as a band kernel reads them.  EXT = false nulls the optional inputs at compile
+// time, so their tests fold away and the kernel's registers carry none of their terms; such an
+// instance computes what the generic one (EXT = true) computes when those pointers are null.
+template <bool EXT> __device__ __forceinline__ ColIn band_in(ColIn in)
+{
+    if constexpr (!EXT) {
+        in.cwp = in.cip = in.crp = in.rew = in.rei = nullptr;
+        in.aer_sw = in.aer_lw = nullptr;
+        in.o3_top = nullptr;
+    }
+    return in;
+}
+
 struct RunPar {
     int mcica, iovr;
     uint64_t seed;

@@ -38,8 +38,9 @@ int wr_init_luts(wr_ctx *c, hipStream_t s)
 int wr_launch_fp64(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs &out, int col0, int n, int band,
                    hipStream_t s)
 {
-    if (par.mcica == 0) return launch_tile_nam<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
-    return launch_tile<double>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
+    if (has_ext(in)) return wr_launch_fp64_ext(c, in, par, out, col0, n, band, s);
+    if (par.mcica == 0) return launch_tile_nam<double, false>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
+    return launch_tile<double, false, false>(c, c->tabs_d, c->sflux_d, in, par, out, col0, n, band, s);
 }
 
 int wr_launch_diag_fp64(wr_ctx *c, const ColIn &in, const RunPar &par, const wr_diag *dg, int col0, int n, hipStream_t s)

@@ -109,6 +109,12 @@ int wr_launch_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const 
                    int band, hipStream_t s);
 int wr_launch_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
                     int band, hipStream_t s);
+// the same tiles with the band kernels that read the optional column inputs (path-form clouds, aerosol,
+// top-layer ozone); their own units, so both instance sets compile in parallel
+int wr_launch_fp64_ext(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
+                       int band, hipStream_t s);
+int wr_launch_mixed_ext(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr::OutPtrs &out, int col0, int n,
+                        int band, hipStream_t s);
 int wr_launch_diag_fp64(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,
                         hipStream_t s);
 int wr_launch_diag_mixed(wr_ctx *c, const wr::ColIn &in, const wr::RunPar &par, const wr_diag *dg, int col0, int n,

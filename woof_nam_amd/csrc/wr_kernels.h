@@ -477,10 +477,11 @@ __device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, cons
     }
 }
 
-template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, AccT<R> acc, Surr sg)
+template <typename R, int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                           int n, ColScal cs, AccT<R> acc, Surr sg)
 {
+    const ColIn in = band_in<EXT>(in_);
     constexpr int GS = lw_gsplit<R>();
     constexpr int NG = LW_NG[B];
     constexpr int NGL = NG / GS;
@@ -761,11 +762,12 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
     }
 }
 
-template <int B, bool SURR = false>
-__global__ void __launch_bounds__(64, lwc_waves<B>()) k_lwc(const Tabs<float> *__restrict__ T, ColIn in, RunPar par,
+template <int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__(64, lwc_waves<B>()) k_lwc(const Tabs<float> *__restrict__ T, ColIn in_, RunPar par,
                                                          int col0, int n, ColScal cs, AccT<float> acc, Surr sg,
                                                          char *__restrict__ pool)
 {
+    const ColIn in = band_in<EXT>(in_);
     const LaneCache<float> cc = lane_cache<float>(pool, LW_NV_F32, LW_NG[B], in.nlay + 2);
     for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
         const int t = blk * 64 + threadIdx.x;
@@ -988,10 +990,12 @@ template <typename R, int B, bool NAM = false> constexpr int sw_waves()
 }
 
 // SW band kernel: blocks of 64 day-list entries, a grid no larger than the pool's slot count.
-template <typename R, int B, bool SURR = false>
-__global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                           ColScal cs, const R *__restrict__ sflux, AccT<R> acc, Surr sg, char *__restrict__ pool)
+template <typename R, int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                           int n, ColScal cs, const R *__restrict__ sflux, AccT<R> acc, Surr sg,
+                                           char *__restrict__ pool)
 {
+    const ColIn in = band_in<EXT>(in_);
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
     const int nday = *cs.nday;
     for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {

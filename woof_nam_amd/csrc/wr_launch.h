@@ -108,11 +108,11 @@ inline unsigned band_grid(const BandJob<R> &j, int nv, int ng)
     return (unsigned)(nblk < nslot ? nblk : nslot);
 }
 
-template <typename R, bool SURR, int B = 0>
+template <typename R, bool SURR, bool EXT, int B = 0>
 void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
-        if (id != B) return launch_band<R, SURR, B + 1>(j, id, s, pool);
+        if (id != B) return launch_band<R, SURR, EXT, B + 1>(j, id, s, pool);
         const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16) {
@@ -125,31 +125,31 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             }
         }
         if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1)
-            hipLaunchKernelGGL((k_lwc<B, SURR>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
+            hipLaunchKernelGGL((k_lwc<B, SURR, EXT>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg, pool);
         else if constexpr (B < 16)
-            hipLaunchKernelGGL((k_lw<R, B, SURR>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
+            hipLaunchKernelGGL((k_lw<R, B, SURR, EXT>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg);
         else
-            hipLaunchKernelGGL((k_sw<R, B - 16, SURR>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+            hipLaunchKernelGGL((k_sw<R, B - 16, SURR, EXT>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, j.sg, pool);
         tend(j.c, s);
     }
 }
 
 // NAM route band kernels (k_lwr / k_swr).
-template <typename R, int B = 0>
+template <typename R, bool EXT, int B = 0>
 void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
 {
     if constexpr (B < 30) {
-        if (id != B) return launch_band_nam<R, B + 1>(j, id, s, pool);
+        if (id != B) return launch_band_nam<R, EXT, B + 1>(j, id, s, pool);
         const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
         if constexpr (B < 16)
-            hipLaunchKernelGGL((k_lwr<R, B>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
+            hipLaunchKernelGGL((k_lwr<R, B, EXT>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
                                j.cs, acc, j.rt);
         else
-            hipLaunchKernelGGL((k_swr<R, B - 16>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
+            hipLaunchKernelGGL((k_swr<R, B - 16, EXT>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
         tend(j.c, s);
     }
@@ -197,8 +197,9 @@ int run_bands(wr_ctx *c, int band, hipStream_t s, F &&one)
 }
 
 // One tile of n columns from col0 (McICA route): column scalars, day list, band kernels, FP64 post.
-// No host synchronisation.
-template <typename R, bool SURR = false>
+// No host synchronisation.  EXT selects the band kernels that read the optional column inputs (the
+// caller passes EXT = has_ext(in) or true).
+template <typename R, bool SURR = false, bool EXT = true>
 int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const RunPar &par, const OutPtrs &out,
                 int col0, int n, int band, hipStream_t s, const Surr &sg = Surr{})
 {
@@ -216,7 +217,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
         hipLaunchKernelGGL((k_lwrec<R>), dim3((n + 63) / 64), dim3(64), 0, s, tabs, in, par, col0, n, j.cs, j.rec);
         tend(c, s);
     }
-    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR>(j, id, q, pool); });
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band<R, SURR, EXT>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
     hipLaunchKernelGGL(k_post<R>, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);
@@ -226,7 +227,7 @@ int launch_tile(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const
 
 // The NAM route (mcica = 0): overlap factors once per tile, then rtrnmr and cldfrc-mixed spcvrtc
 // band kernels.
-template <typename R>
+template <typename R, bool EXT = true>
 int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, const RunPar &par, const OutPtrs &out,
                     int col0, int n, int band, hipStream_t s)
 {
@@ -246,7 +247,7 @@ int launch_tile_nam(wr_ctx *c, const Tabs<R> *tabs, R *sflux, const ColIn &in, c
     tbeg(c, KID_RTRFAC, s);
     hipLaunchKernelGGL((k_rtr_fac<R>), dim3((n + 63) / 64), dim3(64), 0, s, in, col0, n, rt);
     tend(c, s);
-    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band_nam<R>(j, id, q, pool); });
+    int rc = run_bands(c, band, s, [&](int id, hipStream_t q, char *pool) { launch_band_nam<R, EXT>(j, id, q, pool); });
     if (rc != WR_OK) return rc;
     tbeg(c, KID_POST, s);
     hipLaunchKernelGGL(k_post<R>, dim3((n + 255) / 256), dim3(256), 0, s, in, col0, n, j.cs, j.ba, out);

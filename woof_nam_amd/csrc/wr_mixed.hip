@@ -10,8 +10,9 @@ using namespace wr;
 int wr_launch_mixed(wr_ctx *c, const ColIn &in, const RunPar &par, const OutPtrs &out, int col0, int n, int band,
                    hipStream_t s)
 {
-    if (par.mcica == 0) return launch_tile_nam<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
-    return launch_tile<float>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
+    if (has_ext(in)) return wr_launch_mixed_ext(c, in, par, out, col0, n, band, s);
+    if (par.mcica == 0) return launch_tile_nam<float, false>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
+    return launch_tile<float, false, false>(c, c->tabs_f, c->sflux_f, in, par, out, col0, n, band, s);
 }
 
 int wr_launch_diag_mixed(wr_ctx *c, const ColIn &in, const RunPar &par, const wr_diag *dg, int col0, int n, hipStream_t s)

@@ -193,10 +193,11 @@ template <typename R, int B> constexpr int lwr_chunk()
     return chunk_of(LW_NG[B], WR_LWR_CHUNK >= 0 ? WR_LWR_CHUNK : (sizeof(R) == 8 ? LWR64_CHUNK[B] : LWR32_CHUNK[B]));
 }
 
-template <typename R, int B>
-__global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, AccT<R> acc, Rtr<R> rt)
+template <typename R, int B, bool EXT = true>
+__global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                            int n, ColScal cs, AccT<R> acc, Rtr<R> rt)
 {
+    const ColIn in = band_in<EXT>(in_);
     constexpr int NG = LW_NG[B];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= valid_cols(in, col0, n)) return;
@@ -524,10 +525,12 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
 }
 
 // SW band kernel of the NAM route: blocks of 64 day-list entries looped over a pool-sized grid.
-template <typename R, int B>
-__global__ void __launch_bounds__(64, (sw_waves<R, B, true>())) k_swr(const Tabs<R> *__restrict__ T, ColIn in, RunPar par, int col0, int n,
-                                            ColScal cs, const R *__restrict__ sflux, AccT<R> acc, char *__restrict__ pool)
+template <typename R, int B, bool EXT = true>
+__global__ void __launch_bounds__(64, (sw_waves<R, B, true>())) k_swr(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                            int n, ColScal cs, const R *__restrict__ sflux, AccT<R> acc,
+                                            char *__restrict__ pool)
 {
+    const ColIn in = band_in<EXT>(in_);
     const LaneCache<R> cc = lane_cache<R>(pool, sw_nv_mc<R>(), SW_NG[B], in.nlay + 2);
     const int nday = *cs.nday;
     for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
