@@ -86,3 +86,27 @@ def test_mixed_nam_route_within_tolerance(rad):
         assert e[k][0] <= HR_RMS_KDAY, (k, e[k])
     for k in ("flw_up", "flw_dn", "fsw_up", "fsw_dn"):
         assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])
+
+
+@pytest.mark.parametrize("precision,env", [("mixed", "WR_LW_LDS"), ("fp64", "WR_LW_LDS64")])
+def test_lds_staged_lw_kernels_match_global_tables(gpu_device, monkeypatch, precision, env):
+    """The LW kernels with the band tables staged in LDS (k_lwl for FP32, k_lwl64 for FP64) do the
+    arithmetic of the kernels that gather from global memory (k_lwc, k_lw): every output is
+    bit-identical, on grid columns with several 64-column blocks per wave and a partly filled last
+    block."""
+    cols = gen.grid(150, 131, seed=5)  # 19,650 columns
+    print(f"case LDS-staged LW kernels vs global tables, {precision}, grid 150x131", flush=True)
+    dev = {k: torch.from_numpy(v).to(gpu_device) for k, v in cols.items()}
+    outs = {}
+    for mask in ("0", "ffff"):
+        monkeypatch.setenv(env, mask)
+        r = Radiation(0, max_cols=20000)
+        try:
+            out, _ = r.run(dev, precision=precision)
+            torch.cuda.synchronize()
+            outs[mask] = {k: v.cpu().numpy().copy() for k, v in out.items()}
+        finally:
+            r.close()
+    for k, v in outs["0"].items():
+        assert np.array_equal(v.view(np.uint64) if v.dtype == np.float64 else v, outs["ffff"][k].view(np.uint64)
+                              if v.dtype == np.float64 else outs["ffff"][k]), k

@@ -72,11 +72,32 @@ __device__ __forceinline__ LutF lut_f32(int i, bool lw_cap)
 
 
 // ---- tables --------------------------------------------------------------------------------
-template <typename R> struct LwBand {
-    const WR_G R *absa, *absb, *selfref, *forref, *fracrefa, *fracrefb;
-    const WR_G R *ka_m[3], *kb_m[3];
-    const WR_G R *x1, *x2;
+// LDS (the device pass's address space 3), for tables a kernel stages there
+#if defined(__HIP_DEVICE_COMPILE__)
+#define WR_L __attribute__((address_space(3)))
+#else
+#define WR_L
+#endif
+// The tables of one LW band; P is the pointer type (global by default, LDS in the staged kernels),
+// PB that of absb.
+template <typename R, typename P = const WR_G R *, typename PB = P> struct LwBandT {
+    P absa;
+    PB absb;
+    P selfref, forref, fracrefa, fracrefb;
+    P ka_m[3], kb_m[3];
+    P x1, x2;
     R refrat_planck_a, refrat_planck_b, refrat_m_a, refrat_m_b, refrat_m_a3;
+};
+template <typename R> using LwBand = LwBandT<R>;
+// The same tables packed into one image: 14 tables (absa absb selfref forref fracrefa fracrefb
+// ka_m[0..2] kb_m[0..2] x1 x2) at element offsets off[] (multiples of 4, -1 where the image does not
+// hold one), n elements in all (a multiple of 4), from a 256-byte aligned base.  The FP64 image leaves
+// absb out.
+constexpr int LW_IMG_T = 14;
+template <typename R> struct LwImg {
+    const WR_G R *base;
+    int n;
+    int off[LW_IMG_T];
 };
 template <typename R> struct SwBand {
     const WR_G R *absa, *absb, *selfref, *forref;
@@ -95,6 +116,7 @@ template <typename R> struct Tabs {
     R scalekur;
     LwBand<R> lw[16];
     SwBand<R> sw[14];
+    LwImg<R> lw_img[16];
 };
 
 // ---- column inputs ---------------------------------------------------------------------------

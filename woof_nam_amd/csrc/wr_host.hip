@@ -95,6 +95,25 @@ struct Blob {
         }
         return (const WR_G T *)(base + off);
     }
+    // the tables ent[0 .. nt) (null entries skipped) one after another from a 256-byte boundary, each
+    // padded with zeros to a multiple of 4 elements
+    template <typename T> const WR_G T *pack(const TEntry *const *ent, int nt)
+    {
+        const size_t off = align();
+        for (int j = 0; j < nt; ++j) {
+            if (!ent[j]) continue;
+            const size_t at = bytes.size(), cnt = (ent[j]->count + 3) / 4 * 4;
+            bytes.resize(at + cnt * sizeof(T), 0);
+            T *dst = (T *)(bytes.data() + at);
+            if (ent[j]->dtype != 0) {
+                bad = true;
+                continue;
+            }
+            const float *src = (const float *)ent[j]->data;
+            for (uint64_t q = 0; q < ent[j]->count; ++q) dst[q] = (T)src[q];
+        }
+        return (const WR_G T *)(base + off);
+    }
     template <typename T> WR_G T *reserve(size_t n)
     {
         const size_t off = align();
@@ -218,6 +237,26 @@ template <typename R> int bind_tables(const TFile &f, Blob &b, Tabs<R> &T)
         if (s.kind == 1) B.kb_m[s.idx] = p;
         if (s.kind == 2) B.x1 = p;
         if (s.kind == 3) B.x2 = p;
+    }
+    // every LW band's tables packed into one image (k_lwl and k_lwl64 copy it to LDS), in the LwImg
+    // order; the FP64 image leaves absb out
+    for (int k = 0; k < 16; ++k) {
+        static const char *const keys[6] = {"absa", "absb", "selfref", "forref", "fracrefa", "fracrefb"};
+        const TEntry *ent[LW_IMG_T] = {};
+        for (int j = 0; j < 6; ++j)
+            if (j != 1 || sizeof(R) == 4) ent[j] = d.find(bname("lw", k + 1, keys[j], true));
+        for (const Slot &s : slots)
+            if (s.band == k + 1)
+                ent[s.kind == 0 ? 6 + s.idx : s.kind == 1 ? 9 + s.idx : s.kind == 2 ? 12 : 13] =
+                    d.find(bname("lw", s.band, s.key, true));
+        LwImg<R> &im = T.lw_img[k];
+        int n = 0;
+        for (int j = 0; j < LW_IMG_T; ++j) {
+            im.off[j] = ent[j] ? n : -1;
+            if (ent[j]) n += (int)((ent[j]->count + 3) / 4 * 4);
+        }
+        im.n = n;
+        im.base = b.pack<R>(ent, LW_IMG_T);
     }
     // reference ratios of the binary-species bands from chi_mls (taugbNN.cu constants)
     const TEntry *chi = d.find("lw/chi_mls");
@@ -438,6 +477,22 @@ bool lw_glane_env()
     return e && atoi(e) != 0;
 }
 
+// FP32 McICA LW bands (bit B = band B + 1) that run k_lwl, the kernel with the band's tables in LDS:
+// WR_LW_LDS (a hexadecimal mask) overrides the default.  A band whose table image exceeds the
+// device's LDS per workgroup runs k_lwc.
+// WR_LW_LDS64 does the same for the FP64 McICA LW kernels (k_lwl64 in place of k_lw).  The defaults
+// are the measured choice per band on the 954x835x60 grid: every FP32 band (LW sum 101 -> 97 ms per
+// call), every FP64 band but 14 (LW sum 233 -> 216 ms; band 14 ran 7 % slower).
+constexpr uint32_t LW_LDS_DEFAULT = 0xffffu, LW_LDS64_DEFAULT = 0xffffu & ~(1u << 13);
+template <typename R> uint32_t lds_mask(const char *env, uint32_t dflt, const Tabs<R> &t, int lds_max)
+{
+    const char *e = getenv(env);
+    uint32_t m = e ? (uint32_t)strtoul(e, nullptr, 16) : dflt;
+    for (int b = 0; b < 16; ++b)
+        if (t.lw_img[b].n <= 0 || (size_t)t.lw_img[b].n * sizeof(R) > (size_t)lds_max) m &= ~(1u << b);
+    return m & 0xffffu;
+}
+
 // Device bytes of a context's tile work space for tiles of m columns, and the band-stream count.
 struct Sizes {
     size_t acc, iws, dws, rtr, sflux_d, sflux_f, lwrec, pool;
@@ -573,6 +628,11 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
             (z.lwrec && hipMalloc(&c->lwrec, z.lwrec) != hipSuccess))
             rc = WR_ERR_NOMEM;
         c->lw_glane = z.lwrec > 0;
+        if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
+            hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
+            rc = WR_ERR_HIP;
+        c->lw_lds = lds_mask<float>("WR_LW_LDS", LW_LDS_DEFAULT, c->host_f, c->lds_max);
+        c->lw_lds64 = lds_mask<double>("WR_LW_LDS64", LW_LDS64_DEFAULT, c->host_d, c->lds_max);
     }
     if (rc == WR_OK) {
         c->pool_bytes = z.pool;

@@ -28,6 +28,9 @@ struct wr_ctx {
     float *sflux_f;
     void *lwrec;                     // LW layer records of a tile (g-lane LW kernels), sized for FP64
     int lw_glane;                    // 1: McICA LW bands as g-lane kernels over the layer records
+    uint32_t lw_lds;                 // bit B: the FP32 McICA kernel of LW band B + 1 stages its tables in LDS
+    uint32_t lw_lds64;               // the same for the FP64 McICA kernels
+    int n_cu, lds_max;               // compute units; LDS bytes per workgroup
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
     int has_sur;
     wr_surrogate sur;

@@ -423,14 +423,16 @@ template <int GS, typename R> __device__ __forceinline__ R gsum(R s)
 // and chunking bands 1, 2, 10, 11 only lowered the loads in flight.  The two-g-point bands 15 and 16
 // run 11-19 % faster at a three-wave bound (6-10 spilled registers).
 // FP64 recompute kernel (k_lw<double>): two-g-point chunks bring bands 1, 2, 6, 11 to two waves
-// without spills (26-33 % faster); bands 8 and 12 gain 6-10 % from a two-wave bound with spills.
+// without spills (26-33 % faster; band 1 needs the bound as well since the column body became a
+// function shared with the LDS-staged kernel); bands 8 and 12 gain 6-10 % from a two-wave bound
+// with spills.
 // The other bands stay at one wave unchunked: chunked at one wave they lost half their speed (fewer
 // gathers in flight, no second wave to cover the waits).
 // WR_LWC_WAVES / WR_LW64_WAVES > 0 or WR_LW_CHUNK / WR_LW64_CHUNK >= 0 override the tables for every
 // band (tuning builds).
 constexpr int LWC_WAVES[16] = {1, 1, 2, 2, 2, 1, 2, 1, 2, 1, 1, 1, 1, 1, 3, 3};
 constexpr int LWC_CHUNK[16] = {0, 0, 4, 4, 4, 4, 4, 4, 4, 0, 0, 4, 0, 0, 0, 0};
-constexpr int LW64_WAVES[16] = {1, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 2, 1, 1, 1, 1};
+constexpr int LW64_WAVES[16] = {2, 1, 1, 1, 1, 1, 1, 2, 1, 1, 1, 2, 1, 1, 1, 1};
 constexpr int LW64_CHUNK[16] = {2, 2, 0, 0, 0, 2, 0, 4, 0, 0, 2, 2, 0, 0, 0, 0};
 #ifndef WR_LWC_WAVES
 #define WR_LWC_WAVES 0
@@ -462,32 +464,38 @@ template <typename R, int B, int NGL> constexpr int lw_chunk_r()
 {
     return sizeof(R) == 8 ? chunk_of(NGL, WR_LW64_CHUNK >= 0 ? WR_LW64_CHUNK : LW64_CHUNK[B]) : NGL;
 }
-// lw_optics of g-points g0 .. g0 + NGL - 1 in chunks of CH
-template <typename R, int B, int NGL, int CH>
-__device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0)
+// lw_optics of g-points g0 .. g0 + NGL - 1 in chunks of CH, from the band tables Bt
+template <typename R, int B, int NGL, int CH, typename BT>
+__device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, const BT &Bt, const LwLay<R> &L, R *tau, R *fr,
+                                             int g0)
 {
     if constexpr (CH >= NGL) {
-        lw_optics<R, B, NGL>(T, L, tau, fr, g0);
+        lw_optics_t<R, B, NGL>(T, Bt, L, tau, fr, g0);
     } else {
 #pragma unroll
         for (int c = 0; c < NGL; c += CH) {
-            lw_optics<R, B, CH>(T, L, tau + c, fr + c, g0 + c);
+            lw_optics_t<R, B, CH>(T, Bt, L, tau + c, fr + c, g0 + c);
             __builtin_amdgcn_sched_barrier(0);
         }
     }
 }
-
-template <typename R, int B, bool SURR = false, bool EXT = true>
-__global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
-                                           int n, ColScal cs, AccT<R> acc, Surr sg)
+template <typename R, int B, int NGL, int CH>
+__device__ __forceinline__ void lw_optics_ch(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0)
 {
-    const ColIn in = band_in<EXT>(in_);
+    lw_optics_ch<R, B, NGL, CH>(T, T->lw[B], L, tau, fr, g0);
+}
+
+// One column of LW band B (GS lanes per column; gt = the lane's index over the tile), optics from the
+// band tables Bt: down sweep, surface, up sweep with the optics recomputed.
+template <typename R, int B, bool SURR, typename BT>
+__device__ __forceinline__ void lw_column(const Tabs<R> *__restrict__ T, const BT &Bt, const ColIn &in, const RunPar &par,
+                                          int col0, int n, const ColScal &cs, const AccT<R> &acc, const Surr &sg, int gt)
+{
     constexpr int GS = lw_gsplit<R>();
     constexpr int NG = LW_NG[B];
     constexpr int NGL = NG / GS;
     static_assert(NG % GS == 0, "g-points split evenly over the lanes of a column");
     constexpr int NS = LW_NS[B];
-    const int gt = blockIdx.x * blockDim.x + threadIdx.x;
     const int t = gt / GS;
     const int g0 = (gt % GS) * NGL;
     const bool lead = (gt % GS) == 0;  // writes the column's level fluxes
@@ -543,7 +551,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, L, tau, fr, g0);
+            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, Bt, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
@@ -608,7 +616,7 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
             surr_lw<R, B, NGL>(sg, (size_t)k * n + t, tau, fr, g0);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, L, tau, fr, g0);
+            lw_optics_ch<R, B, NGL, lw_chunk_r<R, B, NGL>()>(T, Bt, L, tau, fr, g0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
@@ -654,6 +662,62 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
     }
 }
 
+template <typename R, int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                           int n, ColScal cs, AccT<R> acc, Surr sg)
+{
+    const ColIn in = band_in<EXT>(in_);
+    lw_column<R, B, SURR>(T, T->lw[B], in, par, col0, n, cs, acc, sg, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// The band tables of an LDS image (the LwImg order), absb from PB: the image's when the image holds it,
+// the global table otherwise.  A table the band does not have points at the image start and is never
+// read.  The offsets are multiples of 4 elements (the & ~3 states it, so that 4 g-points of a row
+// load as one 16-byte read).
+template <typename R, typename PB>
+__device__ __forceinline__ LwBandT<R, const WR_L R *, PB> lds_band(const WR_L R *lt, const LwImg<R> &im, const LwBand<R> &G,
+                                                                   PB absb)
+{
+    LwBandT<R, const WR_L R *, PB> Bl;
+    auto at = [&](int j) { return lt + (imax(0, im.off[j]) & ~3); };
+    Bl.absa = at(0), Bl.absb = absb, Bl.selfref = at(2), Bl.forref = at(3), Bl.fracrefa = at(4), Bl.fracrefb = at(5);
+    Bl.ka_m[0] = at(6), Bl.ka_m[1] = at(7), Bl.ka_m[2] = at(8);
+    Bl.kb_m[0] = at(9), Bl.kb_m[1] = at(10), Bl.kb_m[2] = at(11);
+    Bl.x1 = at(12), Bl.x2 = at(13);
+    Bl.refrat_planck_a = G.refrat_planck_a, Bl.refrat_planck_b = G.refrat_planck_b;
+    Bl.refrat_m_a = G.refrat_m_a, Bl.refrat_m_b = G.refrat_m_b, Bl.refrat_m_a3 = G.refrat_m_a3;
+    return Bl;
+}
+// copies the image into the workgroup's LDS (16-byte words) and waits for the whole workgroup
+template <typename R> __device__ __forceinline__ void lds_fill(float4 *dst, const LwImg<R> &im)
+{
+    const WR_G float4 *src = (const WR_G float4 *)im.base;
+    const int n4 = (int)((size_t)im.n * sizeof(R) / 16);
+    for (int q = threadIdx.x; q < n4; q += blockDim.x) dst[q] = src[q];
+    __syncthreads();
+}
+
+// k_lw (FP64, one lane per column) with the band's tables other than absb in LDS (absb, the tables of
+// the layers above laytrop, stays in global memory: the FP64 image of every band then fits in LDS).
+// Persistent workgroups of 4 x lw_waves waves, as many as the CUs hold; each copies the image into LDS
+// once and its waves then run 64-column blocks.  Same arithmetic as k_lw.
+template <typename R, int B> constexpr int lwl64_wpg() { return 4 * lw_waves<R, B>(); }
+template <int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__((64 * lwl64_wpg<double, B>()), (lw_waves<double, B>()))
+    k_lwl64(const Tabs<double> *__restrict__ T, ColIn in_, RunPar par, int col0, int n, ColScal cs, AccT<double> acc, Surr sg)
+{
+    static_assert(lw_gsplit<double>() == 1, "one lane per column");
+    extern __shared__ float4 lwl64_tab[];
+    const ColIn in = band_in<EXT>(in_);
+    const LwImg<double> &im = T->lw_img[B];
+    lds_fill<double>(lwl64_tab, im);
+    const auto Bl = lds_band<double>((const WR_L double *)lwl64_tab, im, T->lw[B], T->lw[B].absb);
+    constexpr int WPG = lwl64_wpg<double, B>();
+    const int lane = threadIdx.x % 64;
+    for (int blk = blockIdx.x * WPG + threadIdx.x / 64; blk * 64 < n; blk += gridDim.x * WPG)
+        lw_column<double, B, SURR>(T, Bl, in, par, col0, n, cs, acc, sg, blk * 64 + lane);
+}
+
 // FP32 LW band kernel with a cached upward sweep.  The down sweep also forms, per layer and g-point,
 // the upward transmittance a = trng * efclrfr and source b (gas plus cloud term) and keeps them in the
 // lane's pool slot (8 bytes per layer and g-point); the up sweep is then rad = rad * a + b, with no
@@ -661,10 +725,10 @@ __global__ void __launch_bounds__(64, (lw_waves<R, B>())) k_lw(const Tabs<R> *__
 // operation order, and twice the bytes would not pay.
 constexpr int LW_NV_F32 = 2;
 // the smallest divisor of the band's g-point count that is >= the chunk asked for
-template <int B, bool SURR>
-__device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T, const ColIn &in, const RunPar &par,
-                                               int col0, int n, const ColScal &cs, const AccT<float> &acc, const Surr &sg,
-                                               int t, const LaneCache<float> &cc)
+template <int B, bool SURR, typename BT>
+__device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T, const BT &Bt, const ColIn &in,
+                                               const RunPar &par, int col0, int n, const ColScal &cs, const AccT<float> &acc,
+                                               const Surr &sg, int t, const LaneCache<float> &cc)
 {
     using R = float;
     constexpr int NG = LW_NG[B];
@@ -704,7 +768,7 @@ __device__ __forceinline__ void lw_column_f32c(const Tabs<float> *__restrict__ T
             surr_lw<R, B>(sg, (size_t)k * n + t, tau, fr);
         } else {
             lw_layer<R>(T, cur, par, la_c, L);
-            lw_optics_ch<R, B, NG, lw_chunk<B>()>(T, L, tau, fr, 0);
+            lw_optics_ch<R, B, NG, lw_chunk<B>()>(T, Bt, L, tau, fr, 0);
         }
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
@@ -771,7 +835,35 @@ __global__ void __launch_bounds__(64, lwc_waves<B>()) k_lwc(const Tabs<float> *_
     const LaneCache<float> cc = lane_cache<float>(pool, LW_NV_F32, LW_NG[B], in.nlay + 2);
     for (int blk = blockIdx.x; blk * 64 < n; blk += gridDim.x) {
         const int t = blk * 64 + threadIdx.x;
-        if (t < valid_cols(in, col0, n)) lw_column_f32c<B, SURR>(T, in, par, col0, n, cs, acc, sg, t, cc);
+        if (t < valid_cols(in, col0, n)) lw_column_f32c<B, SURR>(T, T->lw[B], in, par, col0, n, cs, acc, sg, t, cc);
+    }
+}
+
+// k_lwc with the band's tables in LDS.  Persistent workgroups of one wave per SIMD times the band's
+// wave bound (LWC_WAVES; 1 where the registers alone set the waves, so a CU then holds several); each
+// copies the band's packed table image (T->lw_img[B], up to about 130 KB) into LDS once, and each wave
+// then runs 64-column blocks with its own pool slot, so the optics' table reads are LDS reads instead
+// of L2/HBM gathers.  Same arithmetic as k_lwc.
+template <int B> constexpr int lwl_wpg() { return 4 * lwc_waves<B>(); }
+template <int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__(64 * lwl_wpg<B>(), lwc_waves<B>())
+    k_lwl(const Tabs<float> *__restrict__ T, ColIn in_, RunPar par, int col0, int n, ColScal cs, AccT<float> acc, Surr sg,
+          char *__restrict__ pool)
+{
+    extern __shared__ float4 lwl_tab[];
+    const ColIn in = band_in<EXT>(in_);
+    const LwImg<float> &im = T->lw_img[B];
+    lds_fill<float>(lwl_tab, im);
+    const WR_L float *lt = (const WR_L float *)lwl_tab;
+    const auto Bl = lds_band<float>(lt, im, T->lw[B], lt + (imax(0, im.off[1]) & ~3));
+    constexpr int WPG = lwl_wpg<B>();
+    const int lane = threadIdx.x % 64, slot = blockIdx.x * WPG + threadIdx.x / 64;
+    LaneCache<float> cc;
+    cc.p = (WR_G float *)(pool + (size_t)slot * cache_slot_bytes(LW_NV_F32, LW_NG[B], in.nlay + 2, 4)) + lane;
+    cc.nv = LW_NV_F32;
+    for (int blk = slot; blk * 64 < n; blk += gridDim.x * WPG) {
+        const int t = blk * 64 + lane;
+        if (t < valid_cols(in, col0, n)) lw_column_f32c<B, SURR>(T, Bl, in, par, col0, n, cs, acc, sg, t, cc);
     }
 }
 

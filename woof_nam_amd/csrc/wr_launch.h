@@ -124,10 +124,55 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
                 return;
             }
         }
-        if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1)
-            hipLaunchKernelGGL((k_lwc<B, SURR, EXT>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs, j.in,
-                               j.par, j.col0, j.n, j.cs, acc, j.sg, pool);
-        else if constexpr (B < 16)
+        if constexpr (B < 16 && sizeof(R) == 4 && lw_gsplit<R>() == 1) {
+            // k_lwl: as many persistent workgroups as the CUs hold at once (the runtime's occupancy for
+            // the kernel's registers and LDS), each wave with its own pool slot; at most as many as the
+            // pool has slots for (none fit: k_lwc)
+            constexpr unsigned WPG = lwl_wpg<B>();
+            const size_t lds = (size_t)j.c->host_f.lw_img[B].n * sizeof(float);
+            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            if (per_cu < 0 && ((j.c->lw_lds >> B) & 1)) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwl<B, SURR, EXT>, 64 * WPG, lds) !=
+                    hipSuccess)
+                    nb = 0;
+                per_cu = nb;
+            }
+            const size_t slot = cache_slot_bytes(LW_NV_F32, LW_NG[B], j.in.nlay + 2, (int)sizeof(R));
+            const size_t nwg_pool = j.c->pool_bytes / slot / WPG;
+            const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            nwg = nwg < nwg_pool ? nwg : nwg_pool;
+            nwg = nwg < nwg_cols ? nwg : nwg_cols;
+            if (((j.c->lw_lds >> B) & 1) && nwg > 0) {
+                hipLaunchKernelGGL((k_lwl<B, SURR, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
+                                   j.col0, j.n, j.cs, acc, j.sg, pool);
+            } else {
+                hipLaunchKernelGGL((k_lwc<B, SURR, EXT>), dim3(band_grid(j, LW_NV_F32, LW_NG[B])), dim3(64), 0, s, j.tabs,
+                                   j.in, j.par, j.col0, j.n, j.cs, acc, j.sg, pool);
+            }
+        } else if constexpr (B < 16 && sizeof(R) == 8 && lw_gsplit<R>() == 1) {
+            // k_lwl64: persistent workgroups, as many as the CUs hold at once
+            constexpr unsigned WPG = lwl64_wpg<R, B>();
+            const size_t lds = (size_t)j.c->host_d.lw_img[B].n * sizeof(double);
+            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            if (per_cu < 0 && ((j.c->lw_lds64 >> B) & 1)) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwl64<B, SURR, EXT>, 64 * WPG, lds) !=
+                    hipSuccess)
+                    nb = 0;
+                per_cu = nb;
+            }
+            const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            nwg = nwg < nwg_cols ? nwg : nwg_cols;
+            if (((j.c->lw_lds64 >> B) & 1) && nwg > 0)
+                hipLaunchKernelGGL((k_lwl64<B, SURR, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
+                                   j.col0, j.n, j.cs, acc, j.sg);
+            else
+                hipLaunchKernelGGL((k_lw<R, B, SURR, EXT>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
+                                   j.n, j.cs, acc, j.sg);
+        } else if constexpr (B < 16)
             hipLaunchKernelGGL((k_lw<R, B, SURR, EXT>), dim3((j.n * lw_gsplit<R>() + 63) / 64), dim3(64), 0, s, j.tabs, j.in,
                                j.par, j.col0, j.n, j.cs, acc, j.sg);
         else

@@ -219,21 +219,21 @@ template <typename R> __device__ __forceinline__ Eta<R> eta(R a, R ratio, R b, R
 #define T2(tab, ind, g, ng) (tab)[((ind)-1) * (ng) + (g)]
 #define T3(tab, m, j, nj, g, ng) (tab)[(((m)-1) * (nj) + ((j)-1)) * (ng) + (g)]
 
-template <typename R>
-__device__ __forceinline__ R maj6(const WR_G R *__restrict__ tab, int ng, int g, const Sel<R> &s, R fa, R fb)
+template <typename R, typename P>
+__device__ __forceinline__ R maj6(P __restrict__ tab, int ng, int g, const Sel<R> &s, R fa, R fb)
 {
     return (s.fk0 * fa) * T2(tab, s.id[0], g, ng) + (s.fk0 * fb) * T2(tab, s.id[1], g, ng) +
            (s.fk1 * fa) * T2(tab, s.id[2], g, ng) + (s.fk1 * fb) * T2(tab, s.id[3], g, ng) +
            (s.fk2 * fa) * T2(tab, s.id[4], g, ng) + (s.fk2 * fb) * T2(tab, s.id[5], g, ng);
 }
-template <typename R>
-__device__ __forceinline__ R maj4(const WR_G R *__restrict__ tab, int ng, int g, int i0, R fs, R fa, R fb)
+template <typename R, typename P>
+__device__ __forceinline__ R maj4(P __restrict__ tab, int ng, int g, int i0, R fs, R fa, R fb)
 {
     return (((R)1 - fs) * fa) * T2(tab, i0, g, ng) + (((R)1 - fs) * fb) * T2(tab, i0 + 5, g, ng) +
            (fs * fa) * T2(tab, i0 + 1, g, ng) + (fs * fb) * T2(tab, i0 + 6, g, ng);
 }
-template <typename R>
-__device__ __forceinline__ R eta_minor(const WR_G R *__restrict__ tab, int nj, int ng, int g, int indm, const Eta<R> &em, R mf)
+template <typename R, typename P>
+__device__ __forceinline__ R eta_minor(P __restrict__ tab, int nj, int ng, int g, int indm, const Eta<R> &em, R mf)
 {
     const R m1 = T3(tab, indm, em.js, nj, g, ng) + em.fs * (T3(tab, indm, em.js + 1, nj, g, ng) - T3(tab, indm, em.js, nj, g, ng));
     const R m2 = T3(tab, indm + 1, em.js, nj, g, ng) +
@@ -241,12 +241,13 @@ __device__ __forceinline__ R eta_minor(const WR_G R *__restrict__ tab, int nj, i
     return m1 + mf * (m2 - m1);
 }
 
-// g-points g0 .. g0 + NGL - 1 of band B (all of them by default); tau/fr hold them from index 0
-template <typename R, int B, int NGL = LW_NG[B]>
-__device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0 = 0)
+// g-points g0 .. g0 + NGL - 1 of band B from the band tables Bt (global or staged in LDS); tau/fr
+// hold them from index 0
+template <typename R, int B, int NGL, typename BT>
+__device__ __forceinline__ void lw_optics_t(const Tabs<R> *__restrict__ T, const BT &Bt, const LwLay<R> &L, R *tau, R *fr,
+                                            int g0)
 {
     constexpr int ng = LW_NG[B];
-    const LwBand<R> &Bt = T->lw[B];
     const int lower = L.lower;
     const int jp = L.jp, jt = L.jt, jt1 = L.jt1;
     const R f00 = L.fac00, f01 = L.fac01, f10 = L.fac10, f11 = L.fac11;
@@ -643,6 +644,12 @@ __device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const L
 #undef ABS4
 #undef FRACA
 #undef FRACB
+}
+// the same from the band's tables in global memory (all g-points by default)
+template <typename R, int B, int NGL = LW_NG[B]>
+__device__ __forceinline__ void lw_optics(const Tabs<R> *__restrict__ T, const LwLay<R> &L, R *tau, R *fr, int g0 = 0)
+{
+    lw_optics_t<R, B, NGL>(T, T->lw[B], L, tau, fr, g0);
 }
 
 }  // namespace wr
