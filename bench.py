@@ -41,22 +41,22 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11658, "k_lw_b02": 11461, "k_lw_b03": 8475, "k_lw_b04": 9285, "k_lw_b05": 9480, "k_lw_b06": 9904,
-    "k_lw_b07": 9036, "k_lw_b08": 11586, "k_lw_b09": 8081, "k_lw_b10": 9179, "k_lw_b11": 9895, "k_lw_b12": 9894,
-    "k_lw_b13": 8157, "k_lw_b14": 7666, "k_lw_b15": 8480, "k_lw_b16": 7495, "k_sw_b16": 21330, "k_sw_b17": 33929,
-    "k_sw_b18": 25289, "k_sw_b19": 25269, "k_sw_b20": 29605, "k_sw_b21": 29622, "k_sw_b22": 12240,
-    "k_sw_b23": 29609, "k_sw_b24": 26232, "k_sw_b25": 22005, "k_sw_b26": 19991, "k_sw_b27": 26217,
-    "k_sw_b28": 22213, "k_sw_b29": 33917,
+    "k_lw_b01": 11691, "k_lw_b02": 11414, "k_lw_b03": 7722, "k_lw_b04": 8571, "k_lw_b05": 8696,
+    "k_lw_b06": 9821, "k_lw_b07": 8445, "k_lw_b08": 11026, "k_lw_b09": 7469, "k_lw_b10": 9097,
+    "k_lw_b11": 9847, "k_lw_b12": 9943, "k_lw_b13": 8006, "k_lw_b14": 7668, "k_lw_b15": 8473, "k_lw_b16": 7479,
+    "k_sw_b16": 21339, "k_sw_b17": 33923, "k_sw_b18": 25290, "k_sw_b19": 25266, "k_sw_b20": 29602,
+    "k_sw_b21": 29627, "k_sw_b22": 12241, "k_sw_b23": 29610, "k_sw_b24": 26232, "k_sw_b25": 21988,
+    "k_sw_b26": 19975, "k_sw_b27": 26218, "k_sw_b28": 22201, "k_sw_b29": 33922,
 }
 
 
 TRAFFIC_MIXED = {
-    "k_lw_b01": 18798, "k_lw_b02": 21083, "k_lw_b03": 29220, "k_lw_b04": 24534, "k_lw_b05": 27984,
-    "k_lw_b06": 15474, "k_lw_b07": 21969, "k_lw_b08": 15961, "k_lw_b09": 21459, "k_lw_b10": 13109,
-    "k_lw_b11": 15509, "k_lw_b12": 15525, "k_lw_b13": 10489, "k_lw_b14": 7274, "k_lw_b15": 7750, "k_lw_b16": 7268,
-    "k_sw_b16": 25803, "k_sw_b17": 48083, "k_sw_b18": 33010, "k_sw_b19": 33010, "k_sw_b20": 40622,
-    "k_sw_b21": 40492, "k_sw_b22": 11373, "k_sw_b23": 40636, "k_sw_b24": 33460, "k_sw_b25": 26281,
-    "k_sw_b26": 25288, "k_sw_b27": 33465, "k_sw_b28": 26261, "k_sw_b29": 48027,
+    "k_lw_b01": 18810, "k_lw_b02": 21082, "k_lw_b03": 27694, "k_lw_b04": 24335, "k_lw_b05": 27629,
+    "k_lw_b06": 15479, "k_lw_b07": 21765, "k_lw_b08": 15993, "k_lw_b09": 21276, "k_lw_b10": 13113,
+    "k_lw_b11": 15546, "k_lw_b12": 15549, "k_lw_b13": 10506, "k_lw_b14": 7274, "k_lw_b15": 7757,
+    "k_lw_b16": 7272, "k_sw_b16": 25803, "k_sw_b17": 48094, "k_sw_b18": 33012, "k_sw_b19": 33008,
+    "k_sw_b20": 40600, "k_sw_b21": 40484, "k_sw_b22": 11371, "k_sw_b23": 40595, "k_sw_b24": 33465,
+    "k_sw_b25": 26281, "k_sw_b26": 25289, "k_sw_b27": 33459, "k_sw_b28": 26265, "k_sw_b29": 48029,
 }
 
 
