@@ -88,18 +88,23 @@ def test_mixed_nam_route_within_tolerance(rad):
         assert e[k][0] <= FLUX_RMS_WM2, (k, e[k])
 
 
-@pytest.mark.parametrize("precision,env", [("mixed", "WR_LW_LDS"), ("fp64", "WR_LW_LDS64")])
-def test_lds_staged_lw_kernels_match_global_tables(gpu_device, monkeypatch, precision, env):
-    """The LW kernels with the band tables staged in LDS (k_lwl for FP32, k_lwl64 for FP64) do the
-    arithmetic of the kernels that gather from global memory (k_lwc, k_lw): every output is
-    bit-identical, on grid columns with several 64-column blocks per wave and a partly filled last
-    block."""
+@pytest.mark.parametrize("precision,envs", [("mixed", ("WR_LW_LDS", "WR_SW_LDS")),
+                                            ("fp64", ("WR_LW_LDS64", "WR_SW_LDS64"))])
+def test_lds_staged_band_kernels_match_global_tables(gpu_device, monkeypatch, precision, envs):
+    """The band kernels with the band tables staged in LDS (k_lwl / k_swl for FP32, k_lwl64 / k_swl
+    for FP64) do the arithmetic of the kernels that gather from global memory (k_lwc / k_sw, k_lw /
+    k_sw), on grid columns with several 64-column blocks per wave and a partly filled last block.
+    FP64 (no contraction) and the FP32 LW outputs are bit-identical.  The FP32 SW kernels are
+    compiled with multiply-add contraction, which the compiler may form differently in the two
+    instances: their outputs agree to FP32 rounding (fluxes within 1e-6 of the field's largest value,
+    heating rates within 1e-4 K/day, 1/500 of the mixed path's bar)."""
     cols = gen.grid(150, 131, seed=5)  # 19,650 columns
-    print(f"case LDS-staged LW kernels vs global tables, {precision}, grid 150x131", flush=True)
+    print(f"case LDS-staged band kernels vs global tables, {precision}, grid 150x131", flush=True)
     dev = {k: torch.from_numpy(v).to(gpu_device) for k, v in cols.items()}
     outs = {}
     for mask in ("0", "ffff"):
-        monkeypatch.setenv(env, mask)
+        for env in envs:
+            monkeypatch.setenv(env, mask)
         r = Radiation(0, max_cols=20000)
         try:
             out, _ = r.run(dev, precision=precision)
@@ -108,5 +113,10 @@ def test_lds_staged_lw_kernels_match_global_tables(gpu_device, monkeypatch, prec
         finally:
             r.close()
     for k, v in outs["0"].items():
-        assert np.array_equal(v.view(np.uint64) if v.dtype == np.float64 else v, outs["ffff"][k].view(np.uint64)
-                              if v.dtype == np.float64 else outs["ffff"][k]), k
+        w = outs["ffff"][k]
+        if precision == "mixed" and k.startswith("hr_sw"):
+            assert float(np.abs(v - w).max()) * KDAY <= 1e-4, k
+        elif precision == "mixed" and "sw" in k:
+            assert float(np.abs(v - w).max()) <= 1e-6 * max(float(np.abs(v).max()), 1e-30), k
+        else:
+            assert np.array_equal(v.view(np.uint64), w.view(np.uint64)), k

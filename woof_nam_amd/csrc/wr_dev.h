@@ -99,10 +99,22 @@ template <typename R> struct LwImg {
     int n;
     int off[LW_IMG_T];
 };
-template <typename R> struct SwBand {
-    const WR_G R *absa, *absb, *selfref, *forref;
-    const WR_G R *rayl_v, *rayla, *raylb, *abso3a, *abso3b, *absch4, *absh2o, *absco2;
+// The tables of one SW band (pointer types as in LwBandT) and their packed image: 12 tables (absa
+// absb selfref forref rayl_v rayla raylb abso3a abso3b absch4 absh2o absco2), the FP64 image again
+// without absb.
+template <typename R, typename P = const WR_G R *, typename PB = P> struct SwBandT {
+    P absa;
+    PB absb;
+    P selfref, forref;
+    P rayl_v, rayla, raylb, abso3a, abso3b, absch4, absh2o, absco2;
     R rayl, strrat;
+};
+template <typename R> using SwBand = SwBandT<R>;
+constexpr int SW_IMG_T = 12;
+template <typename R> struct SwImg {
+    const WR_G R *base;
+    int n;
+    int off[SW_IMG_T];
 };
 template <typename R> struct Tabs {
     // index arithmetic stays FP64 in every precision
@@ -117,6 +129,7 @@ template <typename R> struct Tabs {
     LwBand<R> lw[16];
     SwBand<R> sw[14];
     LwImg<R> lw_img[16];
+    SwImg<R> sw_img[14];
 };
 
 // ---- column inputs ---------------------------------------------------------------------------

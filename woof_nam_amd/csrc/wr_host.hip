@@ -312,6 +312,26 @@ template <typename R> int bind_tables(const TFile &f, Blob &b, Tabs<R> &T)
     // strrat(16)/(28) come with their band files (kgb16/kgb28).  Every strrat is the single-precision
     // value of the reference (radsw_datatb_nmmb.f:1108-1110 data; phys_rrtm_sw_taumol18.py:42
     // np.float32(3.89589e+1), phys_rrtm_sw_taumol19.cu:80 5.49281e+0f, ...), as the table holds it.
+    // every SW band's tables packed into one image (k_swl copies it to LDS), in the SwImg order; the
+    // FP64 image leaves absb out, a scalar rayl is not a table
+    for (int k = 0; k < 14; ++k) {
+        static const char *const keys[SW_IMG_T] = {"absa",   "absb",   "selfref", "forref", "rayl",   "rayla",
+                                                   "raylb",  "abso3a", "abso3b",  "absch4", "absh2o", "absco2"};
+        const TEntry *ent[SW_IMG_T] = {};
+        for (int j = 0; j < SW_IMG_T; ++j) {
+            if (j == 1 && sizeof(R) != 4) continue;
+            const TEntry *e = d.find(bname("sw", k + 16, keys[j], false));
+            if (e && (j != 4 || e->count > 1)) ent[j] = e;
+        }
+        SwImg<R> &im = T.sw_img[k];
+        int n = 0;
+        for (int j = 0; j < SW_IMG_T; ++j) {
+            im.off[j] = ent[j] ? n : -1;
+            if (ent[j]) n += (int)((ent[j]->count + 3) / 4 * 4);
+        }
+        im.n = n;
+        im.base = b.pack<R>(ent, SW_IMG_T);
+    }
     bool ok16 = true, ok28 = true;
     const double s16 = d.scalar("sw/b16/strrat16", &ok16), s28 = d.scalar("sw/b28/strrat28", &ok28);
     if (ok16) T.sw[0].strrat = (R)s16;
@@ -484,13 +504,19 @@ bool lw_glane_env()
 // are the measured choice per band on the 954x835x60 grid: every FP32 band (LW sum 101 -> 97 ms per
 // call), every FP64 band but 14 (LW sum 233 -> 216 ms; band 14 ran 7 % slower).
 constexpr uint32_t LW_LDS_DEFAULT = 0xffffu, LW_LDS64_DEFAULT = 0xffffu & ~(1u << 13);
-template <typename R> uint32_t lds_mask(const char *env, uint32_t dflt, const Tabs<R> &t, int lds_max)
+// WR_SW_LDS / WR_SW_LDS64 (bit B = band B + 16) choose k_swl over k_sw the same way.  Measured per
+// band (serialised, 954x835x60): FP64 SW bands 17-24 and 29 run 3-9 % faster from LDS, 25, 26, 28
+// slower (the FP64 call 320 -> 312 ms with this mask); in FP32 the SW sum went 62.0 -> 65.4 ms, so
+// FP32 SW keeps its global-table kernels.
+constexpr uint32_t SW_LDS_DEFAULT = 0x0u, SW_LDS64_DEFAULT = 0x21feu;
+template <typename R, typename I>
+uint32_t lds_mask(const char *env, uint32_t dflt, const I *img, int nb, int lds_max)
 {
     const char *e = getenv(env);
     uint32_t m = e ? (uint32_t)strtoul(e, nullptr, 16) : dflt;
-    for (int b = 0; b < 16; ++b)
-        if (t.lw_img[b].n <= 0 || (size_t)t.lw_img[b].n * sizeof(R) > (size_t)lds_max) m &= ~(1u << b);
-    return m & 0xffffu;
+    for (int b = 0; b < nb; ++b)
+        if (img[b].n <= 0 || (size_t)img[b].n * sizeof(R) > (size_t)lds_max) m &= ~(1u << b);
+    return m & ((1u << nb) - 1u);
 }
 
 // Device bytes of a context's tile work space for tiles of m columns, and the band-stream count.
@@ -631,8 +657,10 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
         if (hipDeviceGetAttribute(&c->n_cu, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess ||
             hipDeviceGetAttribute(&c->lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, device) != hipSuccess)
             rc = WR_ERR_HIP;
-        c->lw_lds = lds_mask<float>("WR_LW_LDS", LW_LDS_DEFAULT, c->host_f, c->lds_max);
-        c->lw_lds64 = lds_mask<double>("WR_LW_LDS64", LW_LDS64_DEFAULT, c->host_d, c->lds_max);
+        c->lw_lds = lds_mask<float>("WR_LW_LDS", LW_LDS_DEFAULT, c->host_f.lw_img, 16, c->lds_max);
+        c->lw_lds64 = lds_mask<double>("WR_LW_LDS64", LW_LDS64_DEFAULT, c->host_d.lw_img, 16, c->lds_max);
+        c->sw_lds = lds_mask<float>("WR_SW_LDS", SW_LDS_DEFAULT, c->host_f.sw_img, 14, c->lds_max);
+        c->sw_lds64 = lds_mask<double>("WR_SW_LDS64", SW_LDS64_DEFAULT, c->host_d.sw_img, 14, c->lds_max);
     }
     if (rc == WR_OK) {
         c->pool_bytes = z.pool;

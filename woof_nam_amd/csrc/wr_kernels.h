@@ -15,6 +15,8 @@
 //               and level fluxes)
 //   k_post      FP64 level fluxes -> heating rates, top-first outputs
 #pragma once
+#include <type_traits>
+
 #include "wr_lw.h"
 #include "wr_sw.h"
 
@@ -911,8 +913,8 @@ __device__ __forceinline__ void layer_rt_eval(const Tabs<R> *__restrict__ T, R t
     }
 }
 
-template <typename R, int B, bool SURR = false>
-__device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
+template <typename R, int B, bool SURR, typename BT>
+__device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const BT &Bt, const RadCol &rc, const RunPar &par, int k,
                                             int lt, double cf1, const uint64_t *mb, R cz, R snt, SwLayerRT<R, B> &o,
                                             const Surr &sg, size_t s, int table, double la)
 {
@@ -932,7 +934,7 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
         }
     } else {
         sw_layer<R>(T, rc, par, kin, la, L);
-        sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+        sw_optics_t<R, B>(T, Bt, L, k + 1, lt, tg, tr);
     }
     double ta, sa, ya;
     rc.aer_sw3(B, kin, ta, sa, ya);
@@ -947,8 +949,8 @@ __device__ __forceinline__ void sw_layer_rt(const Tabs<R> *__restrict__ T, const
 
 // One day column of one SW band (spcvrtc, McICA): bottom-up adding pass (reflectances of each level
 // to the lane cache), then the top-down pass with the layer R/T recomputed and the level fluxes.
-template <typename R, int B, bool SURR>
-__device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
+template <typename R, int B, bool SURR, typename BT>
+__device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const BT &Bt, const ColIn &in, const RunPar &par, int col0,
                                           int n, const ColScal &cs, const R *__restrict__ sflux, const AccT<R> &acc,
                                           const Surr &sg, int t, const LaneCache<R> &cc)
 {
@@ -986,7 +988,7 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
     }
     for (int k = 0; k < nrl; ++k) {
         SwLayerRT<R, B> o;
-        sw_layer_rt<R, B, SURR>(T, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table,
+        sw_layer_rt<R, B, SURR>(T, Bt, rc, par, k, lt, cf1, mb, cz, snt, o, sg, (size_t)k * n + t, table,
                                 cs.la[(size_t)(nrl - 1 - k) * n + t]);
 #pragma unroll
         for (int g = 0; g < NG; ++g) {
@@ -1030,7 +1032,7 @@ __device__ __forceinline__ void sw_column(const Tabs<R> *__restrict__ T, const C
                 o.td[g] = cc.at(5, NG, g, s - 1), o.ld[g] = cc.at(6, NG, g, s - 1);
             }
         } else {
-            sw_layer_rt<R, B, SURR>(T, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table,
+            sw_layer_rt<R, B, SURR>(T, Bt, rc, par, s - 1, lt, cf1, mb, cz, snt, o, sg, (size_t)(s - 1) * n + t, table,
                                     cs.la[(size_t)(nrl - s) * n + t]);
         }
         su = (R)0, sd = (R)0;
@@ -1092,7 +1094,53 @@ __global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__
     const int nday = *cs.nday;
     for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
         const int tt = blk * 64 + threadIdx.x;
-        if (tt < nday) sw_column<R, B, SURR>(T, in, par, col0, n, cs, sflux, acc, sg, cs.dlist[tt], cc);
+        if (tt < nday) sw_column<R, B, SURR>(T, T->sw[B], in, par, col0, n, cs, sflux, acc, sg, cs.dlist[tt], cc);
+    }
+}
+
+// k_sw with the band's tables in LDS (absb in global memory in FP64), as k_lwl / k_lwl64 do for LW:
+// persistent workgroups of 4 x sw_waves waves, as many as the CUs hold at once; each copies the band's
+// packed table image (T->sw_img[B]) into LDS once and each wave then runs blocks of 64 day-list
+// entries with its own pool slot.  Same arithmetic as k_sw.
+template <typename R, int B> constexpr int swl_wpg() { return 4 * sw_waves<R, B>(); }
+template <typename R, int B, bool SURR = false, bool EXT = true>
+__global__ void __launch_bounds__((64 * swl_wpg<R, B>()), (sw_waves<R, B>()))
+    k_swl(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0, int n, ColScal cs, const R *__restrict__ sflux,
+          AccT<R> acc, Surr sg, char *__restrict__ pool)
+{
+    extern __shared__ float4 swl_tab[];
+    const ColIn in = band_in<EXT>(in_);
+    const SwImg<R> &im = T->sw_img[B];
+    {
+        const WR_G float4 *src = (const WR_G float4 *)im.base;
+        const int n4 = (int)((size_t)im.n * sizeof(R) / 16);
+        for (int q = threadIdx.x; q < n4; q += blockDim.x) swl_tab[q] = src[q];
+        __syncthreads();
+    }
+    const WR_L R *lt = (const WR_L R *)swl_tab;
+    const SwBand<R> &G = T->sw[B];
+    // a table the band does not have points at the image start and is never read; offsets are
+    // multiples of 4 elements
+    auto at = [&](int j) { return lt + (imax(0, im.off[j]) & ~3); };
+    using PB = typename std::conditional<sizeof(R) == 4, const WR_L R *, const WR_G R *>::type;
+    SwBandT<R, const WR_L R *, PB> Bl;
+    Bl.absa = at(0);
+    if constexpr (sizeof(R) == 4)
+        Bl.absb = at(1);
+    else
+        Bl.absb = G.absb;
+    Bl.selfref = at(2), Bl.forref = at(3), Bl.rayl_v = at(4), Bl.rayla = at(5), Bl.raylb = at(6);
+    Bl.abso3a = at(7), Bl.abso3b = at(8), Bl.absch4 = at(9), Bl.absh2o = at(10), Bl.absco2 = at(11);
+    Bl.rayl = G.rayl, Bl.strrat = G.strrat;
+    constexpr int WPG = swl_wpg<R, B>();
+    const int lane = threadIdx.x % 64, slot = blockIdx.x * WPG + threadIdx.x / 64;
+    LaneCache<R> cc;
+    cc.p = (WR_G R *)(pool + (size_t)slot * cache_slot_bytes(sw_nv_mc<R>(), SW_NG[B], in.nlay + 2, (int)sizeof(R))) + lane;
+    cc.nv = sw_nv_mc<R>();
+    const int nday = *cs.nday;
+    for (int blk = slot; blk * 64 < nday; blk += gridDim.x * WPG) {
+        const int tt = blk * 64 + lane;
+        if (tt < nday) sw_column<R, B, SURR>(T, Bl, in, par, col0, n, cs, sflux, acc, sg, cs.dlist[tt], cc);
     }
 }
 

@@ -109,14 +109,14 @@ __device__ __forceinline__ void ind8(int id0, int id1, int js, int nrow, int d1,
     id[6] = imin(nrow, id[4] + d1);
     id[7] = imin(nrow, id[4] + d2);
 }
-template <typename R> __device__ __forceinline__ R sum8(const WR_G R *__restrict__ A, int ng, int g, const int *id, const R *fc)
+template <typename R, typename P> __device__ __forceinline__ R sum8(P __restrict__ A, int ng, int g, const int *id, const R *fc)
 {
     return fc[0] * S2(A, id[0], g, ng) + fc[1] * S2(A, id[1], g, ng) + fc[2] * S2(A, id[2], g, ng) +
            fc[3] * S2(A, id[3], g, ng) + fc[4] * S2(A, id[4], g, ng) + fc[5] * S2(A, id[5], g, ng) +
            fc[6] * S2(A, id[6], g, ng) + fc[7] * S2(A, id[7], g, ng);
 }
-template <typename R>
-__device__ __forceinline__ R abs4(const WR_G R *__restrict__ A, int ng, int g, int id0, int id1, int nrow, R f00, R f01, R f10, R f11)
+template <typename R, typename P>
+__device__ __forceinline__ R abs4(P __restrict__ A, int ng, int g, int id0, int id1, int nrow, R f00, R f01, R f10, R f11)
 {
     const int i01 = imax(1, imin(nrow, id0 + 1)), i02 = imin(nrow, i01 + 1);
     const int i11 = imax(1, imin(nrow, id1 + 1)), i12 = imin(nrow, i11 + 1);
@@ -129,12 +129,13 @@ template <typename R> __device__ __forceinline__ int jsfs(R specmult, R &fs)
     return 1 + jsi;
 }
 
-// gas (tg) and Rayleigh (tr) optical depths of band B (index 0..13 = bands 16..29), layer kk (1-based)
-template <typename R, int B>
-__device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const SwLay<R> &L, int kk, int Lt, R *tg, R *tr)
+// gas (tg) and Rayleigh (tr) optical depths of band B (index 0..13 = bands 16..29), layer kk (1-based),
+// from the band tables Bt (global or staged in LDS)
+template <typename R, int B, typename BT>
+__device__ __forceinline__ void sw_optics_t(const Tabs<R> *__restrict__ T, const BT &Bt, const SwLay<R> &L, int kk, int Lt,
+                                            R *tg, R *tr)
 {
     constexpr int ng = SW_NG[B];
-    const SwBand<R> &Bt = T->sw[B];
     const int jp = L.jp, jt = L.jt, jt1 = L.jt1;
     const R f00 = L.fac00, f01 = L.fac01, f10 = L.fac10, f11 = L.fac11;
     const int inds = L.indself, indsp = inds + 1, indf = L.indfor, indfp = indf + 1;
@@ -319,6 +320,12 @@ __device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const S
     }
 #undef SELFI
 #undef FORI
+}
+// the same from the band's tables in global memory
+template <typename R, int B>
+__device__ __forceinline__ void sw_optics(const Tabs<R> *__restrict__ T, const SwLay<R> &L, int kk, int Lt, R *tg, R *tr)
+{
+    sw_optics_t<R, B>(T, T->sw[B], L, kk, Lt, tg, tr);
 }
 
 // cloud optical properties of band B for one layer (taucw, ssacw = tau*ssa, asycw = tau*ssa*g)
