@@ -41,12 +41,12 @@ OUT_ALL = (2 * NL + 4 * (NL + 2)) * 8
 # and WRITE_SIZE on a 160,000-column call, HBM bytes = 2 x FETCH + WRITE (calibrated on a 1 GiB
 # device copy); the tables and their commands are profiles/hbm_traffic_{fp64,mixed}_grid400x400.md.
 TRAFFIC_FP64 = {
-    "k_lw_b01": 11691, "k_lw_b02": 11414, "k_lw_b03": 7722, "k_lw_b04": 8571, "k_lw_b05": 8696,
-    "k_lw_b06": 9821, "k_lw_b07": 8445, "k_lw_b08": 11026, "k_lw_b09": 7469, "k_lw_b10": 9097,
-    "k_lw_b11": 9847, "k_lw_b12": 9943, "k_lw_b13": 8006, "k_lw_b14": 7668, "k_lw_b15": 8473, "k_lw_b16": 7479,
-    "k_sw_b16": 21339, "k_sw_b17": 33923, "k_sw_b18": 25290, "k_sw_b19": 25266, "k_sw_b20": 29602,
-    "k_sw_b21": 29627, "k_sw_b22": 12241, "k_sw_b23": 29610, "k_sw_b24": 26232, "k_sw_b25": 21988,
-    "k_sw_b26": 19975, "k_sw_b27": 26218, "k_sw_b28": 22201, "k_sw_b29": 33922,
+    "k_lw_b01": 11692, "k_lw_b02": 11410, "k_lw_b03": 7700, "k_lw_b04": 8552, "k_lw_b05": 8665,
+    "k_lw_b06": 9819, "k_lw_b07": 8433, "k_lw_b08": 11028, "k_lw_b09": 7461, "k_lw_b10": 9091,
+    "k_lw_b11": 9852, "k_lw_b12": 9941, "k_lw_b13": 8001, "k_lw_b14": 7667, "k_lw_b15": 8471, "k_lw_b16": 7477,
+    "k_sw_b16": 21335, "k_sw_b17": 33790, "k_sw_b18": 25222, "k_sw_b19": 25176, "k_sw_b20": 29498,
+    "k_sw_b21": 29547, "k_sw_b22": 12241, "k_sw_b23": 29526, "k_sw_b24": 26168, "k_sw_b25": 21989,
+    "k_sw_b26": 19979, "k_sw_b27": 26218, "k_sw_b28": 22202, "k_sw_b29": 33784,
 }
 
 
