@@ -120,3 +120,26 @@ def test_lds_staged_band_kernels_match_global_tables(gpu_device, monkeypatch, pr
             assert float(np.abs(v - w).max()) <= 1e-6 * max(float(np.abs(v).max()), 1e-30), k
         else:
             assert np.array_equal(v.view(np.uint64), w.view(np.uint64)), k
+
+
+@pytest.mark.parametrize("precision,env", [("mixed", "WR_LWR_LDS"), ("fp64", "WR_LWR_LDS64")])
+def test_lds_staged_nam_route_lw_kernels_match_global_tables(gpu_device, monkeypatch, precision, env):
+    """The NAM route's LW kernels with the band tables in LDS (k_lwrl) against those that gather from
+    global memory (k_lwr): bit-identical outputs in both precisions (the LW units' arithmetic is the
+    same template)."""
+    cols = gen.grid(150, 131, seed=6)
+    print(f"case NAM-route LDS-staged LW kernels vs global tables, {precision}, grid 150x131", flush=True)
+    dev = {k: torch.from_numpy(v).to(gpu_device) for k, v in cols.items()}
+    p = Radiation.default_params(mcica=0)
+    outs = {}
+    for mask in ("0", "ffff"):
+        monkeypatch.setenv(env, mask)
+        r = Radiation(0, max_cols=20000)
+        try:
+            out, _ = r.run(dev, params=p, precision=precision)
+            torch.cuda.synchronize()
+            outs[mask] = {k: v.cpu().numpy().copy() for k, v in out.items()}
+        finally:
+            r.close()
+    for k, v in outs["0"].items():
+        assert np.array_equal(v.view(np.uint64), outs["ffff"][k].view(np.uint64)), k

@@ -219,10 +219,34 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
         if (id != B) return launch_band_nam<R, EXT, B + 1>(j, id, s, pool);
         const AccT<R> acc = band_acc<R>(j.ba, B);
         tbeg(j.c, (B < 16 ? KID_LW0 + B : KID_SW0 + B - 16), s);
-        if constexpr (B < 16)
-            hipLaunchKernelGGL((k_lwr<R, B, EXT>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0, j.n,
-                               j.cs, acc, j.rt);
-        else
+        if constexpr (B < 16) {
+            // k_lwrl: persistent workgroups with the band's tables in LDS, as many as the CUs hold
+            constexpr unsigned WPG = lwrl_wpg<R, B>();
+            const uint32_t on = ((sizeof(R) == 4 ? j.c->lwr_lds : j.c->lwr_lds64) >> B) & 1u;
+            int img_n = 0;
+            if constexpr (sizeof(R) == 4)
+                img_n = j.c->host_f.lw_img[B].n;
+            else
+                img_n = j.c->host_d.lw_img[B].n;
+            const size_t lds = (size_t)img_n * sizeof(R);
+            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            if (per_cu < 0 && on) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwrl<R, B, EXT>, 64 * WPG, lds) !=
+                    hipSuccess)
+                    nb = 0;
+                per_cu = nb;
+            }
+            const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            nwg = nwg < nwg_cols ? nwg : nwg_cols;
+            if (on && nwg > 0)
+                hipLaunchKernelGGL((k_lwrl<R, B, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
+                                   j.col0, j.n, j.cs, acc, j.rt);
+            else
+                hipLaunchKernelGGL((k_lwr<R, B, EXT>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
+                                   j.n, j.cs, acc, j.rt);
+        } else
             hipLaunchKernelGGL((k_swr<R, B - 16, EXT>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
                                j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
         tend(j.c, s);

@@ -193,13 +193,12 @@ template <typename R, int B> constexpr int lwr_chunk()
     return chunk_of(LW_NG[B], WR_LWR_CHUNK >= 0 ? WR_LWR_CHUNK : (sizeof(R) == 8 ? LWR64_CHUNK[B] : LWR32_CHUNK[B]));
 }
 
-template <typename R, int B, bool EXT = true>
-__global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
-                                            int n, ColScal cs, AccT<R> acc, Rtr<R> rt)
+// One column of LW band B on the NAM route (rtrnmr), optics from the band tables Bt.
+template <typename R, int B, typename BT>
+__device__ __forceinline__ void lwr_column(const Tabs<R> *__restrict__ T, const BT &Bt, const ColIn &in, const RunPar &par,
+                                           int col0, int n, const ColScal &cs, const AccT<R> &acc, const Rtr<R> &rt, int t)
 {
-    const ColIn in = band_in<EXT>(in_);
     constexpr int NG = LW_NG[B];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= valid_cols(in, col0, n)) return;
     const int i = col0 + t;
     const RadCol rc{in, i, in.nlay + 1};
@@ -244,7 +243,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, L, tau, fr, 0);
+        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, Bt, L, tau, fr, 0);
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnkd = planck<R>(T, B, tz_bot) - blay;
         const R taucld = lw_taucld_l<R>(T, B, lcf1, L, rc, kin, par);
@@ -315,7 +314,7 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         LwLay<R> L;
         lw_layer<R>(T, cur, par, la_c, L);
         R tau[NG], fr[NG];
-        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, L, tau, fr, 0);
+        lw_optics_ch<R, B, NG, lwr_chunk<R, B>()>(T, Bt, L, tau, fr, 0);
         const R blay = planck<R>(T, B, L.tavel);
         const R dplnku = planck<R>(T, B, tz_top) - blay;
         if (kk == 1) {  // surface emission and reflection
@@ -368,6 +367,40 @@ __global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *
         acc.lw_up[(size_t)kk * n + t] = s;
         tz_bot = tz_top;
         cur = nxt, nxt = nn;
+    }
+}
+
+template <typename R, int B, bool EXT = true>
+__global__ void __launch_bounds__(64, (lwr_waves<R, B>())) k_lwr(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0,
+                                            int n, ColScal cs, AccT<R> acc, Rtr<R> rt)
+{
+    const ColIn in = band_in<EXT>(in_);
+    lwr_column<R, B>(T, T->lw[B], in, par, col0, n, cs, acc, rt, blockIdx.x * blockDim.x + threadIdx.x);
+}
+
+// k_lwr with the band's tables in LDS (the image of k_lwl / k_lwl64: FP64 keeps absb in global
+// memory): persistent workgroups of 4 x lwr_waves waves, as many as the CUs hold at once, each
+// copying the image into LDS once.  Same arithmetic as k_lwr.
+template <typename R, int B> constexpr int lwrl_wpg() { return 4 * lwr_waves<R, B>(); }
+template <typename R, int B, bool EXT = true>
+__global__ void __launch_bounds__((64 * lwrl_wpg<R, B>()), (lwr_waves<R, B>()))
+    k_lwrl(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0, int n, ColScal cs, AccT<R> acc, Rtr<R> rt)
+{
+    extern __shared__ float4 lwrl_tab[];
+    const ColIn in = band_in<EXT>(in_);
+    const LwImg<R> &im = T->lw_img[B];
+    lds_fill<R>(lwrl_tab, im);
+    const WR_L R *lt = (const WR_L R *)lwrl_tab;
+    constexpr int WPG = lwrl_wpg<R, B>();
+    const int lane = threadIdx.x % 64;
+    if constexpr (sizeof(R) == 4) {
+        const auto Bl = lds_band<R>(lt, im, T->lw[B], lt + (imax(0, im.off[1]) & ~3));
+        for (int blk = blockIdx.x * WPG + threadIdx.x / 64; blk * 64 < n; blk += gridDim.x * WPG)
+            lwr_column<R, B>(T, Bl, in, par, col0, n, cs, acc, rt, blk * 64 + lane);
+    } else {
+        const auto Bl = lds_band<R>(lt, im, T->lw[B], T->lw[B].absb);
+        for (int blk = blockIdx.x * WPG + threadIdx.x / 64; blk * 64 < n; blk += gridDim.x * WPG)
+            lwr_column<R, B>(T, Bl, in, par, col0, n, cs, acc, rt, blk * 64 + lane);
     }
 }
 
