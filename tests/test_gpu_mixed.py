@@ -122,18 +122,18 @@ def test_lds_staged_band_kernels_match_global_tables(gpu_device, monkeypatch, pr
             assert np.array_equal(v.view(np.uint64), w.view(np.uint64)), k
 
 
-@pytest.mark.parametrize("precision,env", [("mixed", "WR_LWR_LDS"), ("fp64", "WR_LWR_LDS64")])
-def test_lds_staged_nam_route_lw_kernels_match_global_tables(gpu_device, monkeypatch, precision, env):
-    """The NAM route's LW kernels with the band tables in LDS (k_lwrl) against those that gather from
-    global memory (k_lwr): bit-identical outputs in both precisions (the LW units' arithmetic is the
-    same template)."""
+@pytest.mark.parametrize("precision,envs", [("mixed", ("WR_LWR_LDS",)), ("fp64", ("WR_LWR_LDS64", "WR_SWR_LDS64"))])
+def test_lds_staged_nam_route_kernels_match_global_tables(gpu_device, monkeypatch, precision, envs):
+    """The NAM route's kernels with the band tables in LDS (k_lwrl, and k_swrl in FP64) against those
+    that gather from global memory (k_lwr, k_swr): bit-identical outputs."""
     cols = gen.grid(150, 131, seed=6)
     print(f"case NAM-route LDS-staged LW kernels vs global tables, {precision}, grid 150x131", flush=True)
     dev = {k: torch.from_numpy(v).to(gpu_device) for k, v in cols.items()}
     p = Radiation.default_params(mcica=0)
     outs = {}
     for mask in ("0", "ffff"):
-        monkeypatch.setenv(env, mask)
+        for env in envs:
+            monkeypatch.setenv(env, mask)
         r = Radiation(0, max_cols=20000)
         try:
             out, _ = r.run(dev, params=p, precision=precision)

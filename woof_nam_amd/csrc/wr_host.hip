@@ -511,6 +511,8 @@ constexpr uint32_t LW_LDS_DEFAULT = 0xffffu, LW_LDS64_DEFAULT = 0xffffu & ~(1u <
 constexpr uint32_t SW_LDS_DEFAULT = 0x0u, SW_LDS64_DEFAULT = 0x21feu;
 // WR_LWR_LDS / WR_LWR_LDS64: the NAM route's LW kernels (k_lwrl over k_lwr), on the LW images.
 constexpr uint32_t LWR_LDS_DEFAULT = 0xffffu, LWR_LDS64_DEFAULT = 0xffffu;
+// WR_SWR_LDS / WR_SWR_LDS64: the NAM route's SW kernels (k_swrl over k_swr), on the SW images.
+constexpr uint32_t SWR_LDS_DEFAULT = 0x0u, SWR_LDS64_DEFAULT = 0x3fffu;
 template <typename R, typename I>
 uint32_t lds_mask(const char *env, uint32_t dflt, const I *img, int nb, int lds_max)
 {
@@ -664,6 +666,8 @@ int wr_create(wr_ctx **out, const char *table_path, int device, int32_t max_cols
         c->sw_lds = lds_mask<float>("WR_SW_LDS", SW_LDS_DEFAULT, c->host_f.sw_img, 14, c->lds_max);
         c->lwr_lds = lds_mask<float>("WR_LWR_LDS", LWR_LDS_DEFAULT, c->host_f.lw_img, 16, c->lds_max);
         c->lwr_lds64 = lds_mask<double>("WR_LWR_LDS64", LWR_LDS64_DEFAULT, c->host_d.lw_img, 16, c->lds_max);
+        c->swr_lds = lds_mask<float>("WR_SWR_LDS", SWR_LDS_DEFAULT, c->host_f.sw_img, 14, c->lds_max);
+        c->swr_lds64 = lds_mask<double>("WR_SWR_LDS64", SWR_LDS64_DEFAULT, c->host_d.sw_img, 14, c->lds_max);
         c->sw_lds64 = lds_mask<double>("WR_SW_LDS64", SW_LDS64_DEFAULT, c->host_d.sw_img, 14, c->lds_max);
     }
     if (rc == WR_OK) {

@@ -32,6 +32,7 @@ struct wr_ctx {
     uint32_t lw_lds64;               // the same for the FP64 McICA kernels
     uint32_t sw_lds, sw_lds64;       // bit B: the McICA SW kernel of band B + 16 stages its tables in LDS
     uint32_t lwr_lds, lwr_lds64;     // bit B: the NAM-route LW kernel of band B + 1 stages its tables in LDS
+    uint32_t swr_lds, swr_lds64;     // bit B: the NAM-route SW kernel of band B + 16 stages its tables in LDS
     int n_cu, lds_max;               // compute units; LDS bytes per workgroup
     // learned gas optics (wr_surrogate_attach): network weights and tile work space
     int has_sur;

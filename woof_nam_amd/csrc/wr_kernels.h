@@ -691,7 +691,7 @@ __device__ __forceinline__ LwBandT<R, const WR_L R *, PB> lds_band(const WR_L R 
     return Bl;
 }
 // copies the image into the workgroup's LDS (16-byte words) and waits for the whole workgroup
-template <typename R> __device__ __forceinline__ void lds_fill(float4 *dst, const LwImg<R> &im)
+template <typename R, typename I> __device__ __forceinline__ void lds_fill(float4 *dst, const I &im)
 {
     const WR_G float4 *src = (const WR_G float4 *)im.base;
     const int n4 = (int)((size_t)im.n * sizeof(R) / 16);
@@ -1098,6 +1098,26 @@ __global__ void __launch_bounds__(64, (sw_waves<R, B>())) k_sw(const Tabs<R> *__
     }
 }
 
+// The SW band tables of an LDS image (the SwImg order); FP64 keeps absb in global memory.  A table the
+// band does not have points at the image start and is never read; offsets are multiples of 4 elements.
+template <typename R>
+__device__ __forceinline__ SwBandT<R, const WR_L R *,
+                                   typename std::conditional<sizeof(R) == 4, const WR_L R *, const WR_G R *>::type>
+sw_lds_band(const WR_L R *lt, const SwImg<R> &im, const SwBand<R> &G)
+{
+    SwBandT<R, const WR_L R *, typename std::conditional<sizeof(R) == 4, const WR_L R *, const WR_G R *>::type> Bl;
+    auto at = [&](int j) { return lt + (imax(0, im.off[j]) & ~3); };
+    Bl.absa = at(0);
+    if constexpr (sizeof(R) == 4)
+        Bl.absb = at(1);
+    else
+        Bl.absb = G.absb;
+    Bl.selfref = at(2), Bl.forref = at(3), Bl.rayl_v = at(4), Bl.rayla = at(5), Bl.raylb = at(6);
+    Bl.abso3a = at(7), Bl.abso3b = at(8), Bl.absch4 = at(9), Bl.absh2o = at(10), Bl.absco2 = at(11);
+    Bl.rayl = G.rayl, Bl.strrat = G.strrat;
+    return Bl;
+}
+
 // k_sw with the band's tables in LDS (absb in global memory in FP64), as k_lwl / k_lwl64 do for LW:
 // persistent workgroups of 4 x sw_waves waves, as many as the CUs hold at once; each copies the band's
 // packed table image (T->sw_img[B]) into LDS once and each wave then runs blocks of 64 day-list
@@ -1110,28 +1130,8 @@ __global__ void __launch_bounds__((64 * swl_wpg<R, B>()), (sw_waves<R, B>()))
 {
     extern __shared__ float4 swl_tab[];
     const ColIn in = band_in<EXT>(in_);
-    const SwImg<R> &im = T->sw_img[B];
-    {
-        const WR_G float4 *src = (const WR_G float4 *)im.base;
-        const int n4 = (int)((size_t)im.n * sizeof(R) / 16);
-        for (int q = threadIdx.x; q < n4; q += blockDim.x) swl_tab[q] = src[q];
-        __syncthreads();
-    }
-    const WR_L R *lt = (const WR_L R *)swl_tab;
-    const SwBand<R> &G = T->sw[B];
-    // a table the band does not have points at the image start and is never read; offsets are
-    // multiples of 4 elements
-    auto at = [&](int j) { return lt + (imax(0, im.off[j]) & ~3); };
-    using PB = typename std::conditional<sizeof(R) == 4, const WR_L R *, const WR_G R *>::type;
-    SwBandT<R, const WR_L R *, PB> Bl;
-    Bl.absa = at(0);
-    if constexpr (sizeof(R) == 4)
-        Bl.absb = at(1);
-    else
-        Bl.absb = G.absb;
-    Bl.selfref = at(2), Bl.forref = at(3), Bl.rayl_v = at(4), Bl.rayla = at(5), Bl.raylb = at(6);
-    Bl.abso3a = at(7), Bl.abso3b = at(8), Bl.absch4 = at(9), Bl.absh2o = at(10), Bl.absco2 = at(11);
-    Bl.rayl = G.rayl, Bl.strrat = G.strrat;
+    lds_fill<R>(swl_tab, T->sw_img[B]);
+    const auto Bl = sw_lds_band<R>((const WR_L R *)swl_tab, T->sw_img[B], T->sw[B]);
     constexpr int WPG = swl_wpg<R, B>();
     const int lane = threadIdx.x % 64, slot = blockIdx.x * WPG + threadIdx.x / 64;
     LaneCache<R> cc;

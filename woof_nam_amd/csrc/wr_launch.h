@@ -246,9 +246,38 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             else
                 hipLaunchKernelGGL((k_lwr<R, B, EXT>), dim3((j.n + 63) / 64), dim3(64), 0, s, j.tabs, j.in, j.par, j.col0,
                                    j.n, j.cs, acc, j.rt);
-        } else
-            hipLaunchKernelGGL((k_swr<R, B - 16, EXT>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[B - 16])), dim3(64), 0, s, j.tabs,
-                               j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
+        } else {
+            // k_swrl: persistent workgroups with the band's tables in LDS, within the pool's slots
+            constexpr int S = B < 16 ? 0 : B - 16;
+            constexpr unsigned WPG = swrl_wpg<R, S>();
+            const uint32_t on = ((sizeof(R) == 4 ? j.c->swr_lds : j.c->swr_lds64) >> S) & 1u;
+            int img_n = 0;
+            if constexpr (sizeof(R) == 4)
+                img_n = j.c->host_f.sw_img[S].n;
+            else
+                img_n = j.c->host_d.sw_img[S].n;
+            const size_t lds = (size_t)img_n * sizeof(R);
+            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            if (per_cu < 0 && on) {
+                int nb = 0;
+                if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_swrl<R, S, EXT>, 64 * WPG, lds) !=
+                    hipSuccess)
+                    nb = 0;
+                per_cu = nb;
+            }
+            const size_t slot = cache_slot_bytes(sw_nv_mc<R>(), SW_NG[S], j.in.nlay + 2, (int)sizeof(R));
+            const size_t nwg_pool = j.c->pool_bytes / slot / WPG;
+            const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            nwg = nwg < nwg_pool ? nwg : nwg_pool;
+            nwg = nwg < nwg_cols ? nwg : nwg_cols;
+            if (on && nwg > 0)
+                hipLaunchKernelGGL((k_swrl<R, S, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
+                                   j.col0, j.n, j.cs, j.sflux, acc, pool);
+            else
+                hipLaunchKernelGGL((k_swr<R, S, EXT>), dim3(band_grid(j, sw_nv_mc<R>(), SW_NG[S])), dim3(64), 0, s, j.tabs,
+                                   j.in, j.par, j.col0, j.n, j.cs, j.sflux, acc, pool);
+        }
         tend(j.c, s);
     }
 }

@@ -407,8 +407,8 @@ __global__ void __launch_bounds__((64 * lwrl_wpg<R, B>()), (lwr_waves<R, B>()))
 // ---- SW band kernel of the NAM route (spcvrtc with cldfrc mixing and zcf1/zcf0 weighting) -------
 // R/T of layer k (surface-first): pass 0 clear sky; pass 1 total sky, the cloudy R/T mixed with the
 // clear ones by the layer cloud fraction zc1 = cfrac / zcf1 (phys_rrtm_sw_spcvrtc.cu, cloudy block)
-template <typename R, int B>
-__device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const RadCol &rc, const RunPar &par, int k,
+template <typename R, int B, typename BT>
+__device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const BT &Bt, const RadCol &rc, const RunPar &par, int k,
                                           int lt, int pass, double cf1, R cz, R snt, SwLayerRT<R, B> &o, double la)
 {
     constexpr int NG = SW_NG[B];
@@ -416,7 +416,7 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
     SwLay<R> L;
     sw_layer<R>(T, rc, par, kin, la, L);
     R tg[NG], tr[NG];
-    sw_optics<R, B>(T, L, k + 1, lt, tg, tr);
+    sw_optics_t<R, B>(T, Bt, L, k + 1, lt, tg, tr);
     double ta, sa, ya;
     rc.aer_sw3(B, kin, ta, sa, ya);
 #pragma unroll
@@ -444,8 +444,8 @@ __device__ __forceinline__ void swr_layer(const Tabs<R> *__restrict__ T, const R
 // One day column of one SW band on the NAM route: clear-sky pass, then (cloudy columns) the
 // total-sky pass; each pass is the bottom-up adding sweep (level reflectances to the lane cache) and
 // the top-down sweep with the layer R/T recomputed.
-template <typename R, int B>
-__device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const ColIn &in, const RunPar &par, int col0,
+template <typename R, int B, typename BT>
+__device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const BT &Bt, const ColIn &in, const RunPar &par, int col0,
                                            int n, const ColScal &cs, const R *__restrict__ sflux, const AccT<R> &acc, int t,
                                            const LaneCache<R> &cc)
 {
@@ -479,7 +479,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
         }
         for (int k = 0; k < nrl; ++k) {
             SwLayerRT<R, B> o;
-            swr_layer<R, B>(T, rc, par, k, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - k) * n + t]);
+            swr_layer<R, B>(T, Bt, rc, par, k, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - k) * n + t]);
 #pragma unroll
             for (int g = 0; g < NG; ++g) {
                 const R zden1 = (R)1 / ((R)1 - pd[g] * o.rd[g]);
@@ -520,7 +520,7 @@ __device__ __forceinline__ void swr_column(const Tabs<R> *__restrict__ T, const 
                         o.td[g] = cc.at(5, NG, g, s), o.ld[g] = cc.at(6, NG, g, s);
                     }
                 } else {
-                    swr_layer<R, B>(T, rc, par, s, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - s) * n + t]);
+                    swr_layer<R, B>(T, Bt, rc, par, s, lt, pass, cf1, cz, snt, o, cs.la[(size_t)(nrl - 1 - s) * n + t]);
                 }
                 su = (R)0, sd = (R)0;
 #pragma unroll
@@ -568,7 +568,31 @@ __global__ void __launch_bounds__(64, (sw_waves<R, B, true>())) k_swr(const Tabs
     const int nday = *cs.nday;
     for (int blk = blockIdx.x; blk * 64 < nday; blk += gridDim.x) {
         const int tt = blk * 64 + threadIdx.x;
-        if (tt < nday) swr_column<R, B>(T, in, par, col0, n, cs, sflux, acc, cs.dlist[tt], cc);
+        if (tt < nday) swr_column<R, B>(T, T->sw[B], in, par, col0, n, cs, sflux, acc, cs.dlist[tt], cc);
+    }
+}
+
+// k_swr with the band's tables in LDS (the images of k_swl): persistent workgroups of 4 x the band's
+// waves, as many as the CUs hold at once, each wave with its own pool slot.  Same arithmetic as k_swr.
+template <typename R, int B> constexpr int swrl_wpg() { return 4 * sw_waves<R, B, true>(); }
+template <typename R, int B, bool EXT = true>
+__global__ void __launch_bounds__((64 * swrl_wpg<R, B>()), (sw_waves<R, B, true>()))
+    k_swrl(const Tabs<R> *__restrict__ T, ColIn in_, RunPar par, int col0, int n, ColScal cs, const R *__restrict__ sflux,
+           AccT<R> acc, char *__restrict__ pool)
+{
+    extern __shared__ float4 swrl_tab[];
+    const ColIn in = band_in<EXT>(in_);
+    lds_fill<R>(swrl_tab, T->sw_img[B]);
+    const auto Bl = sw_lds_band<R>((const WR_L R *)swrl_tab, T->sw_img[B], T->sw[B]);
+    constexpr int WPG = swrl_wpg<R, B>();
+    const int lane = threadIdx.x % 64, slot = blockIdx.x * WPG + threadIdx.x / 64;
+    LaneCache<R> cc;
+    cc.p = (WR_G R *)(pool + (size_t)slot * cache_slot_bytes(sw_nv_mc<R>(), SW_NG[B], in.nlay + 2, (int)sizeof(R))) + lane;
+    cc.nv = sw_nv_mc<R>();
+    const int nday = *cs.nday;
+    for (int blk = slot; blk * 64 < nday; blk += gridDim.x * WPG) {
+        const int tt = blk * 64 + lane;
+        if (tt < nday) swr_column<R, B>(T, Bl, in, par, col0, n, cs, sflux, acc, cs.dlist[tt], cc);
     }
 }
 
