@@ -7,6 +7,9 @@
 // that a finishing band kernel leaves idle with the next band's waves.  k_post adds the planes in
 // band order, so the result does not depend on which stream ran which band.
 #pragma once
+#include <algorithm>
+#include <atomic>
+
 #include "wr_internal.h"
 #include "wr_glw.h"
 #include "wr_nam.h"
@@ -130,7 +133,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             // pool has slots for (none fit: k_lwc)
             constexpr unsigned WPG = lwl_wpg<B>();
             const size_t lds = (size_t)j.c->host_f.lw_img[B].n * sizeof(float);
-            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            static std::atomic<int> per_cu{-1};  // workgroups per CU, once per instance (one table set per process)
             if (per_cu < 0 && ((j.c->lw_lds >> B) & 1)) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwl<B, SURR, EXT>, 64 * WPG, lds) !=
@@ -141,7 +144,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             const size_t slot = cache_slot_bytes(LW_NV_F32, LW_NG[B], j.in.nlay + 2, (int)sizeof(R));
             const size_t nwg_pool = j.c->pool_bytes / slot / WPG;
             const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
-            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)std::max(per_cu.load(), 0);
             nwg = nwg < nwg_pool ? nwg : nwg_pool;
             nwg = nwg < nwg_cols ? nwg : nwg_cols;
             if (((j.c->lw_lds >> B) & 1) && nwg > 0) {
@@ -155,7 +158,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             // k_lwl64: persistent workgroups, as many as the CUs hold at once
             constexpr unsigned WPG = lwl64_wpg<R, B>();
             const size_t lds = (size_t)j.c->host_d.lw_img[B].n * sizeof(double);
-            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            static std::atomic<int> per_cu{-1};  // workgroups per CU, once per instance (one table set per process)
             if (per_cu < 0 && ((j.c->lw_lds64 >> B) & 1)) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwl64<B, SURR, EXT>, 64 * WPG, lds) !=
@@ -164,7 +167,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
                 per_cu = nb;
             }
             const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
-            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)std::max(per_cu.load(), 0);
             nwg = nwg < nwg_cols ? nwg : nwg_cols;
             if (((j.c->lw_lds64 >> B) & 1) && nwg > 0)
                 hipLaunchKernelGGL((k_lwl64<B, SURR, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
@@ -186,7 +189,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             else
                 img_n = j.c->host_d.sw_img[S].n;
             const size_t lds = (size_t)img_n * sizeof(R);
-            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            static std::atomic<int> per_cu{-1};  // workgroups per CU, once per instance (one table set per process)
             if (per_cu < 0 && on) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_swl<R, S, SURR, EXT>, 64 * WPG, lds) !=
@@ -197,7 +200,7 @@ void launch_band(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             const size_t slot = cache_slot_bytes(sw_nv_mc<R>(), SW_NG[S], j.in.nlay + 2, (int)sizeof(R));
             const size_t nwg_pool = j.c->pool_bytes / slot / WPG;
             const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
-            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)std::max(per_cu.load(), 0);
             nwg = nwg < nwg_pool ? nwg : nwg_pool;
             nwg = nwg < nwg_cols ? nwg : nwg_cols;
             if (on && nwg > 0)
@@ -229,7 +232,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             else
                 img_n = j.c->host_d.lw_img[B].n;
             const size_t lds = (size_t)img_n * sizeof(R);
-            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            static std::atomic<int> per_cu{-1};  // workgroups per CU, once per instance (one table set per process)
             if (per_cu < 0 && on) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_lwrl<R, B, EXT>, 64 * WPG, lds) !=
@@ -238,7 +241,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
                 per_cu = nb;
             }
             const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
-            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)std::max(per_cu.load(), 0);
             nwg = nwg < nwg_cols ? nwg : nwg_cols;
             if (on && nwg > 0)
                 hipLaunchKernelGGL((k_lwrl<R, B, EXT>), dim3((unsigned)nwg), dim3(64 * WPG), lds, s, j.tabs, j.in, j.par,
@@ -257,7 +260,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             else
                 img_n = j.c->host_d.sw_img[S].n;
             const size_t lds = (size_t)img_n * sizeof(R);
-            static int per_cu = -1;  // workgroups per CU, once per instance (one table set per process)
+            static std::atomic<int> per_cu{-1};  // workgroups per CU, once per instance (one table set per process)
             if (per_cu < 0 && on) {
                 int nb = 0;
                 if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, (const void *)k_swrl<R, S, EXT>, 64 * WPG, lds) !=
@@ -268,7 +271,7 @@ void launch_band_nam(const BandJob<R> &j, int id, hipStream_t s, char *pool)
             const size_t slot = cache_slot_bytes(sw_nv_mc<R>(), SW_NG[S], j.in.nlay + 2, (int)sizeof(R));
             const size_t nwg_pool = j.c->pool_bytes / slot / WPG;
             const size_t nwg_cols = ((size_t)j.n + 64 * WPG - 1) / (64 * WPG);
-            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)(per_cu > 0 ? per_cu : 0);
+            size_t nwg = (size_t)(j.c->n_cu > 0 ? j.c->n_cu : 1) * (size_t)std::max(per_cu.load(), 0);
             nwg = nwg < nwg_pool ? nwg : nwg_pool;
             nwg = nwg < nwg_cols ? nwg : nwg_cols;
             if (on && nwg > 0)
